@@ -1,0 +1,140 @@
+"""Benchmark of the implicit Newton step at n129: ms per Newton iteration and PCG iterations per Newton
+iteration of the quasi-static sphere drape solved by pseudo-transient continuation (129 x 129 grid,
+16641 vertices, 32768 triangles; isotropic E = 1e6 Pa, nu = 0.3, t = 0.5 mm, 0.2 kg/m^2, hinge 1e-5 N m,
+sphere r = 0.15 m, dhat = 1 mm), block-Jacobi FP32 PCG.
+
+A bench step is one accepted PTC step (one inexact Newton solve of the step's energy, several Newton
+iterations). `--warmup W` steps run untimed from the flat start; the next `--steps K` are timed between
+device synchronisations. Prints one JSON line.
+
+The roofline object is for the dominant kernel, the SpMV of PCG (92% of the reference's n129 run):
+algorithmic bytes 40 B per block + 28 B per row + 4 (vals 36 B + column 4 B per block; row pointer 4 B,
+x gather 12 B and y 12 B per row), divided by the kernel's average duration from HIP events on the
+context's stream. The CPU baseline is the oracle (f64 Newton with band Cholesky, as the reference's Rust
+oracle does) timed for one Newton iteration of the same first PTC step on one core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import drape_amd as D  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(rows: int, blocks: int) -> int:
+    return 40 * blocks + 28 * rows + 4
+
+
+def cpu_baseline(columns: int) -> dict:
+    """One f64 Newton iteration (regularised band-Cholesky direction + Armijo, e7_ptc.rs:129-134) of the first
+    PTC step of the same scene, on one core of this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    s = O.Scene()
+    ps = D.drape_config_scene(columns)
+    for name, _ in O.Scene._fields_:
+        setattr(s, name, getattr(ps, name))
+    od = O.Drape(s)
+    x = od.rest_positions()
+    seconds, ok = od.time_newton_iteration(x, 1.0 / 240.0)
+    iterations = 1
+    try:
+        cpu = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = next((l.split(":", 1)[1].strip() for l in cpu.splitlines() if l.startswith("Model name")), "?")
+    except OSError:
+        model = "?"
+    return {"value": 1000.0 * seconds / iterations, "unit": "ms/Newton iteration", "cores": 1, "kind": "port",
+            "sample": f"one f64 Newton iteration (tangent, band-Cholesky direction, Armijo search) of the first "
+                      f"PTC step of the n{columns} drape, {seconds:.1f} s on 1 core ({model})"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--columns", type=int, default=129)
+    ap.add_argument("--preconditioner", choices=["block_jacobi", "two_level"], default="block_jacobi")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
+                                   "not implemented", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    scene = D.drape_config_scene(args.columns)
+    arrays = D.SceneArrays(scene)
+    rules = D.default_rules(preconditioner=0 if args.preconditioner == "block_jacobi" else 1)
+    ctx = D.Drape(arrays, rules)
+    print(f"[bench] n{args.columns}: V {arrays.V} T {arrays.T} H {arrays.H} rows {arrays.R} blocks {arrays.B}",
+          file=sys.stderr, flush=True)
+
+    start = arrays.array("rest_positions")
+    h = rules.h0
+    if args.warmup > 0:
+        w = ctx.solve_equilibrium(start=start, h_start=h, cap=args.warmup)
+        start, h = w["positions"], w["result"].final_h
+        print(f"[bench] warmup: {w['result'].accepted_steps} steps, stop {w['stop']}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    out = ctx.solve_equilibrium(start=start, h_start=h, cap=args.steps)
+    wall = time.perf_counter() - t0
+    res = out["result"]
+    newton = max(res.newton_iterations, 1)
+    ms_per_newton = 1000.0 * res.seconds / newton
+    pcg_per_newton = res.pcg_iterations / newton
+    print(f"[bench] timed: {res.accepted_steps} steps, {res.newton_iterations} Newton, {res.pcg_iterations} PCG, "
+          f"{res.seconds:.3f} s, stop {out['stop']}, r/L {res.final_residual / res.gravity_load:.3e}",
+          file=sys.stderr, flush=True)
+
+    # dominant kernel: SpMV of the PCG on the last assembled system, timed with HIP events
+    spmv_ms, _ = ctx.spmv_timing(400)
+    nbytes = spmv_bytes(arrays.R, arrays.B)
+    achieved = nbytes / (spmv_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_pcg_spmv", "avg_kernel_us": round(1000.0 * spmv_ms, 3),
+                "algorithmic_bytes": nbytes, "note": "n129 working set (~10 MB) is cache-resident"}
+    line = {
+        "metric": "ms/Newton step + PCG iters at n129: learned 2-level vs block-Jacobi PCG",
+        "value": round(ms_per_newton, 4),
+        "unit": "ms/Newton iteration",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * res.seconds / max(res.accepted_steps, 1), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"quasi-static sphere drape by PTC, n{args.columns} "
+                               f"({arrays.V} vertices, {arrays.T} triangles), isotropic E=1e6 nu=0.3 t=0.5mm, "
+                               "sphere r=0.15 m, dhat=1 mm",
+                   "preconditioner": args.preconditioner, "pcg_per_newton": round(pcg_per_newton, 2),
+                   "newton_iterations": res.newton_iterations, "pcg_iterations": res.pcg_iterations,
+                   "accepted_steps": res.accepted_steps, "phase_ms": [round(v, 2) for v in res.phase_ms],
+                   "fallbacks": res.fallbacks, "wall_s": round(wall, 3)},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline(args.columns)
+        except Exception as e:  # the baseline is reported, never required
+            line["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+// Internal declarations shared by the HIP kernels and the host side of libdrape_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/drape_hip.h"
+
+namespace drape {
+
+constexpr int WG = 256;          // rows per workgroup and per reduction group (pcg.rs:216 WORKGROUP)
+constexpr int SLICE = 64;        // rows per sliced-BSR slice: one wavefront
+constexpr int CHUNK_PLAIN = 32;  // PCG iterations between readbacks without replacement (solver.ts:145-149)
+
+// Scalar CG state on the device (state.wgsl:4-9), rz double-buffered by iteration parity so that every
+// workgroup of a kernel reads the previous value while workgroup 0 writes the next one.
+struct PcgState {
+  float rz[2];
+  float pq, alpha, beta, rr, rr0, tol2, true_rr, above2;
+  int32_t iteration, status, max_iterations, replace_pending;
+};
+
+// Two-level preconditioner data (coarse space) -- see twolevel.hip.
+struct TwoLevel;
+
+struct Context {
+  // sizes
+  int V = 0, T = 0, H = 0, R = 0, B = 0, K = 0;  // K colliders
+  int G = 0;                                     // reduction groups = ceil(R / 256)
+  int S = 0;                                     // slices = ceil(R / 64)
+  bool linear = false;                           // plain BSR system (no mesh)
+  drape_rules rules{};
+  drape_problem problem{};                       // pointers not kept; sizes and constants only
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_spmv0 = nullptr, ev_spmv1 = nullptr;
+
+  // host copies needed by the Newton loop
+  std::vector<uint32_t> row_vertex;
+  std::vector<float> unit_mass, colliders_host, row_mass_host;
+  std::vector<uint32_t> row_ptr_host, col_idx_host;
+
+  // geometry (device)
+  float4* pos = nullptr;      // [V]
+  float4* xhat = nullptr;     // [V]
+  float4* dir = nullptr;      // [V]
+  int4* tri = nullptr;        // [T] v0 v1 v2 -
+  float4* geom0 = nullptr;    // [T] g0x g0y g1x g1y
+  float4* geom1 = nullptr;    // [T] g2x g2y area -
+  int4* hin = nullptr;        // [H]
+  float* hstiff = nullptr;    // [H]
+  uint32_t* rvert = nullptr;  // [R]
+  float* rmass = nullptr;     // [R] M/h^2
+  float* colliders = nullptr; // [4 + 8K]
+  float* mu = nullptr;        // [1]
+  float* unit_mass_dev = nullptr; // [R] vertex masses
+  float4* org = nullptr;      // [V] origin x_n of the Newton solve / accepted PTC state
+  float* material = nullptr;  // [4] wale course coupling shear
+
+  // element-local outputs (reference layout): grad [9T + 12H + 3R], hess [81T + 144H + 9R]
+  float* lgrad = nullptr;
+  float* lhess = nullptr;
+  // scatter maps sorted by row (e3_generate.rs:111-152 order): per CSR block / per row
+  uint32_t* bc_ptr = nullptr;
+  uint32_t* bc_idx = nullptr;
+  uint32_t* fc_ptr = nullptr;
+  uint32_t* fc_idx = nullptr;
+
+  // sliced BSR (SELL-64): per slice pointer (in block slots of 64 lanes) and width
+  uint32_t* slice_ptr = nullptr;  // [S+1]
+  uint32_t* sell_col = nullptr;   // [slots * 64]
+  float* sell_val = nullptr;      // [slots * 64 * 9]: for slot base q, entry e of lane l at 9q*64... see spmv
+  uint32_t* row_cnt = nullptr;    // [S*64] blocks per row (0 beyond R)
+  uint32_t* diag_k = nullptr;     // [R] position of the diagonal block in the row
+  uint32_t* block_dst = nullptr;  // [B] sell_val base index of CSR block b (entry e at + e*64)
+  uint32_t* csr_row_ptr = nullptr;// [R+1]
+  uint32_t* csr_col = nullptr;    // [B]
+  size_t sell_slots = 0;
+
+  // PCG vectors (float4 per row, w = 0)
+  float4 *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  float* dinv = nullptr;          // [9R]
+  float* partials = nullptr;      // [2 * 3 * G]: pq | (rz, rr) | replace/true
+  PcgState* state = nullptr;
+  PcgState* state_host = nullptr; // pinned
+  double* refine = nullptr;       // FP64 refinement vectors [3R] x2
+
+  // reductions for the Newton loop (FP64)
+  double* dpart = nullptr;        // [2 * max groups]
+  double* dout = nullptr;         // [8]
+  double* dout_host = nullptr;    // pinned
+  float* bounds = nullptr;        // [R]
+  float* dists = nullptr;         // [R]
+  float* fpart = nullptr;         // [4 * groups]
+  int32_t* ipart = nullptr;       // [2 * groups]
+
+  bool curvature_dropped = false;
+  TwoLevel* two_level = nullptr;
+
+  // timing of the last PCG solve
+  double last_spmv_ms = 0, last_solve_ms = 0;
+  int last_spmv_launches = 0;
+  bool time_spmv = false;
+};
+
+// error handling
+void set_error(const std::string& message);
+#define HIP_TRY(expr)                                                                           \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess) {                                                                     \
+      ::drape::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                    \
+      return DRAPE_E_DEVICE;                                                                    \
+    }                                                                                           \
+  } while (0)
+
+// ---- kernel launchers (elements.hip) --------------------------------------------------------
+int launch_evaluate(Context& c, bool exact);        // element + row kernels into lgrad/lhess
+int launch_assemble(Context& c);                    // scatter maps -> sell_val and b
+int launch_rhs_norm(Context& c, double* norm2);     // FP64 |b|^2 (synchronises)
+int launch_slope(Context& c, double* slope);        // FP64 -sum b.x (synchronises)
+int launch_scatter_direction(Context& c, const float4* rows_src);  // dir[vertex(row)] = rows_src[row]
+int launch_advance(Context& c, float alpha);        // pos[v] += alpha * dir[v] on rows
+int launch_energy_change(Context& c, float alpha, double* sum, double* abs_sum);
+int launch_max_step(Context& c, float* alpha, int32_t* row, float* min_distance, int32_t* min_row);
+// out = {max|solution row|, max|pos - org| on rows, max|pos| on rows, non-finite count}
+int launch_row_maxes(Context& c, const float4* org, float out[4]);
+int launch_towards(Context& c);                     // dir = x_hat - pos (all vertices)
+int launch_step_all(Context& c, float alpha);       // pos += alpha dir (all vertices)
+int launch_ptc_prepare(Context& c, const float4* base, double h);  // x_hat, M/h^2 of a PTC attempt
+int spmv_timing(Context& c, int launches, double* avg_ms);
+int launch_set_float(Context& c, float* dst, float v);  // device scalar write by kernel argument
+
+// ---- PCG (pcg.hip) --------------------------------------------------------------------------
+int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats);
+
+// ---- two-level preconditioner (twolevel.hip) ------------------------------------------------
+void two_level_destroy(TwoLevel* t);
+
+// ---- helpers --------------------------------------------------------------------------------
+inline int blocks_for(int n, int per = WG) { return (n + per - 1) / per; }
+
+}  // namespace drape

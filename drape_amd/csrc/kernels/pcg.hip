@@ -1,0 +1,335 @@
+// Block-Jacobi FP32 PCG with residual replacement on a sliced 3x3 BSR matrix (SELL-64).
+//
+// Semantics follow the reference's GPU solver and its CPU emulation (solver.ts:109-191, reduce.wgsl,
+// update_dot.wgsl, residual.wgsl; pcg.rs:60-281): per-row 3-term dot products, fixed 256-row workgroup
+// trees and a strided final pass. Built with -ffp-contract=off, the iterates are bit-identical to the
+// oracle's f32 emulation.
+//
+// MI355X layout: one thread per block row, one wavefront per 64-row slice. A slice stores, for each
+// block position k < its width, 64 column indices and 9 planes of 64 values, so every load of the SpMV
+// is a 256-byte coalesced wavefront access; the gathered p vector is float4 per row (one 16-byte load).
+// The scalar reductions that the reference runs in separate single-workgroup passes are folded into the
+// consumers: every workgroup re-reduces the <= ceil(R/256) partials in the same fixed order, so one
+// iteration is three launches (spmv+dot, update+dots, direction).
+#include "../internal.h"
+#include "reduce.hpp"
+
+namespace drape {
+
+struct SellView {
+  const uint32_t* slice_ptr;
+  const uint32_t* col;
+  const float* val;
+  const uint32_t* cnt;
+};
+
+// acc[r] = acc[r] + (v[3r] p0 + v[3r+1] p1 + v[3r+2] p2) over the row's blocks in column order.
+__device__ inline float3 row_times(const SellView& A, const float4* __restrict__ v4, int row) {
+  const int s = row >> 6, l = row & 63;
+  const uint32_t base = A.slice_ptr[s];
+  const uint32_t n = A.cnt[row];
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t q = base + k;
+    const uint32_t c = A.col[q * 64 + l];
+    const float* v = A.val + (size_t)q * 9 * 64 + l;
+    const float4 pc = v4[c];
+    a0 = a0 + (v[0] * pc.x + v[64] * pc.y + v[128] * pc.z);
+    a1 = a1 + (v[192] * pc.x + v[256] * pc.y + v[320] * pc.z);
+    a2 = a2 + (v[384] * pc.x + v[448] * pc.y + v[512] * pc.z);
+  }
+  return make_float3(a0, a1, a2);
+}
+
+__device__ inline float3 precondition(const float* __restrict__ dinv, int row, float a, float b, float c) {
+  const float* d = dinv + 9 * row;
+  return make_float3(d[0] * a + d[1] * b + d[2] * c, d[3] * a + d[4] * b + d[5] * c, d[6] * a + d[7] * b + d[8] * c);
+}
+
+// pcg.rs:268-281 invert3 (adjugate / determinant), same operation order.
+__device__ inline void invert3(const float m[9], float* out) {
+  float a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7], i = m[8];
+  float c00 = e * i - f * h, c01 = c * h - b * i, c02 = b * f - c * e;
+  float c10 = f * g - d * i, c11 = a * i - c * g, c12 = c * d - a * f;
+  float c20 = d * h - e * g, c21 = b * g - a * h, c22 = a * e - b * d;
+  float det = a * c00 + b * c10 + c * c20;
+  out[0] = c00 / det; out[1] = c01 / det; out[2] = c02 / det;
+  out[3] = c10 / det; out[4] = c11 / det; out[5] = c12 / det;
+  out[6] = c20 / det; out[7] = c21 / det; out[8] = c22 / det;
+}
+
+// jacobi_invert + cg_init: D^-1, x = 0, r = b, z = D^-1 r, p = z, partial (r.z, r.r).
+__global__ void __launch_bounds__(256) k_pcg_setup(SellView A, const uint32_t* __restrict__ diag_k,
+                                                   const float4* __restrict__ b, float* __restrict__ dinv,
+                                                   float4* x, float4* r, float4* z, float4* p,
+                                                   float* __restrict__ partials, int R) {
+  __shared__ float2 lds[256];
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  float rz = 0.0f, rr = 0.0f;
+  if (row < R) {
+    const int s = row >> 6, l = row & 63;
+    const uint32_t q = A.slice_ptr[s] + diag_k[row];
+    float m[9], inv[9];
+    for (int e = 0; e < 9; ++e) m[e] = A.val[(size_t)q * 9 * 64 + e * 64 + l];
+    invert3(m, inv);
+    for (int e = 0; e < 9; ++e) dinv[9 * row + e] = inv[e];
+    const float4 bb = b[row];
+    x[row] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r[row] = bb;
+    const float3 zz = precondition(dinv, row, bb.x, bb.y, bb.z);
+    const float4 z4 = make_float4(zz.x, zz.y, zz.z, 0.0f);
+    z[row] = z4;
+    p[row] = z4;
+    rz = bb.x * zz.x + bb.y * zz.y + bb.z * zz.z;
+    rr = bb.x * bb.x + bb.y * bb.y + bb.z * bb.z;
+  }
+  const float2 sum = tree256x2(rz, rr, lds);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sum.x;
+    partials[2 * blockIdx.x + 1] = sum.y;
+  }
+}
+
+// reduce.wgsl MODE 0 (start).
+__global__ void __launch_bounds__(256) k_pcg_start(const float* __restrict__ partials, int G, PcgState* st,
+                                                   float tol2, float above2, int max_iterations) {
+  __shared__ float2 lds[256];
+  const float2 s = reduce_partials2(partials, G, lds);
+  if (threadIdx.x == 0) {
+    st->rz[0] = s.x;
+    st->rz[1] = 0.0f;
+    st->rr = s.y;
+    st->rr0 = s.y;
+    st->tol2 = tol2;
+    st->above2 = above2;
+    st->max_iterations = max_iterations;
+    st->iteration = 0;
+    st->status = s.y <= 0.0f ? 1 : 0;
+    st->alpha = st->beta = st->pq = 0.0f;
+    st->true_rr = 0.0f;
+  }
+}
+
+// spmv_dot.wgsl: q = A p, partial p.q.
+__global__ void __launch_bounds__(256) k_pcg_spmv(SellView A, const float4* __restrict__ p, float4* __restrict__ q,
+                                                  float* __restrict__ partial_pq, const PcgState* st, int R,
+                                                  int ignore_status) {
+  __shared__ float lds[256];
+  if (!ignore_status && st->status != 0) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  float pq = 0.0f;
+  if (row < R) {
+    const float3 a = row_times(A, p, row);
+    q[row] = make_float4(a.x, a.y, a.z, 0.0f);
+    const float4 pr = p[row];
+    pq = pr.x * a.x + pr.y * a.y + pr.z * a.z;
+  }
+  const float s = tree256(pq, lds);
+  if (threadIdx.x == 0) partial_pq[blockIdx.x] = s;
+}
+
+// reduce.wgsl MODE 1 (alpha, breakdown) folded into update_dot.wgsl: x += alpha p, r -= alpha q,
+// z = D^-1 r, partial (r.z, r.r).
+__global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ partial_pq, int G, PcgState* st,
+                                                    const float4* __restrict__ p, const float4* __restrict__ q,
+                                                    const float* __restrict__ dinv, float4* x, float4* r, float4* z,
+                                                    float* __restrict__ partial_rz, int R, int it) {
+  __shared__ float lds[256];
+  __shared__ float2 lds2[256];
+  if (st->status != 0) return;
+  const float pq = reduce_partials(partial_pq, G, lds);
+  const float alpha = st->rz[it & 1] / pq;
+  if (!(pq > 0.0f && finite32(alpha))) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->status = 3;
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pq = pq;
+    st->alpha = alpha;
+  }
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  float rz = 0.0f, rr = 0.0f;
+  if (row < R) {
+    const float4 pr = p[row], qr = q[row];
+    float4 xr = x[row], rv = r[row];
+    xr.x = xr.x + alpha * pr.x;
+    xr.y = xr.y + alpha * pr.y;
+    xr.z = xr.z + alpha * pr.z;
+    rv.x = rv.x - alpha * qr.x;
+    rv.y = rv.y - alpha * qr.y;
+    rv.z = rv.z - alpha * qr.z;
+    x[row] = xr;
+    r[row] = rv;
+    const float3 zz = precondition(dinv, row, rv.x, rv.y, rv.z);
+    z[row] = make_float4(zz.x, zz.y, zz.z, 0.0f);
+    rz = rv.x * zz.x + rv.y * zz.y + rv.z * zz.z;
+    rr = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
+  }
+  const float2 sum = tree256x2(rz, rr, lds2);
+  if (threadIdx.x == 0) {
+    partial_rz[2 * blockIdx.x] = sum.x;
+    partial_rz[2 * blockIdx.x + 1] = sum.y;
+  }
+}
+
+// reduce.wgsl MODE 2 (beta, convergence, iteration limit) folded into direction.wgsl: p = z + beta p.
+__global__ void __launch_bounds__(256) k_pcg_direction(const float* __restrict__ partial_rz, int G, PcgState* st,
+                                                       const float4* __restrict__ z, float4* p, int R, int it) {
+  __shared__ float2 lds[256];
+  if (st->status != 0) return;
+  const float2 s = reduce_partials2(partial_rz, G, lds);
+  const float rz_new = s.x, rr = s.y;
+  const float beta = rz_new / st->rz[it & 1];
+  int status = 0;
+  if (rr <= st->tol2 * st->rr0) status = 1;
+  else if (it + 1 >= st->max_iterations) status = 2;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz[(it + 1) & 1] = rz_new;
+    st->rr = rr;
+    st->beta = beta;
+    st->iteration = it + 1;
+    if (status != 0) st->status = status;
+  }
+  if (status != 0) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row < R) {
+    const float4 zr = z[row];
+    float4 pr = p[row];
+    pr.x = zr.x + beta * pr.x;
+    pr.y = zr.y + beta * pr.y;
+    pr.z = zr.z + beta * pr.z;
+    p[row] = pr;
+  }
+}
+
+// residual.wgsl: s = b - A x; replace (r = s, z = D^-1 s, partial (r.z, r.r)) or measure (partial s.s).
+__global__ void __launch_bounds__(256) k_pcg_residual(SellView A, const float4* __restrict__ b,
+                                                      const float4* __restrict__ x, const float* __restrict__ dinv,
+                                                      float4* r, float4* z, float* __restrict__ partials,
+                                                      const PcgState* st, int R, int replace) {
+  __shared__ float2 lds[256];
+  // replace only while rr > above2 rr0 (pcg.rs:293-296; a NaN rr does not replace)
+  if (replace && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  float a = 0.0f, c = 0.0f;
+  if (row < R) {
+    const float3 ax = row_times(A, x, row);
+    const float4 bb = b[row];
+    const float s0 = bb.x - ax.x, s1 = bb.y - ax.y, s2 = bb.z - ax.z;
+    const float ss = s0 * s0 + s1 * s1 + s2 * s2;
+    if (replace) {
+      const float3 zz = precondition(dinv, row, s0, s1, s2);
+      r[row] = make_float4(s0, s1, s2, 0.0f);
+      z[row] = make_float4(zz.x, zz.y, zz.z, 0.0f);
+      a = s0 * zz.x + s1 * zz.y + s2 * zz.z;
+      c = ss;
+    } else {
+      a = ss;
+    }
+  }
+  const float2 sum = tree256x2(a, c, lds);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sum.x;
+    partials[2 * blockIdx.x + 1] = sum.y;
+  }
+}
+
+// reduce.wgsl MODE 4 (replaced r.z, r.r) and MODE 3 (true residual).
+__global__ void __launch_bounds__(256) k_pcg_residual_final(const float* __restrict__ partials, int G, PcgState* st,
+                                                            int replace, int it) {
+  __shared__ float2 lds[256];
+  const int status = st->status;
+  const float rr = st->rr, gate = st->above2 * st->rr0;
+  __syncthreads();
+  if (replace && (status != 0 || !(rr > gate))) return;
+  const float2 s = reduce_partials2(partials, G, lds);
+  if (threadIdx.x == 0) {
+    if (replace) {
+      st->rz[(it + 1) & 1] = s.x;
+      st->rr = s.y;
+    } else {
+      st->true_rr = s.x;
+    }
+  }
+}
+
+static SellView view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt}; }
+
+int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
+  const int G = c.G, R = c.R;
+  float* part_setup = c.partials;
+  float* part_pq = c.partials + 2 * G;
+  float* part_rz = c.partials + 3 * G;
+  float* part_res = c.partials + 5 * G;
+  SellView A = view_of(c);
+  const float tol2 = (float)(tolerance * tolerance);
+  const float above2 = (float)(c.rules.replace_above * c.rules.replace_above);
+  hipStream_t s = c.stream;
+  HIP_TRY(hipEventRecord(c.ev0, s));
+  k_pcg_setup<<<G, 256, 0, s>>>(A, c.diag_k, c.b, c.dinv, c.x, c.r, c.z, c.p, part_setup, R);
+  k_pcg_start<<<1, 256, 0, s>>>(part_setup, G, c.state, tol2, above2, max_iterations);
+  HIP_TRY(hipGetLastError());
+  const int replace_every = c.rules.replace_every;
+  const int chunk = replace_every > 0 ? replace_every : CHUNK_PLAIN;
+  const int chunk_limit = (max_iterations + chunk - 1) / chunk + 1;
+  int chunks = 0, it = 0;
+  bool stopped = false;
+  for (chunks = 1; chunks <= chunk_limit; ++chunks) {
+    for (int j = 0; j < chunk; ++j, ++it) {
+      k_pcg_spmv<<<G, 256, 0, s>>>(A, c.p, c.q, part_pq, c.state, R, 0);
+      k_pcg_update<<<G, 256, 0, s>>>(part_pq, G, c.state, c.p, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
+      k_pcg_direction<<<G, 256, 0, s>>>(part_rz, G, c.state, c.z, c.p, R, it);
+    }
+    if (replace_every > 0) {
+      k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 1);
+      k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 1, it - 1);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c.state_host, c.state, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (c.state_host->status != 0) {
+      stopped = true;
+      break;
+    }
+  }
+  if (!stopped) {
+    set_error("PCG never raised a stop status: the iteration counter or reduction is broken");
+    return DRAPE_E_STATE;
+  }
+  k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 0);
+  k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 0, 0);
+  HIP_TRY(hipEventRecord(c.ev1, s));
+  HIP_TRY(hipMemcpyAsync(c.state_host, c.state, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  c.last_solve_ms = ms;
+  const PcgState& h = *c.state_host;
+  stats->status = h.status;
+  stats->iterations = h.iteration;
+  stats->recursive_relative_residual = std::sqrt((double)h.rr / (double)h.rr0);
+  stats->true_relative_residual = std::sqrt((double)h.true_rr / (double)h.rr0);
+  stats->chunks = chunks;
+  stats->refinements = 0;
+  stats->fp64_relative_residual = NAN;
+  return DRAPE_OK;
+}
+
+// The SpMV alone, `launches` times back to back between two events on the context's stream (p -> q);
+// returns the average time per launch in ms.
+int spmv_timing(Context& c, int launches, double* avg_ms) {
+  SellView A = view_of(c);
+  hipStream_t s = c.stream;
+  float* part_pq = c.partials + 2 * c.G;
+  k_pcg_spmv<<<c.G, 256, 0, s>>>(A, c.p, c.q, part_pq, c.state, c.R, 1);
+  HIP_TRY(hipEventRecord(c.ev_spmv0, s));
+  for (int i = 0; i < launches; ++i) k_pcg_spmv<<<c.G, 256, 0, s>>>(A, c.p, c.q, part_pq, c.state, c.R, 1);
+  HIP_TRY(hipEventRecord(c.ev_spmv1, s));
+  HIP_TRY(hipEventSynchronize(c.ev_spmv1));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, c.ev_spmv0, c.ev_spmv1));
+  *avg_ms = ms / launches;
+  return DRAPE_OK;
+}
+
+}  // namespace drape

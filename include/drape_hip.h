@@ -1,0 +1,208 @@
+/* drape_hip.h -- C ABI of the MI355X (gfx950) implicit Newton step of drape's cloth solver.
+ *
+ * The reference has no FFI: its GPU runner (engine/gpu, TypeScript over WebGPU) and its Rust oracle talk
+ * through problem.json/problem.bin and record files. This ABI exposes the same step behind plain pointers
+ * and sizes: mesh, material and state go in; positions and Newton/PCG statistics come out. Each entry
+ * point names the reference interface it replaces (file:line relative to the reference repository).
+ *
+ * Conventions
+ *  - Every call returns DRAPE_OK (0) or a negative DRAPE_E_* code; drape_last_error() then describes it.
+ *    Numeric outcomes (PCG status 0-3, Newton step status 0-5, PTC rejection 0-4) are data, not errors.
+ *  - The caller owns every host array; arrays are copied on the call. A context owns its device memory
+ *    and one HIP stream; calls on a context are synchronous at return and a context is not thread-safe.
+ *  - Positions are vertex-major float triples [3V]; "rows" are the free vertices (row_vertex order).
+ */
+#ifndef DRAPE_HIP_H
+#define DRAPE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRAPE_OK 0
+#define DRAPE_E_INVALID (-1) /* malformed problem or argument (problem.ts:28-34,78-86 throw) */
+#define DRAPE_E_DEVICE (-2)  /* HIP runtime or kernel failure (webgpu-backend.ts:23-34 throw) */
+#define DRAPE_E_STATE (-3)   /* the solve never raised a stop status (solver.ts:161 throw) */
+
+typedef struct drape_ctx drape_ctx;
+
+/* The problem.bin arrays the reference's runner loads (e3_generate.rs:84-104, e7_run.rs:46,
+ * e5_checks.rs:80-86; parsed by drape.ts:26-63). The library derives its own row-sorted scatter map
+ * and sliced-BSR layout from them, so block_contrib_* / force_contrib_* / diag_idx / dof_mask are not
+ * needed (all dofs of a row are free: pinned vertices have no row). */
+typedef struct {
+  int32_t vertices, triangles, hinges, rows, blocks;
+  const float* rest_positions;     /* [3V] */
+  const uint32_t* elem_vertices;   /* [3T] */
+  const float* elem_geom;          /* [7T] g0x g0y g1x g1y g2x g2y area */
+  const uint32_t* hinge_vertices;  /* [4H] opposite1 opposite2 edge0 edge1 */
+  const float* hinge_stiffness;    /* [H] */
+  const uint32_t* row_vertex;      /* [R] */
+  const float* row_unit_mass;      /* [R] vertex mass in kg (M/h^2 is formed for each step size) */
+  const uint32_t* row_ptr;         /* [R+1] BSR pattern (bsr.rs:12-29) */
+  const uint32_t* col_idx;         /* [B] sorted per row */
+  const float* colliders;          /* [4 + 8K]: count, dhat, kappa, drop-curvature, then kind a b c w 0 0 0; or NULL */
+  int32_t collider_floats;
+  double wale, course, coupling, shear; /* membrane constants, N/m (material.rs:4-26) */
+  double gravity;                  /* m/s^2 */
+  double time_step;                /* h of M/h^2 for dynamic steps (PTC rewrites it per attempt) */
+} drape_problem;
+
+/* The rule constants problem.json carries (e3_generate.rs:195-229, e4_rules.rs:391-396, e7_run.rs:28-33)
+ * plus the run switches of RobustSettings (robust-newton.ts:27-34). drape_default_rules() fills the
+ * reference's values (E6 rung D / E7 switches). */
+typedef struct {
+  double newton_tolerance;
+  int32_t newton_max_iterations;
+  double armijo;
+  int32_t max_halvings;
+  double pcg_tolerance;
+  int32_t pcg_max_iterations;
+  int32_t replace_every;
+  double replace_above;
+  double eta_start, eta_min, eta_max, eta_gamma, eta_safeguard, update_stop, resolution_factor, mu_start, mu_cap;
+  double h0, h_max, h_min_factor, growth, shrink, inner_tolerance;
+  int32_t inner_max_iterations, max_rejections;
+  double equilibrium_f32;
+  int32_t stagnation_window;
+  double stagnation_factor, gravity_load;
+  int32_t floor_stop, forcing, regularize, noise_aware;
+  int32_t exact_tangent;  /* 1 exact membrane+hinge tangent, 0 projected / Gauss-Newton */
+  int32_t refine_fp64;    /* 1: FP32 PCG inside an FP64 true-residual refinement loop */
+  int32_t preconditioner; /* DRAPE_PRECOND_* */
+} drape_rules;
+
+#define DRAPE_PRECOND_BLOCK_JACOBI 0
+#define DRAPE_PRECOND_TWO_LEVEL 1
+
+/* PCG outcome of one solve (solver.ts:33-43): status 0 running, 1 converged, 2 iteration limit,
+ * 3 breakdown (state.wgsl:2-3). */
+typedef struct {
+  int32_t status;
+  int32_t iterations;
+  double recursive_relative_residual; /* sqrt(rr/rr0) */
+  double true_relative_residual;      /* sqrt(|b - Ax|^2 / rr0), f32 arithmetic on the device */
+  int32_t chunks;                     /* host readbacks */
+  int32_t refinements;                /* FP64 refinement passes (0 without refine_fp64) */
+  double fp64_relative_residual;      /* |b - Ax|/|b| in FP64 (refine_fp64 only, else NaN) */
+} drape_pcg_stats;
+
+/* One Newton iterate: the 9 RECORD_FIELDS and 16 EXTRA_FIELDS of node/outputs.ts:8-11, in that order. */
+typedef struct {
+  double step, iteration, relative_residual, pcg_iterations, pcg_status, linear_residual, step_length, halvings,
+      status;
+  double eta, mu, update_inf, step_inf, retries, resolution, alpha_max, limiting_vertex, min_distance, delta_sum,
+      delta_abs, noise_only, slope, slope_exact, norm_squared, norm_squared_exact;
+} drape_record;
+
+/* One accepted PTC step (drape-e7.ts:14-23, written as the 7-field ptc rows of run-e7.ts:96). */
+typedef struct {
+  double step, h, residual, inner, rejected, pcg_iterations, min_distance, curvature_dropped;
+} drape_ptc_step;
+
+/* One attempt (drape-e7.ts:26-33): reason 0 accepted, 1 regularization cap, 2 line search failed,
+ * 3 non-finite, 4 curvature retry. */
+typedef struct {
+  double step, attempt, h, reason, record_start, record_end;
+} drape_ptc_attempt;
+
+typedef struct {
+  int32_t stop;            /* 0 tolerance, 1 stagnation, 2 cap, 3 stalled, 4 time budget (ptc.ts:260) */
+  int32_t accepted_steps;
+  int32_t newton_iterations; /* accepted updates over all records */
+  int64_t pcg_iterations;
+  int32_t records, attempts;
+  double final_h, final_residual, gravity_load;
+  double seconds;          /* wall clock of the solve loop */
+  double phase_ms[6];      /* assembly, pcg, line_search, ccd, reduce, other (timed-backend.ts:3-53) */
+  int64_t fallbacks;       /* two-level solves that fell back to block-Jacobi */
+} drape_ptc_result;
+
+/* A sheet scene: the reference's Scene / SceneParams and the material and step constants its Rust
+ * generator holds as consts (drape.rs:53-155, cases.rs:9-12, material.rs:15-26), made inputs.
+ * jitter / flip_fraction > 0 select an irregular mesh (no reference counterpart). */
+typedef struct {
+  int32_t columns;
+  double sheet_size, rest_factor;
+  double wale, course, coupling, shear, bend_wale, bend_course, density;
+  double gravity, time_step;
+  int32_t pinned;            /* pin the two top corners (drape.rs:148) */
+  int32_t n_colliders;       /* <= 4 */
+  double colliders[4 * 8];   /* kind a b c w 0 0 0: 1 plane (n, offset), 2 sphere (centre, radius) */
+  double dhat, kappa, lift;  /* kappa <= 0: resting stiffness of the heaviest vertex (contact.rs:104-107) */
+  int32_t exact_tangent;     /* unused by the builder (kept for layout parity with the oracle's scene) */
+  double jitter, flip_fraction;
+  uint64_t mesh_seed;
+} drape_scene;
+
+typedef struct drape_scene_arrays drape_scene_arrays;
+
+/* Builds the problem arrays of a scene (what the reference's generator writes to problem.bin,
+ * e3_generate.rs:84-104). The returned problem points into `arrays`; gravity_load = |m g| over rows
+ * (e7_ptc.rs:38-40). */
+int drape_scene_build(const drape_scene* scene, drape_scene_arrays** arrays);
+const drape_problem* drape_scene_problem(const drape_scene_arrays* arrays, double* gravity_load);
+void drape_scene_free(drape_scene_arrays* arrays);
+
+void drape_default_rules(drape_rules* rules);
+const char* drape_last_error(void);
+
+/* Uploads the problem and builds the device layout (replaces prepareDrapeGpu, drape-gpu.ts:141-162). */
+int drape_create(const drape_problem* problem, const drape_rules* rules, drape_ctx** out);
+void drape_destroy(drape_ctx* ctx);
+
+/* Writes positions [3V], x_hat [3V] (NULL keeps it) and mu; `h` > 0 rewrites M/h^2 from the unit masses
+ * (drape-e7.ts:64-72 prepare; robust-newton.ts:109-111 writeMu). */
+int drape_set_state(drape_ctx* ctx, const float* positions, const float* x_hat, double h, float mu);
+int drape_get_positions(drape_ctx* ctx, float* positions);
+/* Drops (1) or keeps (0) the b'grad^2 d term of the barrier Hessian (contact-host.ts:232-236). */
+int drape_set_barrier_curvature(drape_ctx* ctx, int32_t drop);
+
+/* Evaluate + assemble at the current state and reduce |rhs|^2 (newton-ops.ts:77-84 gpuResidual):
+ * norm = f32(sqrt(norm_squared)). */
+int drape_residual(drape_ctx* ctx, float* norm, double* norm_squared);
+/* One PCG solve of the assembled system from x = 0 (solver.ts:179-191 solveOnce); solution [3R] may be NULL. */
+int drape_pcg(drape_ctx* ctx, double tolerance, int32_t max_iterations, drape_pcg_stats* stats, float* solution);
+/* g^T d of the assembled rhs and the last solution: slope = -sum(rhs * x) (newton-ops.ts:86-90). */
+int drape_slope(drape_ctx* ctx, double* slope);
+/* Scatters the last PCG solution to the vertices as the line-search direction (robust-newton.ts:169-175);
+ * with `rows` non-NULL the direction is taken from it instead ([3R]). */
+int drape_set_direction(drape_ctx* ctx, const float* rows);
+/* Sum and absolute sum of the energy-change terms at alpha (newton-ops.ts:92-97), summed in FP64. */
+int drape_energy_change(drape_ctx* ctx, float alpha, double* sum, double* abs_sum);
+/* alpha_max of the colliders and its limiting vertex (-1 when 1) (contact-host.ts:175-180). */
+int drape_max_step(drape_ctx* ctx, float* alpha, int32_t* vertex, float* min_distance);
+
+/* Robust Newton from x_inout (updated in place) with origin x_n and residual scale
+ * (robust-newton.ts:231-250 solveStepRobust). Records are appended to `records` (capacity `cap`). */
+int drape_newton_step(drape_ctx* ctx, float* x_inout, float scale, int32_t step, const float* origin,
+                      drape_record* records, int32_t cap, int32_t* count);
+
+/* Quasi-static drape by pseudo-transient continuation from `start` with first step size h_start
+ * (drape-e7.ts:156-181 solveEquilibrium). `cap` accepted steps, `budget_s` > 0 a wall-clock budget.
+ * Any output array may be NULL; final positions go to `final_positions` [3V]. */
+int drape_ptc_solve(drape_ctx* ctx, const float* start, double h_start, int32_t cap, double budget_s,
+                    drape_ptc_result* result, drape_ptc_step* steps, int32_t steps_cap, drape_ptc_attempt* attempts,
+                    int32_t attempts_cap, drape_record* records, int32_t records_cap, float* final_positions);
+
+/* Kernel check (drape-step.ts evaluateDrapeKernels / drape-e4.ts:376-388): element and row terms at
+ * `positions` with the current x_hat, mu and masses: grad [9T + 12H + 3R], hess [81T + 144H + 9R]
+ * in the reference's local layout (block (k,l) of element e at 81e + 9(3k+l), row-major). */
+int drape_eval_elements(drape_ctx* ctx, const float* positions, float* grad, float* hess);
+/* The assembled system: rhs [3R] and vals [9B] in CSR block order. */
+int drape_get_system(drape_ctx* ctx, float* rhs, float* vals);
+
+/* A context over a plain BSR system (the E1 cases, generate.rs:330-339): drape_pcg then solves it. */
+int drape_linear_create(int32_t rows, int32_t blocks, const uint32_t* row_ptr, const uint32_t* col_idx,
+                        const float* vals, const float* rhs, const drape_rules* rules, drape_ctx** out);
+
+/* Device timing of the last drape_pcg (HIP events on the context's stream): average SpMV kernel time
+ * and the whole solve, in milliseconds. */
+int drape_pcg_timing(drape_ctx* ctx, double* spmv_ms, double* solve_ms, int32_t* spmv_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DRAPE_HIP_H */

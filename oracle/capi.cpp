@@ -322,6 +322,31 @@ int32_t or_ptc_inner_solve(or_drape* d, const double* xp, double h, double* out,
   return (int32_t)sol.status;
 }
 
+double or_time_newton_iteration(or_drape* d, const double* xp, double h, int32_t* ok) {
+  // One f64 Newton iteration of a PTC attempt at step size h from x (newton.rs:208-258 with mu = 0):
+  // tangent assembly, band Cholesky factorisation and solve, Armijo line search.
+  Drape& dr = d->drape;
+  size_t nv = dr.mesh.rest.size();
+  auto x = vec3s(xp, nv);
+  auto t0 = std::chrono::steady_clock::now();
+  dr.time_step = h;
+  DrapeStep system{&dr, ptc_prediction(dr, x, h)};
+  auto start = feasible_start(system, x);
+  auto g = system.residual(start);
+  BandCholesky f;
+  bool good = f.factor(system.regularized_tangent(start, 0.0));
+  if (good) {
+    std::vector<double> neg(g.size());
+    for (size_t i = 0; i < g.size(); ++i) neg[i] = -g[i];
+    auto direction = system.scatter(f.solve(neg));
+    LineSearch ls;
+    NewtonSettings settings{0.1, 10, 1e-4, 30};
+    good = line_search(system, start, g, direction, settings, &ls);
+  }
+  if (ok) *ok = good ? 1 : 0;
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void or_drape_derivative_check(int32_t columns, or_derivative_check* out) {
   DerivativeCheck c = drape_derivative_check(columns);
   out->gradient_error = c.gradient_error;

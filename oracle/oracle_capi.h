@@ -75,6 +75,9 @@ int32_t or_simulate(or_drape* d, int32_t steps, double tolerance, int32_t max_it
 /* One Newton solve of one PTC attempt at step size h from x (e7_ptc.rs:129-134), for timing. */
 int32_t or_ptc_inner_solve(or_drape* d, const double* x, double h, double* out_positions, int32_t* iterations);
 
+/* Seconds of one f64 Newton iteration (mu = 0) of a PTC attempt at step size h from x. */
+double or_time_newton_iteration(or_drape* d, const double* x, double h, int32_t* ok);
+
 typedef struct {
   double gradient_error, hessian_error, hinge_energy_change_error, energy_change_error;
 } or_derivative_check;

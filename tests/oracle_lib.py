@@ -97,6 +97,8 @@ def lib():
                                   f64p, C.c_int64, i64p, f64p]
         L.or_ptc_inner_solve.restype = C.c_int32
         L.or_ptc_inner_solve.argtypes = [C.c_void_p, f64p, C.c_double, f64p, i32p]
+        L.or_time_newton_iteration.restype = C.c_double
+        L.or_time_newton_iteration.argtypes = [C.c_void_p, f64p, C.c_double, i32p]
         L.or_drape_derivative_check.argtypes = [C.c_int32, C.POINTER(DerivativeCheck)]
         L.or_case_create.restype = C.c_void_p
         L.or_case_create.argtypes = [C.c_int32, C.c_int32]
@@ -240,6 +242,12 @@ class Drape:
         it = C.c_int32(0)
         status = lib().or_ptc_inner_solve(self.h, ptr(x, f64p), h, ptr(out, f64p), C.byref(it))
         return status, it.value, out
+
+    def time_newton_iteration(self, x, h):
+        x = np.ascontiguousarray(x, np.float64)
+        ok = C.c_int32(0)
+        secs = lib().or_time_newton_iteration(self.h, ptr(x, f64p), h, C.byref(ok))
+        return secs, bool(ok.value)
 
     def simulate(self, steps, tolerance, max_iterations, regularized, floor_factor):
         iters = np.zeros(steps, np.int32)
