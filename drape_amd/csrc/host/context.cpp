@@ -38,8 +38,8 @@ static void free_context(Context* c) {
   if (!c) return;
   void* ptrs[] = {c->pos, c->xhat, c->dir, c->tri, c->geom0, c->geom1, c->hin, c->hstiff, c->rvert, c->rmass,
                   c->colliders, c->mu, c->unit_mass_dev, c->org, c->material, c->lgrad, c->lhess, c->bc_ptr,
-                  c->bc_idx, c->fc_ptr, c->fc_idx, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->diag_k,
-                  c->block_dst, c->csr_row_ptr, c->csr_col, c->b, c->x, c->r, c->z, c->p, c->q, c->dinv,
+                  c->bc_idx, c->fc_ptr, c->fc_idx, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->perm, c->diag_slot,
+                  c->block_dst, c->csr_row_ptr, c->csr_col, c->b, c->x, c->r, c->z, c->p, c->p2, c->q, c->dinv,
                   c->partials, c->state, c->refine, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -52,38 +52,53 @@ static void free_context(Context* c) {
   delete c;
 }
 
-// Sliced BSR layout and the per-block destination of the CSR order.
-static int build_sell(Context& c, const uint32_t* row_ptr, const uint32_t* col_idx, std::vector<uint32_t>& block_dst,
-                      std::vector<uint32_t>& slice_ptr, std::vector<uint32_t>& sell_col, std::vector<uint32_t>& cnt,
-                      std::vector<uint32_t>& diag_k) {
+// Sliced BSR layout and the per-block destination of the CSR order. Inside each 256-row reduction group
+// the rows are placed by decreasing block count (stable), so that the 64-row slices carry little padding;
+// the kernels keep every reduction in logical row order through the permutation.
+struct SellLayout {
+  std::vector<uint32_t> block_dst, slice_ptr, sell_col, cnt, perm, diag_slot;
+};
+
+static int build_sell(Context& c, const uint32_t* row_ptr, const uint32_t* col_idx, SellLayout& L) {
   const int R = c.R;
-  c.S = (R + SLICE - 1) / SLICE;
-  slice_ptr.assign(c.S + 1, 0);
-  cnt.assign((size_t)c.S * SLICE, 0);
-  diag_k.assign(R, 0);
-  for (int s = 0; s < c.S; ++s) {
-    uint32_t width = 0;
-    for (int l = 0; l < SLICE; ++l) {
-      int row = s * SLICE + l;
-      if (row >= R) break;
-      uint32_t n = row_ptr[row + 1] - row_ptr[row];
-      cnt[row] = n;
-      width = std::max(width, n);
+  const int G = blocks_for(R);
+  const int P = G * WG;  // physical positions
+  c.S = P / SLICE;
+  L.perm.assign(P, 0xffffffffu);
+  L.cnt.assign(P, 0);
+  std::vector<uint32_t> phys_of(R);
+  for (int g = 0; g < G; ++g) {
+    std::vector<int> rows;
+    for (int r = g * WG; r < std::min(R, (g + 1) * WG); ++r) rows.push_back(r);
+    std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) {
+      return row_ptr[a + 1] - row_ptr[a] > row_ptr[b + 1] - row_ptr[b];
+    });
+    for (size_t t = 0; t < rows.size(); ++t) {
+      const int ph = g * WG + (int)t;
+      L.perm[ph] = (uint32_t)rows[t];
+      L.cnt[ph] = row_ptr[rows[t] + 1] - row_ptr[rows[t]];
+      phys_of[rows[t]] = (uint32_t)ph;
     }
-    slice_ptr[s + 1] = slice_ptr[s] + width;
   }
-  c.sell_slots = slice_ptr[c.S];
-  sell_col.assign(c.sell_slots * SLICE, 0);
-  block_dst.assign(c.B, 0);
+  L.slice_ptr.assign(c.S + 1, 0);
+  for (int sl = 0; sl < c.S; ++sl) {
+    uint32_t width = 0;
+    for (int l = 0; l < SLICE; ++l) width = std::max(width, L.cnt[sl * SLICE + l]);
+    L.slice_ptr[sl + 1] = L.slice_ptr[sl] + width;
+  }
+  c.sell_slots = L.slice_ptr[c.S];
+  if (c.sell_slots * 9 * SLICE >= (size_t)UINT32_MAX) {
+    set_error("matrix too large for 32-bit slot indices");
+    return DRAPE_E_INVALID;
+  }
+  L.sell_col.assign(c.sell_slots * SLICE, 0);
+  L.block_dst.assign(c.B, 0);
+  L.diag_slot.assign(R, 0);
   for (int row = 0; row < R; ++row) {
-    const int s = row / SLICE, l = row % SLICE;
+    const uint32_t ph = phys_of[row];
+    const int sl = (int)(ph / SLICE), l = (int)(ph % SLICE);
     bool has_diag = false;
     for (uint32_t b = row_ptr[row]; b < row_ptr[row + 1]; ++b) {
-      const uint32_t k = b - row_ptr[row];
-      const size_t q = slice_ptr[s] + k;
-      sell_col[q * SLICE + l] = col_idx[b];
-      block_dst[b] = (uint32_t)(q * 9 * SLICE + l);
-      if (col_idx[b] == (uint32_t)row) { diag_k[row] = k; has_diag = true; }
       if (b > row_ptr[row] && col_idx[b] <= col_idx[b - 1]) {
         set_error("col_idx must be strictly increasing within each row");
         return DRAPE_E_INVALID;
@@ -92,43 +107,48 @@ static int build_sell(Context& c, const uint32_t* row_ptr, const uint32_t* col_i
         set_error("col_idx out of range");
         return DRAPE_E_INVALID;
       }
+      const size_t q = L.slice_ptr[sl] + (b - row_ptr[row]);
+      L.sell_col[q * SLICE + l] = col_idx[b];
+      L.block_dst[b] = (uint32_t)(q * 9 * SLICE + l);
+      if (col_idx[b] == (uint32_t)row) {
+        L.diag_slot[row] = L.block_dst[b];
+        has_diag = true;
+      }
     }
     if (!has_diag) {
       set_error("row " + std::to_string(row) + " has no diagonal block");
       return DRAPE_E_INVALID;
     }
   }
-  if (c.sell_slots * 9 * SLICE >= (size_t)UINT32_MAX) {
-    set_error("matrix too large for 32-bit slot indices");
-    return DRAPE_E_INVALID;
-  }
   return DRAPE_OK;
 }
 
 // Allocations shared by mesh and linear contexts.
 static int alloc_solver(Context& c, const std::vector<uint32_t>& row_ptr, const std::vector<uint32_t>& col_idx) {
-  std::vector<uint32_t> block_dst, slice_ptr, sell_col, cnt, diag_k;
-  TRY(build_sell(c, row_ptr.data(), col_idx.data(), block_dst, slice_ptr, sell_col, cnt, diag_k));
+  SellLayout L;
+  TRY(build_sell(c, row_ptr.data(), col_idx.data(), L));
   c.G = blocks_for(c.R);
   hipStream_t s = c.stream;
-  TRY(alloc(&c.slice_ptr, slice_ptr.size()));
-  TRY(alloc(&c.sell_col, sell_col.size()));
+  TRY(alloc(&c.slice_ptr, L.slice_ptr.size()));
+  TRY(alloc(&c.sell_col, L.sell_col.size()));
   TRY(alloc(&c.sell_val, c.sell_slots * 9 * SLICE));
-  TRY(alloc(&c.row_cnt, cnt.size()));
-  TRY(alloc(&c.diag_k, diag_k.size()));
-  TRY(alloc(&c.block_dst, block_dst.size()));
+  TRY(alloc(&c.row_cnt, L.cnt.size()));
+  TRY(alloc(&c.perm, L.perm.size()));
+  TRY(alloc(&c.diag_slot, L.diag_slot.size()));
+  TRY(alloc(&c.block_dst, L.block_dst.size()));
   TRY(alloc(&c.csr_row_ptr, row_ptr.size()));
   TRY(alloc(&c.csr_col, col_idx.size()));
-  TRY(upload(c.slice_ptr, slice_ptr.data(), slice_ptr.size(), s));
-  TRY(upload(c.sell_col, sell_col.data(), sell_col.size(), s));
+  TRY(upload(c.slice_ptr, L.slice_ptr.data(), L.slice_ptr.size(), s));
+  TRY(upload(c.sell_col, L.sell_col.data(), L.sell_col.size(), s));
   HIP_TRY(hipMemsetAsync(c.sell_val, 0, c.sell_slots * 9 * SLICE * sizeof(float), s));
-  TRY(upload(c.row_cnt, cnt.data(), cnt.size(), s));
-  TRY(upload(c.diag_k, diag_k.data(), diag_k.size(), s));
-  TRY(upload(c.block_dst, block_dst.data(), block_dst.size(), s));
+  TRY(upload(c.row_cnt, L.cnt.data(), L.cnt.size(), s));
+  TRY(upload(c.perm, L.perm.data(), L.perm.size(), s));
+  TRY(upload(c.diag_slot, L.diag_slot.data(), L.diag_slot.size(), s));
+  TRY(upload(c.block_dst, L.block_dst.data(), L.block_dst.size(), s));
   TRY(upload(c.csr_row_ptr, row_ptr.data(), row_ptr.size(), s));
   TRY(upload(c.csr_col, col_idx.data(), col_idx.size(), s));
   const size_t Rp = (size_t)c.G * WG;
-  for (float4** v : {&c.b, &c.x, &c.r, &c.z, &c.p, &c.q}) {
+  for (float4** v : {&c.b, &c.x, &c.r, &c.z, &c.p, &c.p2, &c.q}) {
     TRY(alloc(v, Rp));
     HIP_TRY(hipMemsetAsync(*v, 0, Rp * sizeof(float4), s));
   }

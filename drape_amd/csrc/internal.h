@@ -72,15 +72,16 @@ struct Context {
   uint32_t* slice_ptr = nullptr;  // [S+1]
   uint32_t* sell_col = nullptr;   // [slots * 64]
   float* sell_val = nullptr;      // [slots * 64 * 9]: for slot base q, entry e of lane l at 9q*64... see spmv
-  uint32_t* row_cnt = nullptr;    // [S*64] blocks per row (0 beyond R)
-  uint32_t* diag_k = nullptr;     // [R] position of the diagonal block in the row
+  uint32_t* row_cnt = nullptr;    // [G*256] blocks of the row at each physical position (0 beyond R)
+  uint32_t* perm = nullptr;       // [G*256] physical position -> logical row (~0u beyond R)
+  uint32_t* diag_slot = nullptr;  // [R] sell_val base index of the row's diagonal block
   uint32_t* block_dst = nullptr;  // [B] sell_val base index of CSR block b (entry e at + e*64)
   uint32_t* csr_row_ptr = nullptr;// [R+1]
   uint32_t* csr_col = nullptr;    // [B]
   size_t sell_slots = 0;
 
   // PCG vectors (float4 per row, w = 0)
-  float4 *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  float4 *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
   float* dinv = nullptr;          // [9R]
   float* partials = nullptr;      // [2 * 3 * G]: pq | (rz, rr) | replace/true
   PcgState* state = nullptr;

@@ -17,26 +17,62 @@
 namespace drape {
 
 struct SellView {
-  const uint32_t* slice_ptr;
-  const uint32_t* col;
-  const float* val;
-  const uint32_t* cnt;
+  const uint32_t* slice_ptr;  // per physical slice: first block slot
+  const uint32_t* col;        // [slots * 64] logical column of (slot, lane)
+  const float* val;           // [slots * 9 * 64] entry e of (slot q, lane l) at (9q + e) * 64 + l
+  const uint32_t* cnt;        // blocks of the row at each physical position (0 past the last row)
+  const uint32_t* perm;       // physical position -> logical row (same 256-row group); ~0u past the last row
 };
 
-// acc[r] = acc[r] + (v[3r] p0 + v[3r+1] p1 + v[3r+2] p2) over the row's blocks in column order.
-__device__ inline float3 row_times(const SellView& A, const float4* __restrict__ v4, int row) {
-  const int s = row >> 6, l = row & 63;
+// Blocks of a row processed per chunk: every load of a chunk is issued before the first use, so a row
+// costs one column-load and one gather latency per chunk instead of one per block.
+constexpr int CHUNK = 8;
+
+// acc[r] = acc[r] + (v[3r] p0 + v[3r+1] p1 + v[3r+2] p2) over the row's blocks in column order (the
+// reference's per-row order, pcg.rs:359-369): contributions are formed independently, the sum stays
+// sequential. `dir` selects the fused direction update: the gathered value is z[c] + beta p[c].
+template <bool DIR>
+__device__ inline float3 row_times(const SellView& A, int phys, const float4* __restrict__ p,
+                                   const float4* __restrict__ z, float beta) {
+  const int s = phys >> 6, l = phys & 63;
   const uint32_t base = A.slice_ptr[s];
-  const uint32_t n = A.cnt[row];
+  const uint32_t n = A.cnt[phys];
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t q = base + k;
-    const uint32_t c = A.col[q * 64 + l];
-    const float* v = A.val + (size_t)q * 9 * 64 + l;
-    const float4 pc = v4[c];
-    a0 = a0 + (v[0] * pc.x + v[64] * pc.y + v[128] * pc.z);
-    a1 = a1 + (v[192] * pc.x + v[256] * pc.y + v[320] * pc.z);
-    a2 = a2 + (v[384] * pc.x + v[448] * pc.y + v[512] * pc.z);
+  for (uint32_t k0 = 0; k0 < n; k0 += CHUNK) {
+    uint32_t c[CHUNK];
+    float v[CHUNK][9];
+    float4 pc[CHUNK], zc[CHUNK];
+#pragma unroll
+    for (int j = 0; j < CHUNK; ++j) {
+      if (k0 + j < n) {
+        const uint32_t q = base + k0 + j;
+        c[j] = A.col[q * 64 + l];
+        const float* vp = A.val + (size_t)q * 9 * 64 + l;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) v[j][e] = vp[64 * e];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CHUNK; ++j) {
+      if (k0 + j < n) {
+        pc[j] = p[c[j]];
+        if (DIR) zc[j] = z[c[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CHUNK; ++j) {
+      if (k0 + j < n) {
+        float4 w = pc[j];
+        if (DIR) {
+          w.x = zc[j].x + beta * w.x;
+          w.y = zc[j].y + beta * w.y;
+          w.z = zc[j].z + beta * w.z;
+        }
+        a0 = a0 + (v[j][0] * w.x + v[j][1] * w.y + v[j][2] * w.z);
+        a1 = a1 + (v[j][3] * w.x + v[j][4] * w.y + v[j][5] * w.z);
+        a2 = a2 + (v[j][6] * w.x + v[j][7] * w.y + v[j][8] * w.z);
+      }
+    }
   }
   return make_float3(a0, a1, a2);
 }
@@ -44,6 +80,66 @@ __device__ inline float3 row_times(const SellView& A, const float4* __restrict__
 __device__ inline float3 precondition(const float* __restrict__ dinv, int row, float a, float b, float c) {
   const float* d = dinv + 9 * row;
   return make_float3(d[0] * a + d[1] * b + d[2] * c, d[3] * a + d[4] * b + d[5] * c, d[6] * a + d[7] * b + d[8] * c);
+}
+
+// The reference's final reduction pass (reduce.wgsl:33-53, pcg.rs:447-457) inside one wavefront: lane l
+// stands for the four tree lanes l, l+64, l+128, l+192 (each the strided sum of its partials), combines
+// them as the 128- and 64-strides of the tree do, and finishes with shuffles. No LDS, no barrier: every
+// wavefront obtains the identical result.
+__device__ inline float wave_reduce_partials(const float* __restrict__ part, int count) {
+  const int l = threadIdx.x & 63;
+  float v[4];
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    float a = 0.0f;
+    for (int j = l + 64 * qd; j < count; j += 256) a = a + part[j];
+    v[qd] = a;
+  }
+  float s = (v[0] + v[2]) + (v[1] + v[3]);
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1) s = s + __shfl_down(s, st, 64);
+  return __shfl(s, 0, 64);
+}
+
+__device__ inline float2 wave_reduce_partials2(const float* __restrict__ part, int count) {
+  const int l = threadIdx.x & 63;
+  float va[4], vb[4];
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    float a = 0.0f, b = 0.0f;
+    for (int j = l + 64 * qd; j < count; j += 256) {
+      a = a + part[2 * j];
+      b = b + part[2 * j + 1];
+    }
+    va[qd] = a;
+    vb[qd] = b;
+  }
+  float sa = (va[0] + va[2]) + (va[1] + va[3]);
+  float sb = (vb[0] + vb[2]) + (vb[1] + vb[3]);
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1) {
+    sa = sa + __shfl_down(sa, st, 64);
+    sb = sb + __shfl_down(sb, st, 64);
+  }
+  return make_float2(__shfl(sa, 0, 64), __shfl(sb, 0, 64));
+}
+
+// Group tree of per-row values held by physical position: the value goes to its logical slot first.
+__device__ inline float tree_logical(float v, uint32_t row, float* lds) {
+  const uint32_t t = row == 0xffffffffu ? threadIdx.x : row - blockIdx.x * 256;
+  lds[t] = v;
+  __syncthreads();
+  const float own = lds[threadIdx.x];
+  __syncthreads();
+  return tree256(own, lds);
+}
+__device__ inline float2 tree_logical2(float a, float b, uint32_t row, float2* lds) {
+  const uint32_t t = row == 0xffffffffu ? threadIdx.x : row - blockIdx.x * 256;
+  lds[t] = make_float2(a, b);
+  __syncthreads();
+  const float2 own = lds[threadIdx.x];
+  __syncthreads();
+  return tree256x2(own.x, own.y, lds);
 }
 
 // pcg.rs:268-281 invert3 (adjugate / determinant), same operation order.
@@ -59,7 +155,7 @@ __device__ inline void invert3(const float m[9], float* out) {
 }
 
 // jacobi_invert + cg_init: D^-1, x = 0, r = b, z = D^-1 r, p = z, partial (r.z, r.r).
-__global__ void __launch_bounds__(256) k_pcg_setup(SellView A, const uint32_t* __restrict__ diag_k,
+__global__ void __launch_bounds__(256) k_pcg_setup(SellView A, const uint32_t* __restrict__ diag_slot,
                                                    const float4* __restrict__ b, float* __restrict__ dinv,
                                                    float4* x, float4* r, float4* z, float4* p,
                                                    float* __restrict__ partials, int R) {
@@ -67,10 +163,9 @@ __global__ void __launch_bounds__(256) k_pcg_setup(SellView A, const uint32_t* _
   const int row = blockIdx.x * 256 + threadIdx.x;
   float rz = 0.0f, rr = 0.0f;
   if (row < R) {
-    const int s = row >> 6, l = row & 63;
-    const uint32_t q = A.slice_ptr[s] + diag_k[row];
+    const size_t d = diag_slot[row];
     float m[9], inv[9];
-    for (int e = 0; e < 9; ++e) m[e] = A.val[(size_t)q * 9 * 64 + e * 64 + l];
+    for (int e = 0; e < 9; ++e) m[e] = A.val[d + 64 * e];
     invert3(m, inv);
     for (int e = 0; e < 9; ++e) dinv[9 * row + e] = inv[e];
     const float4 bb = b[row];
@@ -110,35 +205,93 @@ __global__ void __launch_bounds__(256) k_pcg_start(const float* __restrict__ par
   }
 }
 
-// spmv_dot.wgsl: q = A p, partial p.q.
+// spmv_dot.wgsl: q = A p, partial p.q (the first iteration of a chunk, and the timing probe).
 __global__ void __launch_bounds__(256) k_pcg_spmv(SellView A, const float4* __restrict__ p, float4* __restrict__ q,
                                                   float* __restrict__ partial_pq, const PcgState* st, int R,
                                                   int ignore_status) {
   __shared__ float lds[256];
   if (!ignore_status && st->status != 0) return;
-  const int row = blockIdx.x * 256 + threadIdx.x;
+  const int phys = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t row = A.perm[phys];
   float pq = 0.0f;
-  if (row < R) {
-    const float3 a = row_times(A, p, row);
+  if (row != 0xffffffffu) {
+    const float3 a = row_times<false>(A, phys, p, nullptr, 0.0f);
     q[row] = make_float4(a.x, a.y, a.z, 0.0f);
     const float4 pr = p[row];
     pq = pr.x * a.x + pr.y * a.y + pr.z * a.z;
   }
-  const float s = tree256(pq, lds);
+  const float s = tree_logical(pq, row, lds);
   if (threadIdx.x == 0) partial_pq[blockIdx.x] = s;
 }
 
+// direction.wgsl of iteration it-1 fused with spmv_dot.wgsl of iteration it: beta and the convergence
+// test from the (r.z, r.r) partials of the last update; then p_it = z + beta p_{it-1} for the own row
+// (written to the other p buffer) and, for the gathered columns, the same value formed on the fly;
+// q = A p_it and partial p.q.
+__global__ void __launch_bounds__(256) k_pcg_dir_spmv(SellView A, const float* __restrict__ partial_rz, int G,
+                                                      PcgState* st, const float4* __restrict__ z,
+                                                      const float4* __restrict__ p_old, float4* __restrict__ p_new,
+                                                      float4* __restrict__ q, float* __restrict__ partial_pq, int R,
+                                                      int it) {
+  __shared__ float lds[256];
+  if (st->status != 0) return;
+  const float rz_old = st->rz[(it - 1) & 1];
+  const float2 s = wave_reduce_partials2(partial_rz, G);
+  const float rz_new = s.x, rr = s.y;
+  const float beta = rz_new / rz_old;
+  int status = 0;
+  if (rr <= st->tol2 * st->rr0) status = 1;
+  else if (it >= st->max_iterations) status = 2;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz[it & 1] = rz_new;
+    st->rr = rr;
+    st->beta = beta;
+    st->iteration = it;
+    if (status != 0) st->status = status;
+  }
+  if (status != 0) return;
+  const int phys = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t row = A.perm[phys];
+  float pq = 0.0f;
+  if (row != 0xffffffffu) {
+    const float3 a = row_times<true>(A, phys, p_old, z, beta);
+    q[row] = make_float4(a.x, a.y, a.z, 0.0f);
+    const float4 zr = z[row], po = p_old[row];
+    float4 pr;
+    pr.x = zr.x + beta * po.x;
+    pr.y = zr.y + beta * po.y;
+    pr.z = zr.z + beta * po.z;
+    pr.w = 0.0f;
+    p_new[row] = pr;
+    pq = pr.x * a.x + pr.y * a.y + pr.z * a.z;
+  }
+  const float t = tree_logical(pq, row, lds);
+  if (threadIdx.x == 0) partial_pq[blockIdx.x] = t;
+}
+
 // reduce.wgsl MODE 1 (alpha, breakdown) folded into update_dot.wgsl: x += alpha p, r -= alpha q,
-// z = D^-1 r, partial (r.z, r.r).
+// z = D^-1 r, partial (r.z, r.r). The row's operands are loaded before the pq reduction.
 __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ partial_pq, int G, PcgState* st,
                                                     const float4* __restrict__ p, const float4* __restrict__ q,
                                                     const float* __restrict__ dinv, float4* x, float4* r, float4* z,
                                                     float* __restrict__ partial_rz, int R, int it) {
-  __shared__ float lds[256];
   __shared__ float2 lds2[256];
   if (st->status != 0) return;
-  const float pq = reduce_partials(partial_pq, G, lds);
-  const float alpha = st->rz[it & 1] / pq;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  const bool live = row < R;
+  float4 pr = make_float4(0, 0, 0, 0), qr = pr, xr = pr, rv = pr;
+  float d[9];
+  if (live) {
+    pr = p[row];
+    qr = q[row];
+    xr = x[row];
+    rv = r[row];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) d[e] = dinv[9 * row + e];
+  }
+  const float rzv = st->rz[it & 1];
+  const float pq = wave_reduce_partials(partial_pq, G);
+  const float alpha = rzv / pq;
   if (!(pq > 0.0f && finite32(alpha))) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->status = 3;
     return;
@@ -147,11 +300,8 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     st->pq = pq;
     st->alpha = alpha;
   }
-  const int row = blockIdx.x * 256 + threadIdx.x;
   float rz = 0.0f, rr = 0.0f;
-  if (row < R) {
-    const float4 pr = p[row], qr = q[row];
-    float4 xr = x[row], rv = r[row];
+  if (live) {
     xr.x = xr.x + alpha * pr.x;
     xr.y = xr.y + alpha * pr.y;
     xr.z = xr.z + alpha * pr.z;
@@ -160,9 +310,11 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     rv.z = rv.z - alpha * qr.z;
     x[row] = xr;
     r[row] = rv;
-    const float3 zz = precondition(dinv, row, rv.x, rv.y, rv.z);
-    z[row] = make_float4(zz.x, zz.y, zz.z, 0.0f);
-    rz = rv.x * zz.x + rv.y * zz.y + rv.z * zz.z;
+    const float z0 = d[0] * rv.x + d[1] * rv.y + d[2] * rv.z;
+    const float z1 = d[3] * rv.x + d[4] * rv.y + d[5] * rv.z;
+    const float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
+    z[row] = make_float4(z0, z1, z2, 0.0f);
+    rz = rv.x * z0 + rv.y * z1 + rv.z * z2;
     rr = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
   }
   const float2 sum = tree256x2(rz, rr, lds2);
@@ -172,14 +324,22 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
   }
 }
 
-// reduce.wgsl MODE 2 (beta, convergence, iteration limit) folded into direction.wgsl: p = z + beta p.
+// reduce.wgsl MODE 2 (beta, convergence, iteration limit) folded into direction.wgsl:
+// p_{it+1} = z + beta p_it, from p_old into p_new.
 __global__ void __launch_bounds__(256) k_pcg_direction(const float* __restrict__ partial_rz, int G, PcgState* st,
-                                                       const float4* __restrict__ z, float4* p, int R, int it) {
-  __shared__ float2 lds[256];
+                                                       const float4* __restrict__ z, const float4* __restrict__ p_old,
+                                                       float4* __restrict__ p_new, int R, int it) {
   if (st->status != 0) return;
-  const float2 s = reduce_partials2(partial_rz, G, lds);
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  float4 zr = make_float4(0, 0, 0, 0), po = zr;
+  if (row < R) {
+    zr = z[row];
+    po = p_old[row];
+  }
+  const float rz_old = st->rz[it & 1];
+  const float2 s = wave_reduce_partials2(partial_rz, G);
   const float rz_new = s.x, rr = s.y;
-  const float beta = rz_new / st->rz[it & 1];
+  const float beta = rz_new / rz_old;
   int status = 0;
   if (rr <= st->tol2 * st->rr0) status = 1;
   else if (it + 1 >= st->max_iterations) status = 2;
@@ -191,14 +351,13 @@ __global__ void __launch_bounds__(256) k_pcg_direction(const float* __restrict__
     if (status != 0) st->status = status;
   }
   if (status != 0) return;
-  const int row = blockIdx.x * 256 + threadIdx.x;
   if (row < R) {
-    const float4 zr = z[row];
-    float4 pr = p[row];
-    pr.x = zr.x + beta * pr.x;
-    pr.y = zr.y + beta * pr.y;
-    pr.z = zr.z + beta * pr.z;
-    p[row] = pr;
+    float4 pr;
+    pr.x = zr.x + beta * po.x;
+    pr.y = zr.y + beta * po.y;
+    pr.z = zr.z + beta * po.z;
+    pr.w = 0.0f;
+    p_new[row] = pr;
   }
 }
 
@@ -210,10 +369,11 @@ __global__ void __launch_bounds__(256) k_pcg_residual(SellView A, const float4* 
   __shared__ float2 lds[256];
   // replace only while rr > above2 rr0 (pcg.rs:293-296; a NaN rr does not replace)
   if (replace && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
-  const int row = blockIdx.x * 256 + threadIdx.x;
+  const int phys = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t row = A.perm[phys];
   float a = 0.0f, c = 0.0f;
-  if (row < R) {
-    const float3 ax = row_times(A, x, row);
+  if (row != 0xffffffffu) {
+    const float3 ax = row_times<false>(A, phys, x, nullptr, 0.0f);
     const float4 bb = b[row];
     const float s0 = bb.x - ax.x, s1 = bb.y - ax.y, s2 = bb.z - ax.z;
     const float ss = s0 * s0 + s1 * s1 + s2 * s2;
@@ -227,7 +387,7 @@ __global__ void __launch_bounds__(256) k_pcg_residual(SellView A, const float4* 
       a = ss;
     }
   }
-  const float2 sum = tree256x2(a, c, lds);
+  const float2 sum = tree_logical2(a, c, row, lds);
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = sum.x;
     partials[2 * blockIdx.x + 1] = sum.y;
@@ -253,7 +413,7 @@ __global__ void __launch_bounds__(256) k_pcg_residual_final(const float* __restr
   }
 }
 
-static SellView view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt}; }
+static SellView view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm}; }
 
 int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
   const int G = c.G, R = c.R;
@@ -266,20 +426,30 @@ int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats*
   const float above2 = (float)(c.rules.replace_above * c.rules.replace_above);
   hipStream_t s = c.stream;
   HIP_TRY(hipEventRecord(c.ev0, s));
-  k_pcg_setup<<<G, 256, 0, s>>>(A, c.diag_k, c.b, c.dinv, c.x, c.r, c.z, c.p, part_setup, R);
+  k_pcg_setup<<<G, 256, 0, s>>>(A, c.diag_slot, c.b, c.dinv, c.x, c.r, c.z, c.p, part_setup, R);
   k_pcg_start<<<1, 256, 0, s>>>(part_setup, G, c.state, tol2, above2, max_iterations);
   HIP_TRY(hipGetLastError());
+  // p ping-pongs between c.p (even iterations) and c.p2 (odd). A chunk opens with a plain SpMV (p is
+  // current), fuses each later direction update into its SpMV, and closes with an explicit direction
+  // update so that a residual replacement sees the updated p (pcg.rs:277-288 order).
   const int replace_every = c.rules.replace_every;
   const int chunk = replace_every > 0 ? replace_every : CHUNK_PLAIN;
   const int chunk_limit = (max_iterations + chunk - 1) / chunk + 1;
+  float4* pbuf[2] = {c.p, c.p2};
   int chunks = 0, it = 0;
   bool stopped = false;
   for (chunks = 1; chunks <= chunk_limit; ++chunks) {
+    const int first = it;
     for (int j = 0; j < chunk; ++j, ++it) {
-      k_pcg_spmv<<<G, 256, 0, s>>>(A, c.p, c.q, part_pq, c.state, R, 0);
-      k_pcg_update<<<G, 256, 0, s>>>(part_pq, G, c.state, c.p, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
-      k_pcg_direction<<<G, 256, 0, s>>>(part_rz, G, c.state, c.z, c.p, R, it);
+      float4* pit = pbuf[it & 1];
+      if (it == first)
+        k_pcg_spmv<<<G, 256, 0, s>>>(A, pit, c.q, part_pq, c.state, R, 0);
+      else
+        k_pcg_dir_spmv<<<G, 256, 0, s>>>(A, part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pit, c.q, part_pq, R, it);
+      k_pcg_update<<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
     }
+    // direction update after the chunk's last iteration it - 1: p_it = z + beta p_{it-1} into pbuf[it & 1]
+    k_pcg_direction<<<G, 256, 0, s>>>(part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pbuf[it & 1], R, it - 1);
     if (replace_every > 0) {
       k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 1);
       k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 1, it - 1);
