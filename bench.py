@@ -1,17 +1,19 @@
 """Benchmark of the implicit Newton step at n129: ms per Newton iteration and PCG iterations per Newton
 iteration of the quasi-static sphere drape solved by pseudo-transient continuation (129 x 129 grid,
 16641 vertices, 32768 triangles; isotropic E = 1e6 Pa, nu = 0.3, t = 0.5 mm, 0.2 kg/m^2, hinge 1e-5 N m,
-sphere r = 0.15 m, dhat = 1 mm), block-Jacobi FP32 PCG.
+sphere r = 0.15 m, dhat = 1 mm), two-level preconditioner against block-Jacobi FP32 PCG.
 
 A bench step is one accepted PTC step (one inexact Newton solve of the step's energy, several Newton
-iterations). `--warmup W` steps run untimed from the flat start; the next `--steps K` are timed between
-device synchronisations. Prints one JSON line.
+iterations). `--warmup W` steps run untimed from the flat start; the next `--steps K` are timed, once with
+the selected preconditioner (the reported value) and once with block-Jacobi from the same state (the
+comparison in `config`). Prints one JSON line.
 
-The roofline object is for the dominant kernel, the SpMV of PCG (92% of the reference's n129 run):
-algorithmic bytes 40 B per block + 28 B per row + 4 (vals 36 B + column 4 B per block; row pointer 4 B,
-x gather 12 B and y 12 B per row), divided by the kernel's average duration from HIP events on the
-context's stream. The CPU baseline is the oracle (f64 Newton with band Cholesky, as the reference's Rust
-oracle does) timed for one Newton iteration of the same first PTC step on one core.
+The roofline object is for the SpMV of PCG, the dominant kernel of the block-Jacobi solve (92% of the
+reference's n129 run) and the largest of the two-level iteration: algorithmic bytes 40 B per block + 28 B
+per row + 4 (vals 36 B + column 4 B per block; row pointer 4 B, x gather 12 B and y 12 B per row), divided
+by the kernel's average duration from HIP events on the context's stream. The CPU baseline is the oracle
+(f64 Newton with band Cholesky, as the reference's Rust oracle does) timed for one Newton iteration of the
+same first PTC step on one core.
 """
 from __future__ import annotations
 
@@ -66,7 +68,8 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--columns", type=int, default=129)
-    ap.add_argument("--preconditioner", choices=["block_jacobi", "two_level"], default="block_jacobi")
+    ap.add_argument("--preconditioner", choices=["block_jacobi", "two_level"], default="two_level")
+    ap.add_argument("--no-compare", action="store_true", help="skip the block-Jacobi comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -76,27 +79,42 @@ def main() -> None:
 
     scene = D.drape_config_scene(args.columns)
     arrays = D.SceneArrays(scene)
-    rules = D.default_rules(preconditioner=0 if args.preconditioner == "block_jacobi" else 1)
-    ctx = D.Drape(arrays, rules)
+    kind = 0 if args.preconditioner == "block_jacobi" else 1
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=kind))
     print(f"[bench] n{args.columns}: V {arrays.V} T {arrays.T} H {arrays.H} rows {arrays.R} blocks {arrays.B}",
           file=sys.stderr, flush=True)
 
     start = arrays.array("rest_positions")
-    h = rules.h0
+    h = ctx.rules.h0
     if args.warmup > 0:
         w = ctx.solve_equilibrium(start=start, h_start=h, cap=args.warmup)
         start, h = w["positions"], w["result"].final_h
         print(f"[bench] warmup: {w['result'].accepted_steps} steps, stop {w['stop']}", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    out = ctx.solve_equilibrium(start=start, h_start=h, cap=args.steps)
-    wall = time.perf_counter() - t0
-    res = out["result"]
-    newton = max(res.newton_iterations, 1)
+
+    def timed(context, label):
+        t0 = time.perf_counter()
+        out = context.solve_equilibrium(start=start, h_start=h, cap=args.steps)
+        wall = time.perf_counter() - t0
+        res = out["result"]
+        newton = max(res.newton_iterations, 1)
+        print(f"[bench] {label}: {res.accepted_steps} steps, {res.newton_iterations} Newton, {res.pcg_iterations} "
+              f"PCG, {res.seconds:.3f} s, fallbacks {res.fallbacks}, stop {out['stop']}, "
+              f"r/L {res.final_residual / res.gravity_load:.3e}", file=sys.stderr, flush=True)
+        return out, res, newton, wall
+
+    out, res, newton, wall = timed(ctx, args.preconditioner)
     ms_per_newton = 1000.0 * res.seconds / newton
     pcg_per_newton = res.pcg_iterations / newton
-    print(f"[bench] timed: {res.accepted_steps} steps, {res.newton_iterations} Newton, {res.pcg_iterations} PCG, "
-          f"{res.seconds:.3f} s, stop {out['stop']}, r/L {res.final_residual / res.gravity_load:.3e}",
-          file=sys.stderr, flush=True)
+    comparison = None
+    if kind != 0 and not args.no_compare:
+        bj = D.Drape(arrays, D.default_rules(preconditioner=0))
+        _, bres, bnewton, _ = timed(bj, "block_jacobi")
+        comparison = {"ms_per_newton": round(1000.0 * bres.seconds / bnewton, 4),
+                      "pcg_per_newton": round(bres.pcg_iterations / bnewton, 2),
+                      "newton_iterations": bres.newton_iterations, "pcg_iterations": bres.pcg_iterations,
+                      "accepted_steps": bres.accepted_steps}
+        bj.close()
+    layout = ctx.two_level_layout() if kind != 0 else None
 
     # dominant kernel: SpMV of the PCG on the last assembled system, timed with HIP events
     spmv_ms, _ = ctx.spmv_timing(400)
@@ -125,7 +143,11 @@ def main() -> None:
                    "preconditioner": args.preconditioner, "pcg_per_newton": round(pcg_per_newton, 2),
                    "newton_iterations": res.newton_iterations, "pcg_iterations": res.pcg_iterations,
                    "accepted_steps": res.accepted_steps, "phase_ms": [round(v, 2) for v in res.phase_ms],
-                   "fallbacks": res.fallbacks, "wall_s": round(wall, 3)},
+                   "fallbacks": res.fallbacks, "wall_s": round(wall, 3),
+                   "coarse_rows": layout[0] if layout else None,
+                   "factorisations": layout[3][0] if layout else None,
+                   "factorisation_ms": round(layout[3][3] / 1000.0, 2) if layout else None,
+                   "block_jacobi": comparison},
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

@@ -36,7 +36,7 @@ class Problem(C.Structure):
                 ("hinge_vertices", u32p), ("hinge_stiffness", f32p), ("row_vertex", u32p), ("row_unit_mass", f32p),
                 ("row_ptr", u32p), ("col_idx", u32p), ("colliders", f32p), ("collider_floats", C.c_int32),
                 ("wale", C.c_double), ("course", C.c_double), ("coupling", C.c_double), ("shear", C.c_double),
-                ("gravity", C.c_double), ("time_step", C.c_double)]
+                ("gravity", C.c_double), ("time_step", C.c_double), ("grid_columns", C.c_int32)]
 
 
 class Rules(C.Structure):
@@ -50,7 +50,8 @@ class Rules(C.Structure):
                 ("stagnation_window", C.c_int32), ("stagnation_factor", C.c_double), ("gravity_load", C.c_double),
                 ("floor_stop", C.c_int32), ("forcing", C.c_int32), ("regularize", C.c_int32),
                 ("noise_aware", C.c_int32), ("exact_tangent", C.c_int32), ("refine_fp64", C.c_int32),
-                ("preconditioner", C.c_int32)]
+                ("preconditioner", C.c_int32), ("coarse_reuse", C.c_int32),
+                ("fallback", C.c_int32), ("coarse_shift", C.c_double)]
 
 
 class PcgStats(C.Structure):
@@ -97,7 +98,7 @@ class Scene(C.Structure):
 
 
 STOP_RULES = ["tolerance", "stagnation", "cap", "stalled", "time_budget"]
-PCG_STATUS = ["running", "converged", "iteration_limit", "breakdown"]
+PCG_STATUS = ["running", "converged", "iteration_limit", "breakdown", "negative_curvature"]
 
 _lib = None
 
@@ -133,6 +134,9 @@ def lib():
         L.drape_eval_elements.argtypes = [vp, f32p, f32p, f32p]
         L.drape_get_system.argtypes = [vp, f32p, f32p]
         L.drape_pcg_timing.argtypes = [vp, f64p, f64p, i32p]
+        L.drape_set_preconditioner.argtypes = [vp, C.c_int32, f32p]
+        L.drape_two_level_layout.argtypes = [vp, i32p, i32p, f32p, C.POINTER(C.c_int64)]
+        L.drape_two_level_coarse.argtypes = [vp, f64p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_problem.restype = C.POINTER(Problem)
         L.drape_scene_problem.argtypes = [vp, f64p]
@@ -333,6 +337,32 @@ class Drape:
         n = C.c_int32(launches)
         _check(lib().drape_pcg_timing(self.h, C.byref(spmv), C.byref(solve), C.byref(n)))
         return spmv.value, solve.value
+
+    def set_preconditioner(self, kind, weights=None):
+        """'block_jacobi' or 'two_level' (optionally with parent-slot weights [4R])."""
+        k = {"block_jacobi": 0, "two_level": 1}[kind] if isinstance(kind, str) else int(kind)
+        w = None if weights is None else _f32(weights)
+        if w is not None and w.size != 4 * self.R:
+            raise ValueError("weights must have 4 entries per row")
+        _check(lib().drape_set_preconditioner(self.h, k, None if w is None else _ptr(w, f32p)))
+        self.rules.preconditioner = k
+
+    def two_level_layout(self):
+        """(coarse rows, parent [R, 4], bilinear weights [R, 4], counters (factorisations, failures,
+        fallbacks, microseconds factorising))."""
+        nc = C.c_int32()
+        parent = np.zeros(4 * self.R, np.int32)
+        w = np.zeros(4 * self.R, np.float32)
+        cnt = (C.c_int64 * 4)()
+        _check(lib().drape_two_level_layout(self.h, C.byref(nc), _ptr(parent, i32p), _ptr(w, f32p), cnt))
+        return nc.value, parent.reshape(-1, 4), w.reshape(-1, 4), tuple(cnt)
+
+    def two_level_coarse(self):
+        """The Galerkin coarse operator P^T A P of the assembled system (dense, f64)."""
+        nc = self.two_level_layout()[0]
+        out = np.zeros((3 * nc, 3 * nc))
+        _check(lib().drape_two_level_coarse(self.h, _ptr(out, f64p)))
+        return out
 
     # ---- the step and its caller ----
     def solve_step_robust(self, x, scale, step, origin, cap=1024):

@@ -18,7 +18,8 @@ SOURCES = [
 ]
 # -ffp-contract=off keeps FP32 arithmetic in the reference's operation order (no FMA contraction), which
 # makes the PCG iterates bit-identical to the CPU emulation; divisions and square roots stay correctly
-# rounded; packed FP32 instructions are disabled (see the build notes in DESIGN.md).
+# rounded; packed FP32 instructions are disabled (some of their forms have given wrong results on this
+# hardware without an error).
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
@@ -40,7 +41,10 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    # rocSOLVER factorises the two-level preconditioner's small dense coarse matrix once per system
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib,
+           "-lrocsolver", "-lrocblas", "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

@@ -321,6 +321,7 @@ static int create_mesh_context(const drape_problem* pr, const drape_rules* rules
   if (c.K > 0) TRY(upload(c.colliders, c.colliders_host.data(), c.colliders_host.size(), s));
   else HIP_TRY(hipMemsetAsync(c.colliders, 0, 4 * sizeof(float), s));
   TRY(launch_set_float(c, c.mu, 0.0f));
+  note_shift(c, 0.0f, pr->time_step);
   const float mat[4] = {(float)pr->wale, (float)pr->course, (float)pr->coupling, (float)pr->shear};
   TRY(upload(c.material, mat, 4, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -366,6 +367,9 @@ void drape_default_rules(drape_rules* r) {
   r->exact_tangent = 1;
   r->refine_fp64 = 0;
   r->preconditioner = DRAPE_PRECOND_BLOCK_JACOBI;
+  r->coarse_reuse = 1;
+  r->fallback = 2;
+  r->coarse_shift = 0.0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }
@@ -379,6 +383,8 @@ int drape_create(const drape_problem* problem, const drape_rules* rules, drape_c
   drape_default_rules(&defaults);
   Context* c = new Context();
   int rc = create_mesh_context(problem, rules ? rules : &defaults, *c);
+  if (rc == DRAPE_OK && c->rules.preconditioner == DRAPE_PRECOND_TWO_LEVEL)
+    rc = two_level_create(*c, problem->grid_columns);
   if (rc != DRAPE_OK) {
     free_context(c);
     *out = nullptr;
@@ -432,6 +438,7 @@ int drape_linear_create(int32_t rows, int32_t blocks, const uint32_t* row_ptr, c
     set_error("upload failed");
     return fail(DRAPE_E_DEVICE);
   }
+  c->system_version = 1;
   *out = reinterpret_cast<drape_ctx*>(c);
   return DRAPE_OK;
 }
@@ -458,6 +465,7 @@ int drape_set_state(drape_ctx* p, const float* positions, const float* x_hat, do
     HIP_TRY(hipMemcpyAsync(c->rmass, c->row_mass_host.data(), c->R * sizeof(float), hipMemcpyHostToDevice, c->stream));
   }
   HIP_TRY(hipMemcpyAsync(c->mu, &mu, sizeof(float), hipMemcpyHostToDevice, c->stream));
+  note_shift(*c, mu, h);
   HIP_TRY(hipStreamSynchronize(c->stream));
   return DRAPE_OK;
 }
@@ -582,6 +590,49 @@ int drape_get_system(drape_ctx* p, float* rhs, float* vals) {
       for (int e = 0; e < 9; ++e) vals[9 * (size_t)b + e] = sell[block_dst[b] + SLICE * e];
   }
   return DRAPE_OK;
+}
+
+int drape_set_preconditioner(drape_ctx* p, int32_t kind, const float* weights) {
+  Context* c = ctx_of(p);
+  if (!c || (kind != DRAPE_PRECOND_BLOCK_JACOBI && kind != DRAPE_PRECOND_TWO_LEVEL)) {
+    set_error("invalid preconditioner");
+    return DRAPE_E_INVALID;
+  }
+  if (kind == DRAPE_PRECOND_TWO_LEVEL) {
+    if (c->linear) {
+      set_error("the two-level preconditioner needs a mesh context");
+      return DRAPE_E_INVALID;
+    }
+    if (!c->two_level) TRY(two_level_create(*c, c->problem.grid_columns));
+    TRY(two_level_set_weights(*c, weights));
+  } else if (weights) {
+    set_error("weights apply to the two-level preconditioner only");
+    return DRAPE_E_INVALID;
+  }
+  c->rules.preconditioner = kind;
+  return DRAPE_OK;
+}
+
+int drape_two_level_layout(drape_ctx* p, int32_t* coarse_rows, int32_t* parent, float* bilinear, int64_t* counters) {
+  Context* c = ctx_of(p);
+  if (!c) { set_error("null context"); return DRAPE_E_INVALID; }
+  int64_t f = 0, failed = 0;
+  double ms = 0.0;
+  TRY(two_level_info(*c, coarse_rows, parent, bilinear, &f, &failed, &ms));
+  if (counters) {
+    counters[0] = f;
+    counters[1] = failed;
+    counters[2] = c->fallbacks;
+    counters[3] = (int64_t)(1000.0 * ms);
+  }
+  return DRAPE_OK;
+}
+
+int drape_two_level_coarse(drape_ctx* p, double* coarse) {
+  Context* c = ctx_of(p);
+  if (!c || !coarse) { set_error("null argument"); return DRAPE_E_INVALID; }
+  if (!c->two_level) { set_error("no two-level layout on this context"); return DRAPE_E_INVALID; }
+  return two_level_galerkin(*c, coarse);
 }
 
 int drape_pcg_timing(drape_ctx* p, double* spmv_ms, double* solve_ms, int32_t* spmv_launches) {

@@ -44,6 +44,7 @@ struct StepState {
 
 int write_mu(Run& run, double mu) {
   if (!run.rules.regularize) return DRAPE_OK;
+  note_shift(run.c, f32(mu), 0.0);
   return launch_set_float(run.c, run.c.mu, f32(mu));
 }
 
@@ -321,6 +322,7 @@ int attempt(Run& run, double h, float scale, int step, bool ready, Attempt* out)
   HIP_TRY(hipMemcpyAsync(c.pos, c.org, (size_t)c.V * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
   int rc = launch_ptc_prepare(c, c.org, h);
   if (rc != DRAPE_OK) return rc;
+  note_shift(c, 0.0f, 0.0);
   rc = launch_set_float(c, c.mu, 0.0f);
   if (rc != DRAPE_OK) return rc;
   if (!ready) {
@@ -347,6 +349,7 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
               std::vector<drape_ptc_step>& steps, std::vector<drape_ptc_attempt>& attempts,
               std::vector<drape_record>& records_out) {
   Run run(c);
+  const int64_t fallbacks0 = c.fallbacks;
   run.rules.newton_tolerance = c.rules.inner_tolerance;
   run.rules.newton_max_iterations = c.rules.inner_max_iterations;
   const drape_rules& r = c.rules;
@@ -451,7 +454,7 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
     tracked += run.phase_ms[i];
   }
   result->phase_ms[5] = 1000.0 * seconds - tracked;
-  result->fallbacks = 0;
+  result->fallbacks = c.fallbacks - fallbacks0;
   records_out = std::move(run.records);
   return DRAPE_OK;
 }

@@ -211,6 +211,7 @@ static int build_scene(const drape_scene* sc, SceneArrays& out) {
   p.wale = sc->wale; p.course = sc->course; p.coupling = sc->coupling; p.shear = sc->shear;
   p.gravity = sc->gravity;
   p.time_step = sc->time_step;
+  p.grid_columns = sc->columns;  // irregular sheets keep the grid's vertex numbering
   return DRAPE_OK;
 }
 

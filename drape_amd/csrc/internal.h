@@ -99,6 +99,11 @@ struct Context {
 
   bool curvature_dropped = false;
   TwoLevel* two_level = nullptr;
+  uint64_t system_version = 0;    // bumped by every assembly: the coarse operator follows it
+  uint64_t shift_version = 0;     // bumped when mu or the step size of M/h^2 changes
+  float shift_mu = 0.0f;
+  double shift_h = 0.0;
+  int64_t fallbacks = 0;          // two-level solves redone with block-Jacobi
 
   // timing of the last PCG solve
   double last_spmv_ms = 0, last_solve_ms = 0;
@@ -115,6 +120,12 @@ void set_error(const std::string& message);
       ::drape::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                    \
       return DRAPE_E_DEVICE;                                                                    \
     }                                                                                           \
+  } while (0)
+
+#define TRY_RC(expr)           \
+  do {                         \
+    int _rc = (expr);          \
+    if (_rc != DRAPE_OK) return _rc; \
   } while (0)
 
 // ---- kernel launchers (elements.hip) --------------------------------------------------------
@@ -139,8 +150,25 @@ int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats*
 
 // ---- two-level preconditioner (twolevel.hip) ------------------------------------------------
 void two_level_destroy(TwoLevel* t);
+int two_level_create(Context& c, int columns);           // coarse layout of a grid sheet, bilinear weights
+int two_level_set_weights(Context& c, const float* w);  // [4R] parent-slot weights (NULL: bilinear)
+int two_level_prepare(Context& c, bool* ok);            // Galerkin + factorisation when the system changed
+int two_level_apply(Context& c, float* partials, int guard);  // z = M^-1 r and partial (r.z, r.r)
+void two_level_observe(Context& c, int iterations, double relative_residual);
+int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
+int two_level_info(const Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
+                   int64_t* failures, double* factor_ms);
 
 // ---- helpers --------------------------------------------------------------------------------
 inline int blocks_for(int n, int per = WG) { return (n + per - 1) / per; }
+
+// Records the regularisation mu and the step size h of the mass term (h <= 0: unchanged); a change
+// makes the next two-level solve refactor its coarse operator.
+inline void note_shift(Context& c, float mu, double h) {
+  const bool changed = mu != c.shift_mu || (h > 0 && h != c.shift_h);
+  c.shift_mu = mu;
+  if (h > 0) c.shift_h = h;
+  if (changed) ++c.shift_version;
+}
 
 }  // namespace drape

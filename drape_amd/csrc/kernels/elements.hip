@@ -650,6 +650,7 @@ int launch_evaluate(Context& c, bool exact) {
 
 int launch_assemble(Context& c) {
   hipStream_t s = c.stream;
+  ++c.system_version;
   k_assemble_blocks<<<blocks_for(c.B), 256, 0, s>>>(c.bc_ptr, c.bc_idx, c.lhess, c.block_dst, c.B, c.sell_val);
   k_assemble_rhs<<<blocks_for(c.R), 256, 0, s>>>(c.fc_ptr, c.fc_idx, c.lgrad, c.R, c.b);
   HIP_TRY(hipGetLastError());
@@ -745,6 +746,7 @@ int launch_step_all(Context& c, float alpha) {
 }
 
 int launch_ptc_prepare(Context& c, const float4* base, double h) {
+  note_shift(c, c.shift_mu, h);
   const float h2f = (float)(h * h);
   const float fall = h2f * (float)(-c.problem.gravity);
   const int n = c.V > c.R ? c.V : c.R;

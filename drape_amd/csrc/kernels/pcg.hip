@@ -270,7 +270,9 @@ __global__ void __launch_bounds__(256) k_pcg_dir_spmv(SellView A, const float* _
 }
 
 // reduce.wgsl MODE 1 (alpha, breakdown) folded into update_dot.wgsl: x += alpha p, r -= alpha q,
-// z = D^-1 r, partial (r.z, r.r). The row's operands are loaded before the pq reduction.
+// z = D^-1 r, partial (r.z, r.r). The row's operands are loaded before the pq reduction. JZ = false (the
+// two-level preconditioner) stops after x and r: z and its dots come from the coarse-correction kernels.
+template <bool JZ>
 __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ partial_pq, int G, PcgState* st,
                                                     const float4* __restrict__ p, const float4* __restrict__ q,
                                                     const float* __restrict__ dinv, float4* x, float4* r, float4* z,
@@ -286,8 +288,10 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     qr = q[row];
     xr = x[row];
     rv = r[row];
+    if (JZ) {
 #pragma unroll
-    for (int e = 0; e < 9; ++e) d[e] = dinv[9 * row + e];
+      for (int e = 0; e < 9; ++e) d[e] = dinv[9 * row + e];
+    }
   }
   const float rzv = st->rz[it & 1];
   const float pq = wave_reduce_partials(partial_pq, G);
@@ -310,6 +314,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     rv.z = rv.z - alpha * qr.z;
     x[row] = xr;
     r[row] = rv;
+    if (!JZ) return;
     const float z0 = d[0] * rv.x + d[1] * rv.y + d[2] * rv.z;
     const float z1 = d[3] * rv.x + d[4] * rv.y + d[5] * rv.z;
     const float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
@@ -317,6 +322,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     rz = rv.x * z0 + rv.y * z1 + rv.z * z2;
     rr = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
   }
+  if (!JZ) return;
   const float2 sum = tree256x2(rz, rr, lds2);
   if (threadIdx.x == 0) {
     partial_rz[2 * blockIdx.x] = sum.x;
@@ -415,7 +421,8 @@ __global__ void __launch_bounds__(256) k_pcg_residual_final(const float* __restr
 
 static SellView view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm}; }
 
-int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
+// One PCG solve; `coarse` adds the two-level coarse correction to block-Jacobi (z = D^-1 r + P A_c^-1 P^T r).
+static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats, bool coarse) {
   const int G = c.G, R = c.R;
   float* part_setup = c.partials;
   float* part_pq = c.partials + 2 * G;
@@ -427,6 +434,7 @@ int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats*
   hipStream_t s = c.stream;
   HIP_TRY(hipEventRecord(c.ev0, s));
   k_pcg_setup<<<G, 256, 0, s>>>(A, c.diag_slot, c.b, c.dinv, c.x, c.r, c.z, c.p, part_setup, R);
+  if (coarse) TRY_RC(two_level_apply(c, part_setup, 2));
   k_pcg_start<<<1, 256, 0, s>>>(part_setup, G, c.state, tol2, above2, max_iterations);
   HIP_TRY(hipGetLastError());
   // p ping-pongs between c.p (even iterations) and c.p2 (odd). A chunk opens with a plain SpMV (p is
@@ -446,12 +454,18 @@ int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats*
         k_pcg_spmv<<<G, 256, 0, s>>>(A, pit, c.q, part_pq, c.state, R, 0);
       else
         k_pcg_dir_spmv<<<G, 256, 0, s>>>(A, part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pit, c.q, part_pq, R, it);
-      k_pcg_update<<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
+      if (coarse) {
+        k_pcg_update<false><<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
+        TRY_RC(two_level_apply(c, part_rz, 0));
+      } else {
+        k_pcg_update<true><<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
+      }
     }
     // direction update after the chunk's last iteration it - 1: p_it = z + beta p_{it-1} into pbuf[it & 1]
     k_pcg_direction<<<G, 256, 0, s>>>(part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pbuf[it & 1], R, it - 1);
     if (replace_every > 0) {
       k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 1);
+      if (coarse) TRY_RC(two_level_apply(c, part_res, 1));
       k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 1, it - 1);
     }
     HIP_TRY(hipGetLastError());
@@ -483,6 +497,33 @@ int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats*
   stats->refinements = 0;
   stats->fp64_relative_residual = NAN;
   return DRAPE_OK;
+}
+
+// The preconditioner the rules select. A two-level solve falls back to block-Jacobi when the coarse
+// operator is not positive definite, and on a breakdown either re-solves with block-Jacobi or reports
+// the breakdown (rules.fallback); every such event is counted.
+int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
+  if (c.rules.preconditioner == DRAPE_PRECOND_TWO_LEVEL && c.two_level) {
+    bool ok = false;
+    TRY_RC(two_level_prepare(c, &ok));
+    if (ok) {
+      TRY_RC(pcg_run(c, tolerance, max_iterations, stats, true));
+      if (stats->status != 3) {
+        two_level_observe(c, stats->iterations, stats->recursive_relative_residual);
+        return DRAPE_OK;
+      }
+      two_level_observe(c, -1, 0.0);
+      c.fallbacks++;
+      if (c.rules.fallback == 2 && stats->iterations > 0) {
+        stats->status = 4;
+        return DRAPE_OK;
+      }
+      if (c.rules.fallback != 1) return DRAPE_OK;
+    } else {
+      c.fallbacks++;
+    }
+  }
+  return pcg_run(c, tolerance, max_iterations, stats, false);
 }
 
 // The SpMV alone, `launches` times back to back between two events on the context's stream (p -> q);

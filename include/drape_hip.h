@@ -48,6 +48,9 @@ typedef struct {
   double wale, course, coupling, shear; /* membrane constants, N/m (material.rs:4-26) */
   double gravity;                  /* m/s^2 */
   double time_step;                /* h of M/h^2 for dynamic steps (PTC rewrites it per attempt) */
+  int32_t grid_columns;            /* vertices per side of a sheet numbered as a grid (vertex = row * columns +
+                                      column; positions may be jittered), 0 otherwise; the two-level
+                                      preconditioner needs it */
 } drape_problem;
 
 /* The rule constants problem.json carries (e3_generate.rs:195-229, e4_rules.rs:391-396, e7_run.rs:28-33)
@@ -72,13 +75,26 @@ typedef struct {
   int32_t exact_tangent;  /* 1 exact membrane+hinge tangent, 0 projected / Gauss-Newton */
   int32_t refine_fp64;    /* 1: FP32 PCG inside an FP64 true-residual refinement loop */
   int32_t preconditioner; /* DRAPE_PRECOND_* */
+  int32_t coarse_reuse;   /* two-level: 0 refactors the coarse operator for every system; 1 keeps it while mu
+                             and the step size are unchanged and PCG needs at most twice the iterations per
+                             decade of the first solve after it; 2 keeps it across mu and step-size changes too
+                             (refactoring only on such a slowdown or after a fallback) */
+  int32_t fallback;       /* two-level breakdown (p^T A p <= 0): 1 re-solves with block-Jacobi, 0 reports the
+                             breakdown to the Newton loop, which raises mu; 2 returns the last iterate before
+                             the negative curvature (status 4) when there is one, else reports the breakdown.
+                             A coarse operator that is not positive definite always falls back to
+                             block-Jacobi. Every such event counts as a fallback */
+  double coarse_shift;    /* two-level: the coarse operator's diagonal is scaled by (1 + coarse_shift) before
+                             the factorisation, capping the coarse correction of modes whose curvature is
+                             below what FP32 resolves of the stiff membrane */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0
 #define DRAPE_PRECOND_TWO_LEVEL 1
 
 /* PCG outcome of one solve (solver.ts:33-43): status 0 running, 1 converged, 2 iteration limit,
- * 3 breakdown (state.wgsl:2-3). */
+ * 3 breakdown (state.wgsl:2-3); 4 (two-level only, rules.fallback = 2) truncated at negative curvature:
+ * the solution is the last iterate before it. */
 typedef struct {
   int32_t status;
   int32_t iterations;
@@ -197,6 +213,23 @@ int drape_get_system(drape_ctx* ctx, float* rhs, float* vals);
 /* A context over a plain BSR system (the E1 cases, generate.rs:330-339): drape_pcg then solves it. */
 int drape_linear_create(int32_t rows, int32_t blocks, const uint32_t* row_ptr, const uint32_t* col_idx,
                         const float* vals, const float* rhs, const drape_rules* rules, drape_ctx** out);
+
+/* Selects the preconditioner of later solves (DRAPE_PRECOND_*). DRAPE_PRECOND_TWO_LEVEL adds a Galerkin
+ * coarse correction on every fourth grid vertex to block-Jacobi; it needs problem.grid_columns with
+ * (grid_columns - 1) divisible by 4. `weights` [4R] are the prolongation weights of the four parent slots
+ * of each row (NULL: bilinear); slots without a parent ignore theirs. No reference counterpart: the
+ * reference's solver is block-Jacobi only (solver.ts:109-191). */
+int drape_set_preconditioner(drape_ctx* ctx, int32_t kind, const float* weights);
+/* The coarse layout of the two-level preconditioner: coarse rows, parent [4R] (coarse row of each parent
+ * slot, -1 if none) and the bilinear weights [4R]; counters [4] = factorisations, failed factorisations
+ * (A_c not positive definite), solves that fell back to block-Jacobi, microseconds spent factorising.
+ * Any output may be NULL. */
+int drape_two_level_layout(drape_ctx* ctx, int32_t* coarse_rows, int32_t* parent, float* bilinear,
+                           int64_t* counters);
+
+/* The Galerkin coarse operator A_c = P^T A P of the assembled system, row-major [3Nc x 3Nc] doubles (Nc from
+ * drape_two_level_layout), for checking; the preconditioner's factorisation is left as it is. */
+int drape_two_level_coarse(drape_ctx* ctx, double* coarse);
 
 /* Device timing of the last drape_pcg (HIP events on the context's stream): average SpMV kernel time
  * and the whole solve, in milliseconds. */

@@ -1,0 +1,143 @@
+"""Oracle for the two-level preconditioner: a numpy f64 statement of the coarse layout, the Galerkin
+coarse operator and two-level PCG. Test infrastructure only: only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may load it, and only as the checker.
+
+The reference has no two-level preconditioner (its PCG is block-Jacobi only, engine/oracle/src/pcg.rs
+and engine/gpu/src/solver.ts:109-191); the nearest code is the bilinear coarse-to-fine prolongation of
+positions in engine/oracle/src/e7_prolong.rs:9-22. There is therefore no golden vector: every function
+here is pinned by mathematics instead (tests/test_two_level_oracle.py):
+  - the layout reproduces bilinear fields exactly and injects at coarse vertices (closed form);
+  - weights of a row whose parents are all free sum to one (partition of unity);
+  - A_c = P^T A P equals the dense triple product and is SPD for SPD A with full-rank P;
+  - two-level PCG converges to the direct solution and in exact arithmetic never takes more iterations
+    than the dimension; with a zero coarse space it is block-Jacobi PCG step for step.
+
+Layout (shared with the HIP library): coarse vertices are every fourth grid vertex in each direction of a
+sheet with `columns` vertices per side ((columns - 1) % 4 == 0); coarse rows are the free coarse vertices
+in vertex order; each fine row has four parent slots k = 0..3 = the corners (a0, b0), (a0 + 1, b0),
+(a0, b0 + 1), (a0 + 1, b0 + 1) of its coarse cell with a0 = min(x // 4, nc - 2), weights
+(1 - fx)(1 - fy), fx (1 - fy), (1 - fx) fy, fx fy; a slot whose corner is pinned, or whose weight is 0,
+is empty (parent -1, weight 0).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+
+def coarse_layout(columns: int, row_vertex: np.ndarray):
+    """(Nc, parent [R, 4] int32, weights [R, 4] float64) of the stride-4 coarse grid."""
+    n = int(columns)
+    if n < 9 or (n - 1) % 4:
+        raise ValueError("the sheet needs (columns - 1) divisible by 4")
+    nc = (n - 1) // 4 + 1
+    row_vertex = np.asarray(row_vertex, np.int64)
+    free = np.zeros(n * n, bool)
+    free[row_vertex] = True
+    coarse_of = -np.ones((nc, nc), np.int64)
+    count = 0
+    for b in range(nc):
+        for a in range(nc):
+            if free[(4 * b) * n + 4 * a]:
+                coarse_of[b, a] = count
+                count += 1
+    R = len(row_vertex)
+    parent = -np.ones((R, 4), np.int32)
+    weights = np.zeros((R, 4), np.float64)
+    for row, v in enumerate(row_vertex):
+        x, y = int(v) % n, int(v) // n
+        a0, b0 = min(x // 4, nc - 2), min(y // 4, nc - 2)
+        fx, fy = (x - 4.0 * a0) / 4.0, (y - 4.0 * b0) / 4.0
+        corners = [(a0, b0), (a0 + 1, b0), (a0, b0 + 1), (a0 + 1, b0 + 1)]
+        wts = [(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy]
+        for k, ((a, b), w) in enumerate(zip(corners, wts)):
+            I = coarse_of[b, a]
+            if I < 0 or w == 0.0:
+                continue
+            parent[row, k] = I
+            weights[row, k] = w
+    return count, parent, weights
+
+
+def prolongation(parent: np.ndarray, weights: np.ndarray, coarse_rows: int) -> sp.csr_matrix:
+    """P = W (x) I3 as a (3R x 3Nc) sparse matrix."""
+    R = parent.shape[0]
+    rows, cols, vals = [], [], []
+    for i in range(R):
+        for k in range(4):
+            I = parent[i, k]
+            if I < 0:
+                continue
+            for d in range(3):
+                rows.append(3 * i + d)
+                cols.append(3 * I + d)
+                vals.append(float(weights[i, k]))
+    return sp.csr_matrix((vals, (rows, cols)), shape=(3 * R, 3 * coarse_rows))
+
+
+def bsr_matrix(row_ptr, col_idx, vals) -> sp.csr_matrix:
+    """The 3x3 BSR system (vals [9B], row-major blocks) as a scalar CSR matrix in f64."""
+    R = len(row_ptr) - 1
+    data = np.asarray(vals, np.float64).reshape(-1, 3, 3)
+    return sp.bsr_matrix((data, np.asarray(col_idx, np.int64), np.asarray(row_ptr, np.int64)),
+                         shape=(3 * R, 3 * R)).tocsr()
+
+
+def galerkin(A: sp.spmatrix, P: sp.spmatrix) -> np.ndarray:
+    return (P.T @ (A @ P)).toarray()
+
+
+def block_jacobi_inverse(A: sp.csr_matrix) -> sp.csr_matrix:
+    R = A.shape[0] // 3
+    blocks = np.zeros((R, 3, 3))
+    Ad = A.tocsr()
+    for i in range(R):
+        blocks[i] = Ad[3 * i:3 * i + 3, 3 * i:3 * i + 3].toarray()
+    inv = np.linalg.inv(blocks)
+    return sp.bsr_matrix((inv, np.arange(R), np.arange(R + 1)), shape=A.shape).tocsr()
+
+
+def two_level_pcg(A: sp.csr_matrix, b: np.ndarray, P: sp.spmatrix | None, tolerance: float,
+                  max_iterations: int = 20000):
+    """f64 PCG with M^-1 = D^-1 + P A_c^-1 P^T (block-Jacobi alone when P is None), x0 = 0, stop when
+    |r|^2 <= tol^2 |r0|^2 (the recursive residual, pcg.rs:247-259 order of tests), breakdown when
+    p^T A p <= 0 or alpha is not finite. Returns (x, iterations, status, history of |r|/|r0|)."""
+    Dinv = block_jacobi_inverse(A)
+    if P is not None:
+        Ac = galerkin(A, P)
+        L = np.linalg.cholesky(Ac)
+        Linv = np.linalg.inv(L)
+        Minv_c = Linv.T @ Linv
+
+        def apply(r):
+            return Dinv @ r + P @ (Minv_c @ (P.T @ r))
+    else:
+        def apply(r):
+            return Dinv @ r
+    b = np.asarray(b, np.float64)
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = apply(r)
+    p = z.copy()
+    rz = r @ z
+    rr0 = r @ r
+    history = [1.0]
+    if rr0 <= 0:
+        return x, 0, 1, history
+    for it in range(1, max_iterations + 1):
+        q = A @ p
+        pq = p @ q
+        alpha = rz / pq if pq != 0 else np.inf
+        if not (pq > 0 and np.isfinite(alpha)):
+            return x, it - 1, 3, history
+        x = x + alpha * p
+        r = r - alpha * q
+        z = apply(r)
+        rz_new = r @ z
+        rr = r @ r
+        history.append(float(np.sqrt(rr / rr0)))
+        if rr <= tolerance * tolerance * rr0:
+            return x, it, 1, history
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, max_iterations, 2, history

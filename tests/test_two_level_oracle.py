@@ -1,0 +1,105 @@
+"""Pins of the two-level preconditioner's oracle (oracle/two_level.py). The preconditioner has no reference
+counterpart and hence no golden vector; each check below is fixed by mathematics, chosen so that a wrong
+corner order, a transposed operand or a dropped term fails one of them."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle_lib as O
+from oracle import two_level as TL
+
+
+def grid_rows(n, pinned):
+    rows = np.arange(n * n)
+    if pinned:  # the reference's pinned top corners (drape.rs:148)
+        rows = rows[(rows != (n - 1) * n) & (rows != n * n - 1)]
+    return rows
+
+
+@pytest.mark.parametrize("n,pinned", [(9, False), (17, True), (33, False), (65, True)])
+def test_layout_reproduces_bilinear_fields_and_injects(n, pinned):
+    rows = grid_rows(n, pinned)
+    nc_count, parent, w = TL.coarse_layout(n, rows)
+    nc = (n - 1) // 4 + 1
+    assert nc_count == nc * nc - (2 if pinned else 0)
+    # coarse coordinates of each coarse row
+    coarse_xy = []
+    for b in range(nc):
+        for a in range(nc):
+            v = 4 * b * n + 4 * a
+            if v in set(rows.tolist()):
+                coarse_xy.append((4 * a, 4 * b))
+    coarse_xy = np.array(coarse_xy, np.float64)
+    P = TL.prolongation(parent, w, nc_count)
+    fine_xy = np.stack([rows % n, rows // n], 1).astype(np.float64)
+    # rows whose coarse cell has four free corners
+    pinned_corners = {(0, nc - 1), (nc - 1, nc - 1)} if pinned else set()
+    a0 = np.minimum(fine_xy[:, 0] // 4, nc - 2).astype(int)
+    b0 = np.minimum(fine_xy[:, 1] // 4, nc - 2).astype(int)
+    complete = np.array([not ({(a, b), (a + 1, b), (a, b + 1), (a + 1, b + 1)} & pinned_corners)
+                         for a, b in zip(a0, b0)])
+    for f in (lambda x, y: 2.0 * x - 0.5 * y + 3.0, lambda x, y: x * y, lambda x, y: 1.0 + 0 * x):
+        fc = f(coarse_xy[:, 0], coarse_xy[:, 1])
+        got = P @ np.repeat(fc, 3)
+        want = np.repeat(f(fine_xy[:, 0], fine_xy[:, 1]), 3)
+        mask = np.repeat(complete, 3)
+        assert np.allclose(got[mask], want[mask], rtol=0, atol=1e-12)
+    # partition of unity where no parent is pinned; injection at coarse vertices
+    assert np.allclose(w[complete].sum(1), 1.0)
+    on_coarse = (fine_xy[:, 0] % 4 == 0) & (fine_xy[:, 1] % 4 == 0)
+    assert np.all(np.sort(w[on_coarse], 1)[:, -1] == 1.0)
+    assert np.all((w > 0) == (parent >= 0))
+
+
+def scene_system(n, mu=0.0, bent=False, h=1.0 / 240.0):
+    """The pinned sheet's Newton system at rest (SPD: K(rest) + M/h^2) or at the seed-97 bent state of
+    e3_generate.rs:164-166 (indefinite where the exact tangent is), with a sagged x_hat."""
+    od = O.Drape(O.reference_scene(n, sphere=False, placed=True))
+    a = od.problem_arrays()
+    rest = od.rest_positions()
+    x = a["kernel_state"].astype(np.float64) if bent else rest
+    x_hat = rest.copy()
+    x_hat[2::3] -= 1e-3
+    od.set_time_step(h)
+    rhs, vals = od.assemble(x, x_hat, mu, True)
+    A = TL.bsr_matrix(a["row_ptr"], a["col_idx"], vals)
+    return od, a, A, rhs, vals
+
+
+def test_galerkin_is_the_triple_product_and_spd():
+    od, a, A, rhs, _ = scene_system(17, mu=1.0, bent=True)
+    nc, parent, w = TL.coarse_layout(17, a["row_vertex"])
+    P = TL.prolongation(parent, w, nc)
+    Ac = TL.galerkin(A, P)
+    Pd, Ad = P.toarray(), A.toarray()
+    assert np.allclose(Ac, Pd.T @ Ad @ Pd, rtol=1e-12, atol=1e-12 * np.abs(Ac).max())
+    assert np.allclose(Ac, Ac.T, rtol=1e-10, atol=1e-10 * np.abs(Ac).max())
+    assert np.linalg.eigvalsh(0.5 * (Ac + Ac.T)).min() > 0
+    # a transposed prolongation would not even conform; a dropped weight breaks the triple product
+    w2 = w.copy()
+    w2[5, :] = 0.0
+    Ac2 = TL.galerkin(A, TL.prolongation(parent, w2, nc))
+    assert not np.allclose(Ac2, Ac)
+
+
+def test_two_level_pcg_solves_and_beats_block_jacobi():
+    od, a, A, rhs, _ = scene_system(33, h=1.0)
+    nc, parent, w = TL.coarse_layout(33, a["row_vertex"])
+    P = TL.prolongation(parent, w, nc)
+    direct = spla.spsolve(A.tocsc(), rhs)
+    x2, it2, st2, _ = TL.two_level_pcg(A, rhs, P, 1e-10, 5000)
+    x1, it1, st1, _ = TL.two_level_pcg(A, rhs, None, 1e-10, 20000)
+    assert st2 == 1 and st1 == 1
+    assert np.linalg.norm(x2 - direct) <= 1e-6 * np.linalg.norm(direct)
+    assert it2 * 3 < it1, (it2, it1)
+
+
+def test_without_coarse_space_it_is_the_block_jacobi_pcg_of_the_oracle():
+    """P = None is block-Jacobi PCG: iteration count and solution agree with the C++ restatement of
+    pcg.rs:60-281 in f64 (a different reduction order moves only the last bits)."""
+    od, a, A, rhs, vals = scene_system(17, h=1.0)
+    x1, it1, st1, _ = TL.two_level_pcg(A, rhs, None, 1e-8, 5000)
+    xo, so, _ = O.pcg(a["row_ptr"].astype(np.int32), a["col_idx"].astype(np.int32), vals, rhs, 1e-8, 5000, 0, 0.0,
+                      single=False)
+    assert abs(it1 - so.iterations) <= 1
+    assert np.linalg.norm(x1 - xo) <= 1e-7 * np.linalg.norm(xo)
