@@ -137,6 +137,7 @@ def lib():
         L.drape_set_preconditioner.argtypes = [vp, C.c_int32, f32p]
         L.drape_two_level_layout.argtypes = [vp, i32p, i32p, f32p, C.POINTER(C.c_int64)]
         L.drape_two_level_coarse.argtypes = [vp, f64p]
+        L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_problem.restype = C.POINTER(Problem)
         L.drape_scene_problem.argtypes = [vp, f64p]
@@ -362,6 +363,13 @@ class Drape:
         nc = self.two_level_layout()[0]
         out = np.zeros((3 * nc, 3 * nc))
         _check(lib().drape_two_level_coarse(self.h, _ptr(out, f64p)))
+        return out
+
+    def two_level_inverse_factor(self):
+        """B = L^-1 of the current coarse factorisation (FP32, dense lower triangular)."""
+        nc = self.two_level_layout()[0]
+        out = np.zeros((3 * nc, 3 * nc), np.float32)
+        _check(lib().drape_two_level_inverse_factor(self.h, _ptr(out, f32p)))
         return out
 
     # ---- the step and its caller ----

@@ -41,10 +41,7 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
-    # rocSOLVER factorises the two-level preconditioner's small dense coarse matrix once per system
-    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib,
-           "-lrocsolver", "-lrocblas", "-o", tmp]
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

@@ -635,6 +635,13 @@ int drape_two_level_coarse(drape_ctx* p, double* coarse) {
   return two_level_galerkin(*c, coarse);
 }
 
+int drape_two_level_inverse_factor(drape_ctx* p, float* factor) {
+  Context* c = ctx_of(p);
+  if (!c || !factor) { set_error("null argument"); return DRAPE_E_INVALID; }
+  if (!c->two_level) { set_error("no two-level layout on this context"); return DRAPE_E_INVALID; }
+  return two_level_inverse_factor(*c, factor);
+}
+
 int drape_pcg_timing(drape_ctx* p, double* spmv_ms, double* solve_ms, int32_t* spmv_launches) {
   Context* c = ctx_of(p);
   if (!c) { set_error("null context"); return DRAPE_E_INVALID; }

@@ -21,6 +21,7 @@ struct PcgState {
   float rz[2];
   float pq, alpha, beta, rr, rr0, tol2, true_rr, above2;
   int32_t iteration, status, max_iterations, replace_pending;
+  float diverge2;  // rr > diverge2 rr0 is a breakdown (two-level solves; +inf otherwise)
 };
 
 // Two-level preconditioner data (coarse space) -- see twolevel.hip.
@@ -156,6 +157,7 @@ int two_level_prepare(Context& c, bool* ok);            // Galerkin + factorisat
 int two_level_apply(Context& c, float* partials, int guard);  // z = M^-1 r and partial (r.z, r.r)
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
+int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host
 int two_level_info(const Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
                    int64_t* failures, double* factor_ms);
 

@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) k_pcg_setup(SellView A, const uint32_t* _
 
 // reduce.wgsl MODE 0 (start).
 __global__ void __launch_bounds__(256) k_pcg_start(const float* __restrict__ partials, int G, PcgState* st,
-                                                   float tol2, float above2, int max_iterations) {
+                                                   float tol2, float above2, int max_iterations, float diverge2) {
   __shared__ float2 lds[256];
   const float2 s = reduce_partials2(partials, G, lds);
   if (threadIdx.x == 0) {
@@ -197,6 +197,7 @@ __global__ void __launch_bounds__(256) k_pcg_start(const float* __restrict__ par
     st->rr0 = s.y;
     st->tol2 = tol2;
     st->above2 = above2;
+    st->diverge2 = diverge2;
     st->max_iterations = max_iterations;
     st->iteration = 0;
     st->status = s.y <= 0.0f ? 1 : 0;
@@ -242,6 +243,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir_spmv(SellView A, const float* _
   int status = 0;
   if (rr <= st->tol2 * st->rr0) status = 1;
   else if (it >= st->max_iterations) status = 2;
+  else if (rr > st->diverge2 * st->rr0) status = 3;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rz[it & 1] = rz_new;
     st->rr = rr;
@@ -349,6 +351,7 @@ __global__ void __launch_bounds__(256) k_pcg_direction(const float* __restrict__
   int status = 0;
   if (rr <= st->tol2 * st->rr0) status = 1;
   else if (it + 1 >= st->max_iterations) status = 2;
+  else if (rr > st->diverge2 * st->rr0) status = 3;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rz[(it + 1) & 1] = rz_new;
     st->rr = rr;
@@ -435,7 +438,9 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
   HIP_TRY(hipEventRecord(c.ev0, s));
   k_pcg_setup<<<G, 256, 0, s>>>(A, c.diag_slot, c.b, c.dinv, c.x, c.r, c.z, c.p, part_setup, R);
   if (coarse) TRY_RC(two_level_apply(c, part_setup, 2));
-  k_pcg_start<<<1, 256, 0, s>>>(part_setup, G, c.state, tol2, above2, max_iterations);
+  // a two-level solve whose residual grows 1e4-fold has met curvature FP32 cannot resolve: a breakdown
+  const float diverge2 = coarse ? 1e8f : INFINITY;
+  k_pcg_start<<<1, 256, 0, s>>>(part_setup, G, c.state, tol2, above2, max_iterations, diverge2);
   HIP_TRY(hipGetLastError());
   // p ping-pongs between c.p (even iterations) and c.p2 (odd). A chunk opens with a plain SpMV (p is
   // current), fuses each later direction update into its SpMV, and closes with an explicit direction
@@ -514,7 +519,8 @@ int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats*
       }
       two_level_observe(c, -1, 0.0);
       c.fallbacks++;
-      if (c.rules.fallback == 2 && stats->iterations > 0) {
+      // truncation returns the last iterate before the negative curvature, if it improved on x = 0
+      if (c.rules.fallback == 2 && stats->iterations > 0 && stats->recursive_relative_residual < 1.0) {
         stats->status = 4;
         return DRAPE_OK;
       }

@@ -6,19 +6,22 @@
 // direction (1/16 of the vertices); P = W (x) I3 with four parent slots per fine row whose weights are
 // bilinear by default or set by the learned prolongation network.
 //
-// Device work per factorisation: the Galerkin product (one thread per coarse 3x3 block, contributions in a
-// fixed order, FP64 accumulation), the FP64 Cholesky factor A_c = L L^T and its inverse B = L^-1 (rocSOLVER
-// potrf/trtri: 3Nc <= ~3300 unknowns), and FP32 copies of B and B^T. The coarse solve is applied as
-// B^T (B r_c): a Gram product, so the FP32 operator stays positive semidefinite whatever the rounding of B
-// (an FP32 copy of A_c^-1 itself turns indefinite once cond(A_c) nears 1e7). Per PCG iteration: restriction
-// (one wavefront per coarse node), two triangular GEMVs (one wavefront per row, float4 loads, fixed shuffle
-// reduction) and the prolongation fused with the block-Jacobi term and the (r.z, r.r) partial sums.
-// Everything is deterministic. The factorisation is kept across Newton systems while mu and the step size
-// are unchanged and PCG stays within twice the iterations per decade of the first solve after it: any
-// fixed SPD operator is a valid preconditioner, and a stale Galerkin operator of a nearby system is one.
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
+// Device work per factorisation, all hand-written and deterministic:
+//  - the Galerkin product into banded tile storage (one thread per coarse 3x3 block pair, FP64, its terms
+//    in a fixed order); coarse vertices are numbered row by row, so A_c is banded (half-bandwidth about
+//    6 coarse columns of unknowns) and its Cholesky factor keeps that band;
+//  - a blocked band Cholesky A_c = L L^T in FP64 on 64 x 64 tiles: per tile column, one workgroup factors
+//    the diagonal tile in LDS by 16-column panels and inverts it blockwise, workgroups of 16 rows form the
+//    sub-diagonal factor tiles as A_ik L_kk^-T, and workgroups of 16 rows apply the rank-64 trailing update;
+//  - B = L^-1 (dense lower triangular) by blocked forward substitution, one workgroup per 16 columns, with
+//    FP32 copies of B and B^T written directly.
+// The coarse solve is applied as B^T (B r_c): a Gram product, so the FP32 operator stays positive
+// semidefinite whatever the rounding of B (an FP32 copy of A_c^-1 itself turns indefinite once cond(A_c)
+// nears 1e7). Per PCG iteration: restriction (one wavefront per coarse node), two triangular GEMVs (one
+// wavefront per row, float4 loads, fixed shuffle reduction) and the prolongation fused with the block-Jacobi
+// term and the (r.z, r.r) partial sums. The factorisation is kept across Newton systems while mu and the
+// step size are unchanged and PCG stays within twice the iterations per decade of the first solve after it:
+// any fixed SPD operator is a valid preconditioner, and a stale Galerkin operator of a nearby system is one.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -51,7 +54,12 @@ struct TwoLevel {
   uint32_t* d_pair_ptr = nullptr;   // [npairs+1]
   uint32_t* d_pair_key = nullptr;   // [npairs] I * Nc + J
   uint4* d_pair_terms = nullptr;    // (block b, slot ik, slot jl, 0)
-  double* d_Ac = nullptr;           // [N3 * N3] column-major; L, then L^-1 in the lower triangle
+  int nt = 0;                       // 64-row tiles of the coarse matrix (padded with identity rows)
+  int kb = 0;                       // sub-diagonal tiles in the band
+  double* d_band = nullptr;         // [nt][kb + 1][64 x 64]: tile (k + d, k) of A_c, then of L, row-major
+  double* d_linv = nullptr;         // [nt][64 x 64]: inverse of each diagonal tile of L
+  double* d_lpanel = nullptr;       // [nt][kb][64 x 64]: tile (k + d, k) of L, d = 1..kb
+  double* d_Ac = nullptr;           // [N3 * N3] dense A_c for the check entry point (allocated on demand)
   float* d_B = nullptr;             // [N3 * N3p] row-major B = L^-1 (zero above the diagonal)
   float* d_BT = nullptr;            // [N3 * N3p] row-major B^T (zero below the diagonal)
   float* d_t = nullptr;             // [N3p] B r_c
@@ -59,31 +67,440 @@ struct TwoLevel {
   float* d_yc = nullptr;            // [N3p]
   int* d_info = nullptr;
   int* h_info = nullptr;
-  rocblas_handle handle = nullptr;
   int64_t factorisations = 0, failures = 0;
-  double factor_ms = 0.0;     // wall time of all factorisations (Galerkin product, potrf, trtri, copies)
+  double factor_ms = 0.0;     // wall time of all factorisations (Galerkin product, Cholesky, inverse)
 };
 
 void two_level_destroy(TwoLevel* t) {
   if (!t) return;
   for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
-                  (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT,
+                  (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT,
                   (void*)t->d_t, (void*)t->d_rc, (void*)t->d_yc, (void*)t->d_info})
     if (p) (void)hipFree(p);
   if (t->h_info) (void)hipHostFree(t->h_info);
-  if (t->handle) rocblas_destroy_handle(t->handle);
   delete t;
 }
 
 // ---- kernels ----------------------------------------------------------------------------------
-// A_c(I, J) = sum over fine blocks b = (i, j) and parent slots k of i, l of j with parent I, J of
-// w_ik w_jl A_b, in the fixed order of the pair's term list.
-__global__ void __launch_bounds__(256) k_galerkin(const uint32_t* __restrict__ pair_ptr,
-                                                  const uint32_t* __restrict__ pair_key,
-                                                  const uint4* __restrict__ terms, const float* __restrict__ w,
-                                                  const float* __restrict__ sell_val,
-                                                  const uint32_t* __restrict__ block_dst, int npairs, int Nc,
-                                                  double* __restrict__ Ac) {
+// ---- banded FP64 Cholesky and inverse on 64 x 64 tiles ----------------------------------------------
+constexpr int TILE = 64;
+constexpr int MAX_KB = 8;   // sub-diagonal tiles: coarse grids up to 83 vertices a side
+constexpr int LSTR = 65;    // LDS row stride of column-accessed tiles (bank spread)
+
+__device__ inline double* band_tile(double* band, int kb, int k, int d) {
+  return band + (((size_t)k * (kb + 1) + d) << 12);
+}
+
+// Entry q of the coarse block pair (I, J) into the band where it lies on or below the diagonal: one thread
+// per (pair, entry), the pair's terms in their fixed order.
+__global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restrict__ pair_ptr,
+                                                       const uint32_t* __restrict__ pair_key,
+                                                       const uint4* __restrict__ terms, const float* __restrict__ w,
+                                                       const float* __restrict__ sell_val,
+                                                       const uint32_t* __restrict__ block_dst, int npairs, int Nc,
+                                                       int kb, double* __restrict__ band) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= 9 * npairs) return;
+  const int p = g / 9, q = g % 9;
+  const uint32_t key = pair_key[p];
+  const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
+  const int R = 3 * I + q / 3, C = 3 * J + q % 3;
+  if (R < C) return;
+  double acc = 0.0;
+  const uint32_t t1 = pair_ptr[p + 1];
+  for (uint32_t t = pair_ptr[p]; t < t1; ++t) {
+    const uint4 e = terms[t];
+    acc += (double)w[e.y] * (double)w[e.z] * (double)sell_val[block_dst[e.x] + 64 * q];
+  }
+  const int k = C >> 6, d = (R >> 6) - k;
+  band_tile(band, kb, k, d)[((R & 63) << 6) + (C & 63)] = acc;
+}
+
+// Diagonal of A_c scaled by `factor`; the padding rows past N3 get a unit diagonal.
+__global__ void k_band_diagonal(double* __restrict__ band, int kb, int nt, int N3, double factor) {
+  const int R = blockIdx.x * blockDim.x + threadIdx.x;
+  if (R >= nt * TILE) return;
+  double* a = band_tile(band, kb, R >> 6, 0) + ((R & 63) << 6) + (R & 63);
+  *a = R < N3 ? *a * factor : 1.0;
+}
+
+// LDS traffic between the lanes of one wavefront: LDS instructions of a wavefront execute in order, so a
+// wavefront-scope fence (no instruction, an ordering constraint on the compiler) keeps a store before the
+// loads that read it back.
+__device__ inline void wave_order() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+// 1/sqrt(x) to double precision: the hardware estimate refined by two Newton steps.
+__device__ inline double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y;
+}
+
+// Diagonal tile of tile column k, blocked by 16-column panels. Wavefront 0 factors each panel four columns
+// at a time: the 4 x 4 diagonal block is broadcast through LDS and factored redundantly by every lane, each
+// lane (= row) solves its four factor entries and broadcasts them, and the lanes apply the rank-4 update to
+// the rest of their panel row. All four wavefronts then apply the panel's rank-16 update to the trailing
+// lower triangle. The tile's inverse follows blockwise: the four 16 x 16 diagonal blocks in parallel, then
+// the off-diagonal blocks by distance from the diagonal, X_ab = -X_aa sum_c L_ac X_cb. A pivot that is not
+// positive (or not finite) records 64 k + column + 1 in *info and stops every later kernel.
+__global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ band, double* __restrict__ linv, int k,
+                                                    int kb, int* info) {
+  __shared__ double Ls[TILE * LSTR];
+  __shared__ double Xs[TILE * LSTR];
+  __shared__ double Ys[3 * 16 * 17];
+  __shared__ double blk[16];
+  __shared__ double colbuf[4 * TILE];
+  __shared__ double rdiag[TILE];
+  __shared__ int fail;
+  if (*info != 0) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double* tile = band_tile(band, kb, k, 0);
+  for (int e = tid; e < TILE * TILE; e += 256) {
+    Ls[(e >> 6) * LSTR + (e & 63)] = tile[e];
+    Xs[(e >> 6) * LSTR + (e & 63)] = 0.0;
+  }
+  if (tid == 0) fail = 0;
+  __syncthreads();
+  for (int b = 0; b < 4; ++b) {
+    const int c0 = 16 * b;
+    if (wave == 0) {
+      const int row = c0 + lane;
+      const bool act = row < TILE;
+      double p[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) p[j] = act ? Ls[row * LSTR + c0 + j] : 0.0;
+      int bad = 0;
+#pragma unroll
+      for (int q = 0; q < 16; q += 4) {
+        if (bad == 0) {
+          // the 4 x 4 diagonal block (rows and columns q .. q + 3 of the panel), lower triangle
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (lane == q + u) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) blk[u * 4 + v] = p[q + v];
+            }
+          wave_order();
+          double d[4][4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) d[u][v] = blk[u * 4 + v];
+          // factor it redundantly: l[u][v], u >= v, and the reciprocal diagonal
+          double l[4][4] = {}, rd[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double piv = d[v][v];
+#pragma unroll
+            for (int m = 0; m < v; ++m) piv -= l[v][m] * l[v][m];
+            if (!(piv > 0.0) || !(piv < INFINITY)) {
+              if (bad == 0) bad = c0 + q + v + 1;
+              piv = 1.0;
+            }
+            rd[v] = rsqrt_f64(piv);
+            l[v][v] = piv * rd[v];
+#pragma unroll
+            for (int u = v + 1; u < 4; ++u) {
+              double a = d[u][v];
+#pragma unroll
+              for (int m = 0; m < v; ++m) a -= l[u][m] * l[v][m];
+              l[u][v] = a * rd[v];
+            }
+          }
+          // this lane's factor entries in columns q .. q + 3 (rows below the block), the block's own rows
+          double x[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double a = p[q + v];
+#pragma unroll
+            for (int m = 0; m < v; ++m) a -= x[m] * l[v][m];
+            x[v] = a * rd[v];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (lane == q + u) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) x[v] = u >= v ? l[u][v] : 0.0;
+            }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            p[q + v] = x[v];
+            colbuf[v * TILE + lane] = x[v];
+          }
+          if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) rdiag[c0 + q + v] = rd[v];
+          }
+          wave_order();
+          // rank-4 update of the rest of the panel row: p[j] -= sum_v x[v] L[c0 + j][c0 + q + v]
+#pragma unroll
+          for (int j = q + 4; j < 16; ++j) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) p[j] -= x[v] * colbuf[v * TILE + j];
+          }
+          wave_order();
+        }
+      }
+      if (act) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) Ls[row * LSTR + c0 + j] = (lane >= 16 || j <= lane) ? p[j] : 0.0;
+      }
+      if (lane == 0 && bad != 0) fail = bad;
+    }
+    __syncthreads();
+    if (fail != 0) {
+      if (tid == 0) *info = TILE * k + fail;
+      return;
+    }
+    // trailing update of rows and columns c0 + 16 .. 63: one 4 x 4 block of the lower triangle per thread
+    const int t4 = (TILE - c0 - 16) / 4;
+    if (tid < t4 * t4) {
+      const int bi = tid / t4, bj = tid % t4;
+      if (bj <= bi) {
+        const int i0 = c0 + 16 + 4 * bi, j0 = c0 + 16 + 4 * bj;
+        double o[4][4] = {};
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+          double li[4], lj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            li[u] = Ls[(i0 + u) * LSTR + c0 + m];
+            lj[u] = Ls[(j0 + u) * LSTR + c0 + m];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) o[u][v] += li[u] * lj[v];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Ls[(i0 + u) * LSTR + j0 + v] -= o[u][v];
+      }
+    }
+    __syncthreads();
+  }
+  // the factor back, zero above the diagonal
+  for (int e = tid; e < TILE * TILE; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    tile[e] = j <= i ? Ls[i * LSTR + j] : 0.0;
+  }
+  // inverse: diagonal 16 x 16 blocks, wavefront w inverts block w (lane j < 16 holds column j)
+  {
+    const int o = 16 * wave;
+    if (lane < 16) {
+      double x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        double acc = lane == i ? 1.0 : 0.0;
+#pragma unroll
+        for (int m = 0; m < i; ++m) acc -= Ls[(o + i) * LSTR + o + m] * x[m];
+        x[i] = acc * rdiag[o + i];
+        Xs[(o + i) * LSTR + o + lane] = x[i];
+      }
+    }
+  }
+  __syncthreads();
+  // off-diagonal blocks (a, b = a - dist): Y = sum_{c=b}^{a-1} L_ac X_cb, then X_ab = -X_aa Y
+  for (int dist = 1; dist < 4; ++dist) {
+    const int nblk = 4 - dist;
+    for (int q = tid; q < nblk * 256; q += 256) {
+      const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
+      const int a = dist + blk, b = blk;
+      double acc = 0.0;
+      for (int m = 16 * b; m < 16 * a; ++m) acc += Ls[(16 * a + i) * LSTR + m] * Xs[m * LSTR + 16 * b + j];
+      Ys[blk * 16 * 17 + i * 17 + j] = acc;
+    }
+    __syncthreads();
+    for (int q = tid; q < nblk * 256; q += 256) {
+      const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
+      const int a = dist + blk, b = blk;
+      double acc = 0.0;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) acc += Xs[(16 * a + i) * LSTR + 16 * a + m] * Ys[blk * 16 * 17 + m * 17 + j];
+      Xs[(16 * a + i) * LSTR + 16 * b + j] = -acc;
+    }
+    __syncthreads();
+  }
+  double* inv = linv + ((size_t)k << 12);
+  for (int e = tid; e < TILE * TILE; e += 256) inv[e] = Xs[(e >> 6) * LSTR + (e & 63)];
+}
+
+// Sub-diagonal factor tiles of tile column k: L(k + d, k) = A(k + d, k) L_kk^-T. Workgroup (d, quarter)
+// forms 16 rows of tile d.
+__global__ void __launch_bounds__(256) k_chol_trsm(const double* __restrict__ band, double* __restrict__ lpanel,
+                                                   const double* __restrict__ linv, int k, int nt, int kb,
+                                                   const int* info) {
+  __shared__ double Xs[TILE * LSTR];
+  __shared__ double As[16 * LSTR];
+  if (*info != 0) return;
+  const int d = blockIdx.x / 4 + 1, quarter = blockIdx.x % 4;
+  if (d > kb || k + d >= nt) return;
+  const int tid = threadIdx.x;
+  const double* a = band_tile(const_cast<double*>(band), kb, k, d) + (quarter * 16 << 6);
+  const double* inv = linv + ((size_t)k << 12);
+  for (int e = tid; e < TILE * TILE; e += 256) Xs[(e >> 6) * LSTR + (e & 63)] = inv[e];
+  for (int e = tid; e < 16 * TILE; e += 256) As[(e >> 6) * LSTR + (e & 63)] = a[e];
+  __syncthreads();
+  // thread: row r (0..15), columns c0 .. c0 + 3
+  const int r = tid >> 4, c0 = (tid & 15) * 4;
+  double o[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int m = 0; m < TILE; ++m) {
+    const double av = As[r * LSTR + m];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] += av * Xs[(c0 + v) * LSTR + m];
+  }
+  double* l = lpanel + (((size_t)k * kb + (d - 1)) << 12) + ((quarter * 16 + r) << 6) + c0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) l[v] = o[v];
+}
+
+// Trailing update after tile column k: tile (k + d1, k + d2) -= L(k + d1, k) L(k + d2, k)^T for
+// 1 <= d2 <= d1 <= kb; workgroup (pair, quarter) updates 16 rows of the tile.
+__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ band, const double* __restrict__ lpanel,
+                                                     int k, int nt, int kb, const int* info) {
+  __shared__ double Ps[16 * LSTR];
+  __shared__ double Qs[TILE * LSTR];
+  if (*info != 0) return;
+  int d1 = 1, idx = blockIdx.x / 4;
+  const int quarter = blockIdx.x % 4;
+  while (idx >= d1) {
+    idx -= d1;
+    ++d1;
+  }
+  const int d2 = idx + 1;
+  if (d1 > kb || k + d1 >= nt) return;
+  const int tid = threadIdx.x;
+  const double* p = lpanel + (((size_t)k * kb + (d1 - 1)) << 12) + (quarter * 16 << 6);
+  const double* q = lpanel + (((size_t)k * kb + (d2 - 1)) << 12);
+  for (int e = tid; e < TILE * TILE; e += 256) Qs[(e >> 6) * LSTR + (e & 63)] = q[e];
+  for (int e = tid; e < 16 * TILE; e += 256) Ps[(e >> 6) * LSTR + (e & 63)] = p[e];
+  __syncthreads();
+  const int r = tid >> 4, c0 = (tid & 15) * 4;
+  double o[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int m = 0; m < TILE; ++m) {
+    const double pv = Ps[r * LSTR + m];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] += pv * Qs[(c0 + v) * LSTR + m];
+  }
+  double* out = band_tile(band, kb, k + d2, d1 - d2) + ((quarter * 16 + r) << 6) + c0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[v] -= o[v];
+}
+
+// B = L^-1 by blocked forward substitution: workgroup (J, g) owns the 16 columns 64 J + 16 g .. + 15 and
+// walks the row tiles I = J, J + 1, ...: X_I = Linv_II (E_I - sum_d L(I, I - d) X_{I - d}), the last kb
+// tiles of X kept in LDS. The factor tiles stream through one LDS tile buffer, the next one prefetched into
+// registers while the current one is used. FP32 B (row-major) and B^T (row-major) receive the rows and
+// columns below N3.
+constexpr int INV_COLS = 16;
+__global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__ lpanel,
+                                                      const double* __restrict__ linv, int nt, int kb, int N3,
+                                                      int N3p, float* __restrict__ B, float* __restrict__ BT,
+                                                      const int* info) {
+  __shared__ double Xh[(MAX_KB + 1) * TILE * INV_COLS];
+  __shared__ double T[TILE * INV_COLS];
+  __shared__ double Lb[TILE * LSTR];
+  if (*info != 0) return;
+  const int J = blockIdx.x / (TILE / INV_COLS), g = blockIdx.x % (TILE / INV_COLS);
+  const int tid = threadIdx.x;
+  const int r = tid >> 2, cc = (tid & 3) * 4;   // output row r, columns cc .. cc + 3 of the group
+  const int j0 = TILE * J + INV_COLS * g;       // first global column of the group
+  // stage (I, d): d = 1 .. min(I - J, kb) are the factor tiles L(I, I - d), d = 0 the diagonal inverse
+  auto tile_of = [&](int I, int d) -> const double* {
+    return d == 0 ? linv + ((size_t)I << 12) : lpanel + (((size_t)(I - d) * kb + (d - 1)) << 12);
+  };
+  auto first_stage = [&](int I) { return (I - J < kb ? I - J : kb) > 0 ? 1 : 0; };
+  auto next_stage = [&](int I, int d, int* nI, int* nd) {
+    const int dmax = I - J < kb ? I - J : kb;
+    if (d != 0 && d < dmax) {
+      *nI = I;
+      *nd = d + 1;
+    } else if (d != 0) {
+      *nI = I;
+      *nd = 0;
+    } else {
+      *nI = I + 1;
+      *nd = first_stage(I + 1);
+    }
+  };
+  double pre[16];
+  {
+    const double* src = tile_of(J, first_stage(J));
+#pragma unroll
+    for (int i = 0; i < 16; ++i) pre[i] = src[tid + 256 * i];
+  }
+  int I = J, d = first_stage(J);
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  if (d == 0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[u] = (r == INV_COLS * g + cc + u) ? 1.0 : 0.0;
+  }
+  while (I < nt) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = tid + 256 * i;
+      Lb[(e >> 6) * LSTR + (e & 63)] = pre[i];
+    }
+    if (d == 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) T[r * INV_COLS + cc + u] = t[u];
+    }
+    __syncthreads();
+    int nI, nd;
+    next_stage(I, d, &nI, &nd);
+    if (nI < nt) {
+      const double* src = tile_of(nI, nd);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) pre[i] = src[tid + 256 * i];
+    }
+    if (d != 0) {
+      const double* x = Xh + ((I - d) % (kb + 1)) * TILE * INV_COLS;
+#pragma unroll 8
+      for (int m = 0; m < TILE; ++m) {
+        const double lv = Lb[r * LSTR + m];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] -= lv * x[m * INV_COLS + cc + u];
+      }
+    } else {
+      double xo[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+      for (int m = 0; m < TILE; ++m) {
+        const double lv = Lb[r * LSTR + m];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xo[u] += lv * T[m * INV_COLS + cc + u];
+      }
+      double* slot = Xh + (I % (kb + 1)) * TILE * INV_COLS;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) slot[r * INV_COLS + cc + u] = xo[u];
+      const int R = TILE * I + r;
+      if (R < N3) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int C = j0 + cc + u;
+          if (C < N3) {
+            B[(size_t)R * N3p + C] = (float)xo[u];
+            BT[(size_t)C * N3p + R] = (float)xo[u];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = 0.0;
+    }
+    I = nI;
+    d = nd;
+  }
+}
+
+// dense A_c (both triangles, row-major) for the check entry point
+__global__ void __launch_bounds__(256) k_galerkin_dense(const uint32_t* __restrict__ pair_ptr,
+                                                        const uint32_t* __restrict__ pair_key,
+                                                        const uint4* __restrict__ terms, const float* __restrict__ w,
+                                                        const float* __restrict__ sell_val,
+                                                        const uint32_t* __restrict__ block_dst, int npairs, int Nc,
+                                                        double* __restrict__ Ac) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -101,31 +518,6 @@ __global__ void __launch_bounds__(256) k_galerkin(const uint32_t* __restrict__ p
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) Ac[(3 * (size_t)I + r) * N3 + 3 * (size_t)J + c] = acc[3 * r + c];
-}
-
-// B = L^-1 from trtri (lower triangle of the column-major A) as row-major FP32 B and B^T, row stride N3p.
-__global__ void k_inverse_to_f32(const double* __restrict__ A, int N3, int N3p, float* __restrict__ B,
-                                 float* __restrict__ BT) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)N3 * N3p) return;
-  const int r = (int)(idx / N3p), c = (int)(idx % N3p);
-  float lower = 0.0f, upper = 0.0f;
-  if (c < N3) {
-    if (c <= r) lower = (float)A[(size_t)c * N3 + r];  // B(r, c)
-    if (c >= r) upper = (float)A[(size_t)r * N3 + c];  // B^T(r, c) = B(c, r)
-  }
-  B[idx] = lower;
-  BT[idx] = upper;
-}
-
-__global__ void k_scale_diagonal(double* __restrict__ A, int n, double factor) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) A[(size_t)i * n + i] *= factor;
-}
-
-__global__ void k_identity(double* __restrict__ A, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) A[(size_t)i * n + i] = 1.0;
 }
 
 // r_c[I] = sum over the children (i, k) of I of w_ik r_i: one wavefront per coarse node, lane j takes
@@ -308,6 +700,17 @@ int two_level_create(Context& c, int columns) {
     pair_ptr.push_back((uint32_t)terms.size());
   }
   t->npairs = (int)pair_key.size();
+  // band of the coarse matrix in 64-row tiles
+  t->nt = (t->N3 + TILE - 1) / TILE;
+  for (uint32_t key : pair_key) {
+    const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
+    if (I >= J) t->kb = std::max(t->kb, (3 * I + 2) / TILE - (3 * J) / TILE);
+  }
+  if (t->kb > MAX_KB) {
+    set_error("two-level preconditioner: the coarse grid is too large for the banded coarse solve");
+    delete t;
+    return DRAPE_E_INVALID;
+  }
   hipStream_t s = c.stream;
   auto fail = [&](hipError_t e) {
     set_error(std::string("two-level setup: ") + hipGetErrorString(e));
@@ -324,7 +727,9 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_pair_ptr, pair_ptr.size());
   TL_ALLOC(t->d_pair_key, pair_key.size());
   TL_ALLOC(t->d_pair_terms, terms.size());
-  TL_ALLOC(t->d_Ac, (size_t)t->N3 * t->N3);
+  TL_ALLOC(t->d_band, (size_t)t->nt * (t->kb + 1) * TILE * TILE);
+  TL_ALLOC(t->d_linv, (size_t)t->nt * TILE * TILE);
+  TL_ALLOC(t->d_lpanel, (size_t)t->nt * std::max(t->kb, 1) * TILE * TILE);
   TL_ALLOC(t->d_B, (size_t)t->N3 * t->N3p);
   TL_ALLOC(t->d_BT, (size_t)t->N3 * t->N3p);
   TL_ALLOC(t->d_t, (size_t)t->N3p);
@@ -346,28 +751,10 @@ int two_level_create(Context& c, int columns) {
       (e = hipMemsetAsync(t->d_rc, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_yc, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_t, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_B, 0, (size_t)t->N3 * t->N3p * 4, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_BT, 0, (size_t)t->N3 * t->N3p * 4, s)) != hipSuccess ||
       (e = hipStreamSynchronize(s)) != hipSuccess)
     return fail(e);
-  if (rocblas_create_handle(&t->handle) != rocblas_status_success ||
-      rocblas_set_stream(t->handle, s) != rocblas_status_success) {
-    set_error("two-level setup: rocblas handle");
-    two_level_destroy(t);
-    return DRAPE_E_DEVICE;
-  }
-  // load rocSOLVER's kernels for this size now rather than inside the first timed solve: factor the identity
-  {
-    const size_t n2 = (size_t)t->N3 * t->N3;
-    if ((e = hipMemsetAsync(t->d_Ac, 0, n2 * sizeof(double), s)) != hipSuccess) return fail(e);
-    k_identity<<<blocks_for(t->N3), 256, 0, s>>>(t->d_Ac, t->N3);
-    if (rocsolver_dpotrf(t->handle, rocblas_fill_lower, t->N3, t->d_Ac, t->N3, t->d_info) != rocblas_status_success ||
-        rocsolver_dtrtri(t->handle, rocblas_fill_lower, rocblas_diagonal_non_unit, t->N3, t->d_Ac, t->N3,
-                         t->d_info) != rocblas_status_success) {
-      set_error("two-level setup: rocsolver warm-up");
-      two_level_destroy(t);
-      return DRAPE_E_DEVICE;
-    }
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
-  }
   if (c.two_level) two_level_destroy(c.two_level);
   c.two_level = t;
   return DRAPE_OK;
@@ -412,52 +799,53 @@ int two_level_prepare(Context& c, bool* ok) {
     std::chrono::steady_clock::time_point t0;
     ~Clock() { t->factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
   } clock{t, started};
-  const size_t n2 = (size_t)t->N3 * t->N3;
-  HIP_TRY(hipMemsetAsync(t->d_Ac, 0, n2 * sizeof(double), s));
-  k_galerkin<<<blocks_for(t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w, c.sell_val,
-                                                   c.block_dst, t->npairs, t->Nc, t->d_Ac);
-  if (c.rules.coarse_shift != 0.0)
-    k_scale_diagonal<<<blocks_for(t->N3), 256, 0, s>>>(t->d_Ac, t->N3, 1.0 + c.rules.coarse_shift);
-  HIP_TRY(hipGetLastError());
-  if (rocsolver_dpotrf(t->handle, rocblas_fill_lower, t->N3, t->d_Ac, t->N3, t->d_info) != rocblas_status_success) {
-    set_error("rocsolver_dpotrf failed");
-    return DRAPE_E_DEVICE;
+  HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * TILE * TILE * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
+  k_galerkin_band<<<blocks_for(9 * t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
+                                                        c.sell_val, c.block_dst, t->npairs, t->Nc, t->kb, t->d_band);
+  k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(t->d_band, t->kb, t->nt, t->N3, 1.0 + c.rules.coarse_shift);
+  const int updates = t->kb * (t->kb + 1) / 2;
+  for (int k = 0; k < t->nt; ++k) {
+    k_chol_panel<<<1, 256, 0, s>>>(t->d_band, t->d_linv, k, t->kb, t->d_info);
+    if (k + 1 < t->nt && updates > 0) {
+      k_chol_trsm<<<4 * t->kb, 256, 0, s>>>(t->d_band, t->d_lpanel, t->d_linv, k, t->nt, t->kb, t->d_info);
+      k_chol_update<<<4 * updates, 256, 0, s>>>(t->d_band, t->d_lpanel, k, t->nt, t->kb, t->d_info);
+    }
   }
+  k_band_inverse<<<t->nt * (TILE / INV_COLS), 256, 0, s>>>(t->d_lpanel, t->d_linv, t->nt, t->kb, t->N3, t->N3p, t->d_B,
+                                                           t->d_BT, t->d_info);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(t->h_info, t->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   t->factorisations++;
-  if (*t->h_info != 0) {
-    t->failures++;
-    return DRAPE_OK;
-  }
-  if (rocsolver_dtrtri(t->handle, rocblas_fill_lower, rocblas_diagonal_non_unit, t->N3, t->d_Ac, t->N3, t->d_info) !=
-      rocblas_status_success) {
-    set_error("rocsolver_dtrtri failed");
-    return DRAPE_E_DEVICE;
-  }
-  const size_t nm = (size_t)t->N3 * t->N3p;
-  k_inverse_to_f32<<<(unsigned)((nm + 255) / 256), 256, 0, s>>>(t->d_Ac, t->N3, t->N3p, t->d_B, t->d_BT);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(t->h_info, t->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
   t->ok = *t->h_info == 0;
   if (!t->ok) t->failures++;
   *ok = t->ok;
   return DRAPE_OK;
 }
 
-// The Galerkin operator of the assembled system, row-major [3Nc x 3Nc], into host memory. The current
-// factorisation lives in B and B^T and is unaffected.
+// The Galerkin operator of the assembled system, row-major [3Nc x 3Nc], into host memory (a dense scratch
+// copy allocated on first use). The current factorisation is unaffected.
 int two_level_galerkin(Context& c, double* out) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
   const size_t n2 = (size_t)t->N3 * t->N3;
+  if (!t->d_Ac) HIP_TRY(hipMalloc((void**)&t->d_Ac, n2 * sizeof(double)));
   HIP_TRY(hipMemsetAsync(t->d_Ac, 0, n2 * sizeof(double), s));
-  k_galerkin<<<blocks_for(t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w, c.sell_val,
-                                                   c.block_dst, t->npairs, t->Nc, t->d_Ac);
+  k_galerkin_dense<<<blocks_for(t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
+                                                         c.sell_val, c.block_dst, t->npairs, t->Nc, t->d_Ac);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, t->d_Ac, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return DRAPE_OK;
+}
+
+// B = L^-1 of the current factorisation (FP32, row-major [3Nc x 3Nc]) into host memory.
+int two_level_inverse_factor(Context& c, float* out) {
+  TwoLevel* t = c.two_level;
+  HIP_TRY(hipMemcpy2DAsync(out, (size_t)t->N3 * sizeof(float), t->d_B, (size_t)t->N3p * sizeof(float),
+                           (size_t)t->N3 * sizeof(float), t->N3, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
   return DRAPE_OK;
 }
 

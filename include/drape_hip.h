@@ -231,6 +231,10 @@ int drape_two_level_layout(drape_ctx* ctx, int32_t* coarse_rows, int32_t* parent
  * drape_two_level_layout), for checking; the preconditioner's factorisation is left as it is. */
 int drape_two_level_coarse(drape_ctx* ctx, double* coarse);
 
+/* B = L^-1 (A_c = L L^T) of the current coarse factorisation as FP32, row-major [3Nc x 3Nc] (zero above the
+ * diagonal), for checking; the coarse solve applies B^T B. */
+int drape_two_level_inverse_factor(drape_ctx* ctx, float* factor);
+
 /* Device timing of the last drape_pcg (HIP events on the context's stream): average SpMV kernel time
  * and the whole solve, in milliseconds. */
 int drape_pcg_timing(drape_ctx* ctx, double* spmv_ms, double* solve_ms, int32_t* spmv_launches);

@@ -55,6 +55,34 @@ def test_galerkin_operator_matches_f64_triple_product():
     ctx.close()
 
 
+@pytest.mark.parametrize("n", [33, 65, 129])
+def test_coarse_inverse_factor_matches_f64_cholesky(n):
+    """The banded FP64 Cholesky and blocked inverse on the device: B = L^-1 (FP32 copy) against numpy's
+    inverse of the f64 Cholesky factor of the same Galerkin operator, max error within 1e-6 of max|B|, and
+    exactly zero above the diagonal."""
+    if n == 129:
+        arrays = D.SceneArrays(D.drape_config_scene(129))
+        ctx = D.Drape(arrays)
+        rest = arrays.array("rest_positions")
+        ctx.set_state(rest, rest, 1.0 / 60.0, 0.0)
+        ctx.residual()
+    else:
+        arrays, ctx = pinned_system(n, 0.1)
+    ctx.set_preconditioner("two_level")
+    ctx.solve_once(1e-2, 200)
+    nc, parent, w, counters = ctx.two_level_layout()
+    Ac = ctx.two_level_coarse()
+    B = ctx.two_level_inverse_factor().astype(np.float64)
+    want = np.linalg.inv(np.linalg.cholesky(0.5 * (Ac + Ac.T)))
+    err = np.abs(B - want).max() / np.abs(want).max()
+    print(f"[gpu] coarse inverse factor n{n}: {3 * nc} unknowns, counters {counters}, max error / max|B| = {err:.2e}",
+          flush=True)
+    assert counters[1] == 0
+    assert err <= 1e-6
+    assert not np.any(np.triu(B, 1))
+    ctx.close()
+
+
 @pytest.mark.parametrize("n,h,tol", [(65, 0.1, 1e-2), (65, 0.03, 1e-3)])
 def test_two_level_pcg_matches_f64_two_level(n, h, tol):
     """On the pinned n65 sheet at step sizes where the stiffness dominates the mass term, the GPU's FP32
