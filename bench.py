@@ -32,10 +32,42 @@ sys.path.insert(0, ROOT)
 import drape_amd as D  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_pcg_spmv, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by the same
+# counters of a known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part); keyed by grid columns.
+SPMV_TRAFFIC = {1025: 534_700_676}
 
 
 def spmv_bytes(rows: int, blocks: int) -> int:
     return 40 * blocks + 28 * rows + 4
+
+
+def large_grid_spmv(columns: int, launches: int = 50) -> dict:
+    """The plain SpMV of PCG on a grid whose matrix exceeds the 256 MB Infinity Cache (n1025: 11.5 M blocks,
+    414 MB of values), timed with HIP events: the HBM-bound roofline of the kernel."""
+    scene = D.drape_config_scene(columns)
+    arrays = D.SceneArrays(scene)
+    ctx = D.Drape(arrays)
+    rest = arrays.array("rest_positions")
+    ctx.set_state(rest, rest, 1.0 / 60.0, 0.0)
+    ctx.residual()
+    ctx.solve_once(1e-2, 2)   # fills the direction vector the SpMV reads
+    spmv_ms, _ = ctx.spmv_timing(launches)
+    nbytes = spmv_bytes(arrays.R, arrays.B)
+    achieved = nbytes / (spmv_ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": SPMV_TRAFFIC.get(columns), "kernel": "k_pcg_spmv",
+           "avg_kernel_us": round(1000.0 * spmv_ms, 3), "algorithmic_bytes": nbytes,
+           "workload": f"n{columns} ({arrays.R} rows, {arrays.B} blocks)"}
+    ctx.close()
+    return out
+
+
+def copy_calibration(mbytes: int = 512) -> dict:
+    """A plain device copy of known size (k_copy: read + write), for scaling the PMC byte counters."""
+    nbytes = mbytes << 20
+    ms = D.copy_timing(nbytes, 10)
+    return {"kernel": "k_copy", "bytes_read": nbytes, "bytes_written": nbytes, "avg_kernel_us": round(1000.0 * ms, 3),
+            "achieved_gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def cpu_baseline(columns: int) -> dict:
@@ -70,12 +102,21 @@ def main() -> None:
     ap.add_argument("--columns", type=int, default=129)
     ap.add_argument("--preconditioner", choices=["block_jacobi", "two_level"], default="two_level")
     ap.add_argument("--no-compare", action="store_true", help="skip the block-Jacobi comparison run")
+    ap.add_argument("--roofline-columns", type=int, default=1025,
+                    help="grid for the HBM-bound SpMV roofline (0 skips it)")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the large-grid SpMV and a copy of known size (for PMC byte counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
                                    "not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.spmv_only:
+        print(json.dumps({"spmv": large_grid_spmv(args.roofline_columns or 1025), "copy": copy_calibration()}),
+              flush=True)
+        return
 
     scene = D.drape_config_scene(args.columns)
     arrays = D.SceneArrays(scene)
@@ -150,6 +191,8 @@ def main() -> None:
                    "block_jacobi": comparison},
         "roofline": roofline,
     }
+    if args.roofline_columns > 0:
+        line["roofline_large_grid"] = large_grid_spmv(args.roofline_columns)
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(args.columns)

@@ -137,6 +137,7 @@ def lib():
         L.drape_set_preconditioner.argtypes = [vp, C.c_int32, f32p]
         L.drape_two_level_layout.argtypes = [vp, i32p, i32p, f32p, C.POINTER(C.c_int64)]
         L.drape_two_level_coarse.argtypes = [vp, f64p]
+        L.drape_copy_timing.argtypes = [C.c_int64, C.c_int32, f64p]
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_problem.restype = C.POINTER(Problem)
@@ -157,6 +158,13 @@ def _f32(a) -> np.ndarray:
 
 def _ptr(a: np.ndarray, t):
     return a.ctypes.data_as(t)
+
+
+def copy_timing(nbytes: int, launches: int = 10) -> float:
+    """Average ms of a plain device copy of nbytes (the byte-counter calibration kernel)."""
+    ms = C.c_double()
+    _check(lib().drape_copy_timing(nbytes, launches, C.byref(ms)))
+    return ms.value
 
 
 def default_rules(**overrides) -> Rules:

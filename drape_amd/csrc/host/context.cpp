@@ -642,6 +642,11 @@ int drape_two_level_inverse_factor(drape_ctx* p, float* factor) {
   return two_level_inverse_factor(*c, factor);
 }
 
+int drape_copy_timing(int64_t bytes, int32_t launches, double* avg_ms) {
+  if (bytes <= 0 || !avg_ms) { set_error("invalid copy timing arguments"); return DRAPE_E_INVALID; }
+  return copy_timing((size_t)bytes, launches, avg_ms);
+}
+
 int drape_pcg_timing(drape_ctx* p, double* spmv_ms, double* solve_ms, int32_t* spmv_launches) {
   Context* c = ctx_of(p);
   if (!c) { set_error("null context"); return DRAPE_E_INVALID; }

@@ -144,6 +144,7 @@ int launch_towards(Context& c);                     // dir = x_hat - pos (all ve
 int launch_step_all(Context& c, float alpha);       // pos += alpha dir (all vertices)
 int launch_ptc_prepare(Context& c, const float4* base, double h);  // x_hat, M/h^2 of a PTC attempt
 int spmv_timing(Context& c, int launches, double* avg_ms);
+int copy_timing(size_t bytes, int launches, double* avg_ms);  // known-size copy kernel
 int launch_set_float(Context& c, float* dst, float v);  // device scalar write by kernel argument
 
 // ---- PCG (pcg.hip) --------------------------------------------------------------------------

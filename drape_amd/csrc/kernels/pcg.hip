@@ -549,4 +549,48 @@ int spmv_timing(Context& c, int launches, double* avg_ms) {
   return DRAPE_OK;
 }
 
+// A plain copy of n float4 (16 bytes each), grid-stride: the known-size kernel that scales hardware
+// byte counters.
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+int copy_timing(size_t bytes, int launches, double* avg_ms) {
+  const size_t n = bytes / 16;
+  if (n == 0 || launches <= 0) {
+    set_error("copy timing needs at least 16 bytes and one launch");
+    return DRAPE_E_INVALID;
+  }
+  float4 *src = nullptr, *dst = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = DRAPE_OK;
+  auto check = [&](hipError_t e) {
+    if (e != hipSuccess && rc == DRAPE_OK) {
+      set_error(std::string("copy timing: ") + hipGetErrorString(e));
+      rc = DRAPE_E_DEVICE;
+    }
+    return rc == DRAPE_OK;
+  };
+  if (check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) && check(hipEventCreate(&e0)) &&
+      check(hipEventCreate(&e1)) && check(hipMalloc((void**)&src, n * 16)) && check(hipMalloc((void**)&dst, n * 16)) &&
+      check(hipMemsetAsync(src, 0, n * 16, s))) {
+    const unsigned grid = 1024u * 8u;  // eight workgroups per CU slot of the 256-CU part, grid-stride beyond
+    k_copy<<<grid, 256, 0, s>>>(src, dst, n);
+    check(hipGetLastError());
+    if (check(hipEventRecord(e0, s))) {
+      for (int i = 0; i < launches; ++i) k_copy<<<grid, 256, 0, s>>>(src, dst, n);
+      float ms = 0.0f;
+      if (check(hipEventRecord(e1, s)) && check(hipEventSynchronize(e1)) && check(hipEventElapsedTime(&ms, e0, e1)))
+        *avg_ms = ms / launches;
+    }
+  }
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  return rc;
+}
+
 }  // namespace drape

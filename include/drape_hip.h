@@ -239,6 +239,11 @@ int drape_two_level_inverse_factor(drape_ctx* ctx, float* factor);
  * and the whole solve, in milliseconds. */
 int drape_pcg_timing(drape_ctx* ctx, double* spmv_ms, double* solve_ms, int32_t* spmv_launches);
 
+/* A plain device-to-device copy of `bytes` (16-byte units), `launches` times between HIP events on a stream of
+ * its own: the average kernel time in ms. No reference counterpart: the known-size kernel by which hardware
+ * byte counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) are scaled. */
+int drape_copy_timing(int64_t bytes, int32_t launches, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
