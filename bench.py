@@ -34,7 +34,8 @@ import drape_amd as D  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_pcg_spmv, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by the same
 # counters of a known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part); keyed by grid columns.
-SPMV_TRAFFIC = {1025: 534_700_676}
+# (At n129 the bytes are L2 misses served mostly by the Infinity Cache.)
+SPMV_TRAFFIC = {129: 8_574_432, 1025: 534_700_676}
 
 
 def spmv_bytes(rows: int, blocks: int) -> int:
@@ -162,7 +163,7 @@ def main() -> None:
     nbytes = spmv_bytes(arrays.R, arrays.B)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": SPMV_TRAFFIC.get(args.columns),
                 "kernel": "k_pcg_spmv", "avg_kernel_us": round(1000.0 * spmv_ms, 3),
                 "algorithmic_bytes": nbytes, "note": "n129 working set (~10 MB) is cache-resident"}
     line = {

@@ -302,24 +302,29 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ band, d
     }
   }
   __syncthreads();
-  // off-diagonal blocks (a, b = a - dist): Y = sum_{c=b}^{a-1} L_ac X_cb, then X_ab = -X_aa Y
+  // off-diagonal blocks (a, b = a - dist): Y = sum_{c=b}^{a-1} L_ac X_cb, then X_ab = -X_aa Y; the sums run
+  // over compile-time lengths in four independent partial sums so that the LDS loads pipeline
+#pragma unroll
   for (int dist = 1; dist < 4; ++dist) {
     const int nblk = 4 - dist;
     for (int q = tid; q < nblk * 256; q += 256) {
       const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
       const int a = dist + blk, b = blk;
-      double acc = 0.0;
-      for (int m = 16 * b; m < 16 * a; ++m) acc += Ls[(16 * a + i) * LSTR + m] * Xs[m * LSTR + 16 * b + j];
-      Ys[blk * 16 * 17 + i * 17 + j] = acc;
+      const double* lrow = Ls + (16 * a + i) * LSTR + 16 * b;
+      const double* xcol = Xs + (16 * b) * LSTR + 16 * b + j;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m = 0; m < 16 * dist; ++m) acc[m & 3] += lrow[m] * xcol[m * LSTR];
+      Ys[blk * 16 * 17 + i * 17 + j] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
     __syncthreads();
     for (int q = tid; q < nblk * 256; q += 256) {
       const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
       const int a = dist + blk, b = blk;
-      double acc = 0.0;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int m = 0; m < 16; ++m) acc += Xs[(16 * a + i) * LSTR + 16 * a + m] * Ys[blk * 16 * 17 + m * 17 + j];
-      Xs[(16 * a + i) * LSTR + 16 * b + j] = -acc;
+      for (int m = 0; m < 16; ++m) acc[m & 3] += Xs[(16 * a + i) * LSTR + 16 * a + m] * Ys[blk * 16 * 17 + m * 17 + j];
+      Xs[(16 * a + i) * LSTR + 16 * b + j] = -((acc[0] + acc[1]) + (acc[2] + acc[3]));
     }
     __syncthreads();
   }
@@ -343,17 +348,18 @@ __global__ void __launch_bounds__(256) k_chol_trsm(const double* __restrict__ ba
   for (int e = tid; e < TILE * TILE; e += 256) Xs[(e >> 6) * LSTR + (e & 63)] = inv[e];
   for (int e = tid; e < 16 * TILE; e += 256) As[(e >> 6) * LSTR + (e & 63)] = a[e];
   __syncthreads();
-  // thread: row r (0..15), columns c0 .. c0 + 3
-  const int r = tid >> 4, c0 = (tid & 15) * 4;
+  // thread: row r (0..15), columns c0 + 16 v (16 consecutive rows of Xs per load: no bank conflicts)
+  const int r = tid >> 4, c0 = tid & 15;
   double o[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
   for (int m = 0; m < TILE; ++m) {
     const double av = As[r * LSTR + m];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) o[v] += av * Xs[(c0 + v) * LSTR + m];
+    for (int v = 0; v < 4; ++v) o[v] += av * Xs[(c0 + 16 * v) * LSTR + m];
   }
   double* l = lpanel + (((size_t)k * kb + (d - 1)) << 12) + ((quarter * 16 + r) << 6) + c0;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) l[v] = o[v];
+  for (int v = 0; v < 4; ++v) l[16 * v] = o[v];
 }
 
 // Trailing update after tile column k: tile (k + d1, k + d2) -= L(k + d1, k) L(k + d2, k)^T for
@@ -377,23 +383,25 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ band, 
   for (int e = tid; e < TILE * TILE; e += 256) Qs[(e >> 6) * LSTR + (e & 63)] = q[e];
   for (int e = tid; e < 16 * TILE; e += 256) Ps[(e >> 6) * LSTR + (e & 63)] = p[e];
   __syncthreads();
-  const int r = tid >> 4, c0 = (tid & 15) * 4;
+  const int r = tid >> 4, c0 = tid & 15;
   double o[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
   for (int m = 0; m < TILE; ++m) {
     const double pv = Ps[r * LSTR + m];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) o[v] += pv * Qs[(c0 + v) * LSTR + m];
+    for (int v = 0; v < 4; ++v) o[v] += pv * Qs[(c0 + 16 * v) * LSTR + m];
   }
   double* out = band_tile(band, kb, k + d2, d1 - d2) + ((quarter * 16 + r) << 6) + c0;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) out[v] -= o[v];
+  for (int v = 0; v < 4; ++v) out[16 * v] -= o[v];
 }
 
 // B = L^-1 by blocked forward substitution: workgroup (J, g) owns the 16 columns 64 J + 16 g .. + 15 and
 // walks the row tiles I = J, J + 1, ...: X_I = Linv_II (E_I - sum_d L(I, I - d) X_{I - d}), the last kb
-// tiles of X kept in LDS. The factor tiles stream through one LDS tile buffer, the next one prefetched into
-// registers while the current one is used. FP32 B (row-major) and B^T (row-major) receive the rows and
-// columns below N3.
+// tiles of X kept in LDS. The factor tiles stream through one LDS tile buffer, the next two prefetched into
+// registers while the current one is used. Each product splits its 64-long sums over the four wavefronts
+// (16 terms each, a 4 x 4 block of outputs per lane), and the partial sums meet in LDS in a fixed order.
+// FP32 B (row-major) and B^T (row-major) receive the rows and columns below N3.
 constexpr int INV_COLS = 16;
 __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__ lpanel,
                                                       const double* __restrict__ linv, int nt, int kb, int N3,
@@ -401,12 +409,13 @@ __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__
                                                       const int* info) {
   __shared__ double Xh[(MAX_KB + 1) * TILE * INV_COLS];
   __shared__ double T[TILE * INV_COLS];
+  __shared__ double Part[4 * TILE * INV_COLS];
   __shared__ double Lb[TILE * LSTR];
   if (*info != 0) return;
   const int J = blockIdx.x / (TILE / INV_COLS), g = blockIdx.x % (TILE / INV_COLS);
-  const int tid = threadIdx.x;
-  const int r = tid >> 2, cc = (tid & 3) * 4;   // output row r, columns cc .. cc + 3 of the group
-  const int j0 = TILE * J + INV_COLS * g;       // first global column of the group
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int r0 = (lane >> 2) * 4, c0 = (lane & 3) * 4, m0 = 16 * wave;  // 4 x 4 outputs, 16 terms per wave
+  const int j0 = TILE * J + INV_COLS * g;                                // first global column of the group
   // stage (I, d): d = 1 .. min(I - J, kb) are the factor tiles L(I, I - d), d = 0 the diagonal inverse
   auto tile_of = [&](int I, int d) -> const double* {
     return d == 0 ? linv + ((size_t)I << 12) : lpanel + (((size_t)(I - d) * kb + (d - 1)) << 12);
@@ -425,18 +434,22 @@ __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__
       *nd = first_stage(I + 1);
     }
   };
-  double pre[16];
+  // two stages of factor tiles in flight in registers
+  double pre[16], pre2[16];
+  int I = J, d = first_stage(J);
+  int I2, d2;
+  next_stage(I, d, &I2, &d2);
   {
-    const double* src = tile_of(J, first_stage(J));
+    const double* src = tile_of(I, d);
 #pragma unroll
     for (int i = 0; i < 16; ++i) pre[i] = src[tid + 256 * i];
-  }
-  int I = J, d = first_stage(J);
-  double t[4] = {0.0, 0.0, 0.0, 0.0};
-  if (d == 0) {
+    if (I2 < nt) {
+      const double* src2 = tile_of(I2, d2);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) t[u] = (r == INV_COLS * g + cc + u) ? 1.0 : 0.0;
+      for (int i = 0; i < 16; ++i) pre2[i] = src2[tid + 256 * i];
+    }
   }
+  double acc[4][4] = {};   // this wave's part of -sum_d L(I, I - d) X_{I - d}
   while (I < nt) {
     __syncthreads();
 #pragma unroll
@@ -445,49 +458,79 @@ __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__
       Lb[(e >> 6) * LSTR + (e & 63)] = pre[i];
     }
     if (d == 0) {
+      // partial sums of the factor tiles, then T = E_I + their sum
 #pragma unroll
-      for (int u = 0; u < 4; ++u) T[r * INV_COLS + cc + u] = t[u];
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Part[wave * TILE * INV_COLS + (r0 + u) * INV_COLS + c0 + v] = acc[u][v];
     }
     __syncthreads();
-    int nI, nd;
+    int nI, nd, nI2 = nt, nd2 = 0;
     next_stage(I, d, &nI, &nd);
-    if (nI < nt) {
-      const double* src = tile_of(nI, nd);
+    if (nI < nt) next_stage(nI, nd, &nI2, &nd2);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) pre[i] = src[tid + 256 * i];
+    for (int i = 0; i < 16; ++i) pre[i] = pre2[i];
+    if (nI2 < nt) {
+      const double* src = tile_of(nI2, nd2);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) pre2[i] = src[tid + 256 * i];
     }
     if (d != 0) {
       const double* x = Xh + ((I - d) % (kb + 1)) * TILE * INV_COLS;
-#pragma unroll 8
-      for (int m = 0; m < TILE; ++m) {
-        const double lv = Lb[r * LSTR + m];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) t[u] -= lv * x[m * INV_COLS + cc + u];
+      for (int m = m0; m < m0 + 16; ++m) {
+        double lv[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) lv[u] = Lb[(r0 + u) * LSTR + m];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xv[v] = x[m * INV_COLS + c0 + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] -= lv[u] * xv[v];
       }
     } else {
-      double xo[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-      for (int m = 0; m < TILE; ++m) {
-        const double lv = Lb[r * LSTR + m];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xo[u] += lv * T[m * INV_COLS + cc + u];
+      // T = E_I + sum over the four partials
+      for (int e = tid; e < TILE * INV_COLS; e += 256) {
+        const int rr = e / INV_COLS, cc = e % INV_COLS;
+        const double eye = (I == J && rr == INV_COLS * g + cc) ? 1.0 : 0.0;
+        T[e] = eye + ((Part[e] + Part[TILE * INV_COLS + e]) +
+                      (Part[2 * TILE * INV_COLS + e] + Part[3 * TILE * INV_COLS + e]));
       }
+      __syncthreads();
+      double xo[4][4] = {};
+#pragma unroll
+      for (int m = m0; m < m0 + 16; ++m) {
+        double lv[4], tv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) lv[u] = Lb[(r0 + u) * LSTR + m];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) tv[v] = T[m * INV_COLS + c0 + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) xo[u][v] += lv[u] * tv[v];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Part[wave * TILE * INV_COLS + (r0 + u) * INV_COLS + c0 + v] = xo[u][v];
+      __syncthreads();
       double* slot = Xh + (I % (kb + 1)) * TILE * INV_COLS;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) slot[r * INV_COLS + cc + u] = xo[u];
-      const int R = TILE * I + r;
-      if (R < N3) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int C = j0 + cc + u;
-          if (C < N3) {
-            B[(size_t)R * N3p + C] = (float)xo[u];
-            BT[(size_t)C * N3p + R] = (float)xo[u];
-          }
+      for (int e = tid; e < TILE * INV_COLS; e += 256) {
+        const double xv = (Part[e] + Part[TILE * INV_COLS + e]) +
+                          (Part[2 * TILE * INV_COLS + e] + Part[3 * TILE * INV_COLS + e]);
+        slot[e] = xv;
+        const int R = TILE * I + e / INV_COLS, C = j0 + e % INV_COLS;
+        if (R < N3 && C < N3) {
+          B[(size_t)R * N3p + C] = (float)xv;
+          BT[(size_t)C * N3p + R] = (float)xv;
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) t[u] = 0.0;
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
     }
     I = nI;
     d = nd;
