@@ -138,6 +138,9 @@ def lib():
         L.drape_two_level_layout.argtypes = [vp, i32p, i32p, f32p, C.POINTER(C.c_int64)]
         L.drape_two_level_coarse.argtypes = [vp, f64p]
         L.drape_copy_timing.argtypes = [C.c_int64, C.c_int32, f64p]
+        L.drape_set_prolongation_model.argtypes = [vp, f32p, C.c_int32]
+        L.drape_two_level_weights.argtypes = [vp, f32p]
+        L.drape_solution_fp64.argtypes = [vp, f64p]
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_problem.restype = C.POINTER(Problem)
@@ -221,13 +224,15 @@ def reference_scene(columns, sphere_on_floor=True, material_factor=1.0, radius_f
                       dhat=activation, kappa=0.0, lift=2.0 * radius + clearance)
 
 
-def drape_config_scene(columns, *, radius=0.15, clearance=None, dhat=1e-3, time_step=1.0 / 60.0):
-    """The quasi-static / dynamic sphere drape of the configured workloads: isotropic E = 1e6 Pa,
-    nu = 0.3, t = 0.5 mm, areal density 0.2 kg/m^2, hinge bending 1e-5 N m, a sphere of the given radius
+def drape_config_scene(columns, *, radius=0.15, clearance=None, dhat=1e-3, time_step=1.0 / 60.0, E=1e6, bend=1e-5,
+                       jitter=0.0, flip_fraction=0.0, mesh_seed=0):
+    """The quasi-static / dynamic sphere drape of the configured workloads: isotropic E (1e6 Pa), nu = 0.3,
+    t = 0.5 mm, areal density 0.2 kg/m^2, hinge bending `bend` (1e-5 N m), a sphere of the given radius
     centred under the 0.5 m sheet (resting on z = 0, no floor), barrier dhat, sheet released flat
-    `clearance` above the sphere top (2 dhat when None)."""
+    `clearance` above the sphere top (2 dhat when None); jitter / flip_fraction > 0 give an irregular mesh."""
     clearance = 2.0 * dhat if clearance is None else clearance
-    return make_scene(columns, material=isotropic_material(), bend=(1e-5, 1e-5), density=0.2, time_step=time_step,
+    return make_scene(columns, material=isotropic_material(E=E), bend=(bend, bend), density=0.2, time_step=time_step,
+                      jitter=jitter, flip_fraction=flip_fraction, mesh_seed=mesh_seed,
                       pinned=False, colliders=[(2, 0.25, 0.25, radius, radius)], dhat=dhat, kappa=0.0,
                       lift=2.0 * radius + clearance)
 
@@ -372,6 +377,26 @@ class Drape:
         out = np.zeros((3 * nc, 3 * nc))
         _check(lib().drape_two_level_coarse(self.h, _ptr(out, f64p)))
         return out
+
+    def solution_fp64(self):
+        """The FP64 iterate of the last refined solve (rules.refine_fp64)."""
+        out = np.zeros(3 * self.R)
+        _check(lib().drape_solution_fp64(self.h, _ptr(out, f64p)))
+        return out
+
+    def set_prolongation_model(self, params=None):
+        """The learned prolongation (flat float32 parameters; None: bilinear weights)."""
+        if params is None:
+            _check(lib().drape_set_prolongation_model(self.h, None, 0))
+            return
+        theta = _f32(params)
+        _check(lib().drape_set_prolongation_model(self.h, _ptr(theta, f32p), theta.size))
+
+    def two_level_weights(self):
+        """Parent-slot weights [R, 4] of the current coarse factorisation."""
+        out = np.zeros(4 * self.R, np.float32)
+        _check(lib().drape_two_level_weights(self.h, _ptr(out, f32p)))
+        return out.reshape(-1, 4)
 
     def two_level_inverse_factor(self):
         """B = L^-1 of the current coarse factorisation (FP32, dense lower triangular)."""

@@ -12,6 +12,7 @@ SOURCES = [
     "kernels/pcg.hip",
     "kernels/elements.hip",
     "kernels/twolevel.hip",
+    "kernels/gnn.hip",
     "host/context.cpp",
     "host/newton.cpp",
     "host/problem.cpp",

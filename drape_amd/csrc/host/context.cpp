@@ -40,7 +40,7 @@ static void free_context(Context* c) {
                   c->colliders, c->mu, c->unit_mass_dev, c->org, c->material, c->lgrad, c->lhess, c->bc_ptr,
                   c->bc_idx, c->fc_ptr, c->fc_idx, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->perm, c->diag_slot,
                   c->block_dst, c->csr_row_ptr, c->csr_col, c->b, c->x, c->r, c->z, c->p, c->p2, c->q, c->dinv,
-                  c->partials, c->state, c->refine, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
+                  c->partials, c->state, c->refine, c->b_saved, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->state_host) (void)hipHostFree(c->state_host);
@@ -161,6 +161,7 @@ static int alloc_solver(Context& c, const std::vector<uint32_t>& row_ptr, const 
   TRY(alloc(&c.fpart, 4 * (size_t)c.G + 16));
   TRY(alloc(&c.ipart, 2 * (size_t)c.G + 16));
   TRY(alloc(&c.refine, 6 * Rp));
+  TRY(alloc(&c.b_saved, Rp));
   c.row_ptr_host = row_ptr;
   c.col_idx_host = col_idx;
   HIP_TRY(hipEventCreate(&c.ev0));
@@ -640,6 +641,29 @@ int drape_two_level_inverse_factor(drape_ctx* p, float* factor) {
   if (!c || !factor) { set_error("null argument"); return DRAPE_E_INVALID; }
   if (!c->two_level) { set_error("no two-level layout on this context"); return DRAPE_E_INVALID; }
   return two_level_inverse_factor(*c, factor);
+}
+
+int drape_set_prolongation_model(drape_ctx* p, const float* params, int32_t count) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
+  if (!c->two_level) TRY(two_level_create(*c, c->problem.grid_columns));
+  return two_level_set_model(*c, params, count);
+}
+
+int drape_two_level_weights(drape_ctx* p, float* weights) {
+  Context* c = ctx_of(p);
+  if (!c || !weights) { set_error("null argument"); return DRAPE_E_INVALID; }
+  if (!c->two_level) { set_error("no two-level layout on this context"); return DRAPE_E_INVALID; }
+  return two_level_weights(*c, weights);
+}
+
+int drape_solution_fp64(drape_ctx* p, double* solution) {
+  Context* c = ctx_of(p);
+  if (!c || !solution) { set_error("null argument"); return DRAPE_E_INVALID; }
+  if (!c->rules.refine_fp64) { set_error("the FP64 solution exists only with rules.refine_fp64"); return DRAPE_E_STATE; }
+  HIP_TRY(hipMemcpyAsync(solution, c->refine, 3 * (size_t)c->R * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DRAPE_OK;
 }
 
 int drape_copy_timing(int64_t bytes, int32_t launches, double* avg_ms) {

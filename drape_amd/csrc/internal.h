@@ -88,6 +88,7 @@ struct Context {
   PcgState* state = nullptr;
   PcgState* state_host = nullptr; // pinned
   double* refine = nullptr;       // FP64 refinement vectors [3R] x2
+  float4* b_saved = nullptr;      // the right-hand side while the refinement loop solves for corrections
 
   // reductions for the Newton loop (FP64)
   double* dpart = nullptr;        // [2 * max groups]
@@ -159,8 +160,28 @@ int two_level_apply(Context& c, float* partials, int guard);  // z = M^-1 r and 
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host
+int two_level_set_model(Context& c, const float* theta, int count);  // learned prolongation (NULL: none)
+int two_level_weights(Context& c, float* out);         // [4R] weights of the current factorisation
 int two_level_info(const Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
                    int64_t* failures, double* factor_ms);
+
+// ---- learned prolongation (gnn.hip) ---------------------------------------------------------
+struct GnnArgs {
+  hipStream_t stream;
+  int R;
+  const float* sell_val;
+  const uint32_t* block_dst;
+  const uint32_t* diag_slot;
+  const uint32_t* row_ptr;  // CSR pattern [R + 1]
+  const uint32_t* col;      // [B]
+  const float2* fxy;        // [R] position inside the coarse cell
+  const float* wbil;        // [4R] bilinear weights (0 on empty slots)
+  const float* theta;       // parameters
+  float *scale, *features, *edge, *h0, *h1, *msg;  // scratch: [R], [13R], [B], [32R] x 3
+  float* w;                 // [4R] output weights
+};
+int gnn_param_count();
+int gnn_infer(const GnnArgs& a);
 
 // ---- helpers --------------------------------------------------------------------------------
 inline int blocks_for(int n, int per = WG) { return (n + per - 1) / per; }

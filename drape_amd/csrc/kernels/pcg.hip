@@ -507,7 +507,7 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
 // The preconditioner the rules select. A two-level solve falls back to block-Jacobi when the coarse
 // operator is not positive definite, and on a breakdown either re-solves with block-Jacobi or reports
 // the breakdown (rules.fallback); every such event is counted.
-int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
+static int pcg_solve_once(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
   if (c.rules.preconditioner == DRAPE_PRECOND_TWO_LEVEL && c.two_level) {
     bool ok = false;
     TRY_RC(two_level_prepare(c, &ok));
@@ -530,6 +530,153 @@ int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats*
     }
   }
   return pcg_run(c, tolerance, max_iterations, stats, false);
+}
+
+// ---- FP64 iterative refinement around the FP32 PCG ------------------------------------------------
+// r = b - A x with the FP32 matrix values and FP64 x and sums; partial |r|^2 per 256-row group by logical row.
+__global__ void __launch_bounds__(256) k_residual64(SellView A, const double* __restrict__ x,
+                                                   const float4* __restrict__ b, double* __restrict__ r,
+                                                   double* __restrict__ part) {
+  __shared__ double lds[256];
+  const int phys = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t row = A.perm[phys];
+  double rr = 0.0;
+  if (row != 0xffffffffu) {
+    const int sl = phys >> 6, l = phys & 63;
+    const uint32_t base = A.slice_ptr[sl], n = A.cnt[phys];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t q = base + k;
+      const uint32_t col = A.col[q * 64 + l];
+      const float* v = A.val + (size_t)q * 9 * 64 + l;
+      const double x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];
+      a0 += (double)v[0] * x0 + (double)v[64] * x1 + (double)v[128] * x2;
+      a1 += (double)v[192] * x0 + (double)v[256] * x1 + (double)v[320] * x2;
+      a2 += (double)v[384] * x0 + (double)v[448] * x1 + (double)v[512] * x2;
+    }
+    const float4 bb = b[row];
+    const double r0 = (double)bb.x - a0, r1 = (double)bb.y - a1, r2 = (double)bb.z - a2;
+    r[3 * row] = r0;
+    r[3 * row + 1] = r1;
+    r[3 * row + 2] = r2;
+    rr = r0 * r0 + r1 * r1 + r2 * r2;
+  }
+  // logical slot order, then a fixed tree
+  const uint32_t t = row == 0xffffffffu ? threadIdx.x : row - blockIdx.x * 256;
+  lds[t] = rr;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = lds[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum64(const double* __restrict__ part, int count, double* __restrict__ out) {
+  __shared__ double lds[256];
+  double a = 0.0;
+  for (int j = threadIdx.x; j < count; j += 256) a += part[j];
+  lds[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = lds[0];
+}
+
+// b_pcg = f32(r); and after the inner solve x64 += x32
+__global__ void k_refine_rhs(const double* __restrict__ r, float4* __restrict__ b, int R) {
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row < R) b[row] = make_float4((float)r[3 * row], (float)r[3 * row + 1], (float)r[3 * row + 2], 0.0f);
+}
+__global__ void k_refine_add(const float4* __restrict__ d, double* __restrict__ x, int R) {
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row < R) {
+    const float4 v = d[row];
+    x[3 * row] += (double)v.x;
+    x[3 * row + 1] += (double)v.y;
+    x[3 * row + 2] += (double)v.z;
+  }
+}
+// x32 = f32(x64); b = saved b
+__global__ void k_refine_finish(const double* __restrict__ x64, float4* __restrict__ x, const float4* __restrict__ saved,
+                                float4* __restrict__ b, int R) {
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row < R) {
+    x[row] = make_float4((float)x64[3 * row], (float)x64[3 * row + 1], (float)x64[3 * row + 2], 0.0f);
+    b[row] = saved[row];
+  }
+}
+
+constexpr int MAX_REFINEMENTS = 8;
+constexpr double INNER_FLOOR = 1e-3;  // the FP32 solve is asked for no more than three decades at a time
+
+// FP32 PCG inside an FP64 true-residual loop: r = b - A x in FP64, solve A d = r in FP32 to
+// max(tol |b| / |r|, 1e-3), x += d in FP64, until |b - A x| <= tol |b| (at most 8 passes). The solution
+// left in x (FP32) and every statistic refer to the FP64 iterate.
+static int pcg_refine(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
+  const int R = c.R, G = c.G;
+  const size_t Rp = (size_t)G * WG;
+  double* x64 = c.refine;
+  double* r64 = c.refine + 3 * Rp;
+  hipStream_t s = c.stream;
+  SellView A = view_of(c);
+  HIP_TRY(hipMemcpyAsync(c.b_saved, c.b, (size_t)R * sizeof(float4), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipMemsetAsync(x64, 0, 3 * Rp * sizeof(double), s));
+  double bb = 0.0, rel = 1.0;
+  int total = 0, passes = 0, status = 0;
+  drape_pcg_stats inner{};
+  for (int pass = 0;; ++pass) {
+    k_residual64<<<G, 256, 0, s>>>(A, x64, c.b_saved, r64, c.dpart);
+    k_sum64<<<1, 256, 0, s>>>(c.dpart, G, c.dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c.dout_host, c.dout, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const double rr = c.dout_host[0];
+    if (pass == 0) bb = rr;
+    rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    if (!(rel > tolerance)) {
+      status = 1;
+      break;
+    }
+    if (pass == MAX_REFINEMENTS || total >= max_iterations) {
+      status = 2;
+      break;
+    }
+    k_refine_rhs<<<blocks_for(R), 256, 0, s>>>(r64, c.b, R);
+    HIP_TRY(hipGetLastError());
+    const double inner_tol = std::max(tolerance / rel, INNER_FLOOR);
+    TRY_RC(pcg_solve_once(c, inner_tol, max_iterations - total, &inner));
+    total += inner.iterations;
+    if (inner.status == 3) {  // the correction is discarded; the Newton loop sees the breakdown
+      status = 3;
+      break;
+    }
+    k_refine_add<<<blocks_for(R), 256, 0, s>>>(c.x, x64, R);
+    HIP_TRY(hipGetLastError());
+    ++passes;
+    if (inner.status == 4) {  // truncated at negative curvature: keep the descent step, stop refining
+      status = 4;
+      break;
+    }
+  }
+  k_refine_finish<<<blocks_for(R), 256, 0, s>>>(x64, c.x, c.b_saved, c.b, R);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  stats->status = status;
+  stats->iterations = total;
+  stats->recursive_relative_residual = rel;
+  stats->true_relative_residual = rel;
+  stats->chunks = inner.chunks;
+  stats->refinements = passes;
+  stats->fp64_relative_residual = rel;
+  return DRAPE_OK;
+}
+
+int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
+  if (c.rules.refine_fp64) return pcg_refine(c, tolerance, max_iterations, stats);
+  return pcg_solve_once(c, tolerance, max_iterations, stats);
 }
 
 // The SpMV alone, `launches` times back to back between two events on the context's stream (p -> q);

@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <string>
 #include <vector>
 
 #include "../internal.h"
@@ -49,6 +50,10 @@ struct TwoLevel {
   std::vector<float> bilinear;   // [4R] default weights
   int32_t* d_parent = nullptr;
   float* d_w = nullptr;          // [4R] current weights
+  float* d_wbil = nullptr;       // [4R] bilinear weights
+  float2* d_fxy = nullptr;       // [R] position of each row inside its coarse cell
+  float* d_theta = nullptr;      // learned-prolongation parameters (null: fixed weights)
+  float* d_gnn = nullptr;        // its scratch: scale [R], features [13R], edge [B], activations [32R] x 3
   uint32_t* d_child_ptr = nullptr;  // [Nc+1]
   uint32_t* d_child = nullptr;      // 4 i + k of each child slot, in fine-row order
   uint32_t* d_pair_ptr = nullptr;   // [npairs+1]
@@ -73,7 +78,8 @@ struct TwoLevel {
 
 void two_level_destroy(TwoLevel* t) {
   if (!t) return;
-  for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
+  for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_wbil, (void*)t->d_fxy, (void*)t->d_theta,
+                  (void*)t->d_gnn, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
                   (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT,
                   (void*)t->d_t, (void*)t->d_rc, (void*)t->d_yc, (void*)t->d_info})
     if (p) (void)hipFree(p);
@@ -710,6 +716,13 @@ int two_level_create(Context& c, int columns) {
       t->bilinear[4 * row + k] = (float)wt[k];
     }
   }
+  std::vector<float2> fxy(c.R);
+  for (int row = 0; row < c.R; ++row) {
+    const int v = (int)c.row_vertex[row];
+    const int x = v % n, y = v / n;
+    const int a0 = std::min(x / 4, nc - 2), b0 = std::min(y / 4, nc - 2);
+    fxy[row] = make_float2((float)((x - 4.0 * a0) / 4.0), (float)((y - 4.0 * b0) / 4.0));
+  }
   // children of each coarse node, in fine-row order
   std::vector<std::vector<uint32_t>> kids(Nc);
   for (int row = 0; row < c.R; ++row)
@@ -765,6 +778,8 @@ int two_level_create(Context& c, int columns) {
   if ((e = hipMalloc((void**)&(ptr), std::max<size_t>(1, (n)) * sizeof(*(ptr)))) != hipSuccess) return fail(e)
   TL_ALLOC(t->d_parent, t->parent.size());
   TL_ALLOC(t->d_w, t->bilinear.size());
+  TL_ALLOC(t->d_wbil, t->bilinear.size());
+  TL_ALLOC(t->d_fxy, fxy.size());
   TL_ALLOC(t->d_child_ptr, child_ptr.size());
   TL_ALLOC(t->d_child, child.size());
   TL_ALLOC(t->d_pair_ptr, pair_ptr.size());
@@ -786,6 +801,8 @@ int two_level_create(Context& c, int columns) {
   };
   if ((e = up(t->d_parent, t->parent.data(), t->parent.size() * 4)) != hipSuccess ||
       (e = up(t->d_w, t->bilinear.data(), t->bilinear.size() * 4)) != hipSuccess ||
+      (e = up(t->d_wbil, t->bilinear.data(), t->bilinear.size() * 4)) != hipSuccess ||
+      (e = up(t->d_fxy, fxy.data(), fxy.size() * sizeof(float2))) != hipSuccess ||
       (e = up(t->d_child_ptr, child_ptr.data(), child_ptr.size() * 4)) != hipSuccess ||
       (e = up(t->d_child, child.data(), child.size() * 4)) != hipSuccess ||
       (e = up(t->d_pair_ptr, pair_ptr.data(), pair_ptr.size() * 4)) != hipSuccess ||
@@ -813,8 +830,66 @@ int two_level_set_weights(Context& c, const float* w) {
   for (size_t i = 0; i < v.size(); ++i) v[i] = t->parent[i] >= 0 ? (w ? w[i] : t->bilinear[i]) : 0.0f;
   HIP_TRY(hipMemcpyAsync(t->d_w, v.data(), v.size() * 4, hipMemcpyHostToDevice, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
+  if (t->d_theta) {  // fixed weights replace the learned prolongation
+    HIP_TRY(hipFree(t->d_theta));
+    t->d_theta = nullptr;
+  }
   t->factored = ~0ull;
   return DRAPE_OK;
+}
+
+// The learned prolongation: from the next factorisation on, the weights come from the network evaluated on
+// each assembled system (null theta: back to the bilinear weights).
+int two_level_set_model(Context& c, const float* theta, int count) {
+  TwoLevel* t = c.two_level;
+  if (!t) {
+    set_error("no two-level layout on this context");
+    return DRAPE_E_INVALID;
+  }
+  if (!theta) return two_level_set_weights(c, nullptr);
+  if (count != gnn_param_count()) {
+    set_error("prolongation model: expected " + std::to_string(gnn_param_count()) + " parameters");
+    return DRAPE_E_INVALID;
+  }
+  if (!t->d_theta) HIP_TRY(hipMalloc((void**)&t->d_theta, (size_t)count * sizeof(float)));
+  if (!t->d_gnn) {
+    const size_t n = (size_t)c.R * (1 + 13 + 3 * 32) + (size_t)c.B;
+    HIP_TRY(hipMalloc((void**)&t->d_gnn, n * sizeof(float)));
+  }
+  HIP_TRY(hipMemcpyAsync(t->d_theta, theta, (size_t)count * sizeof(float), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  t->factored = ~0ull;
+  return DRAPE_OK;
+}
+
+int two_level_weights(Context& c, float* out) {
+  TwoLevel* t = c.two_level;
+  HIP_TRY(hipMemcpyAsync(out, t->d_w, 4 * (size_t)c.R * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  return DRAPE_OK;
+}
+
+static int run_model(Context& c) {
+  TwoLevel* t = c.two_level;
+  GnnArgs a;
+  a.stream = c.stream;
+  a.R = c.R;
+  a.sell_val = c.sell_val;
+  a.block_dst = c.block_dst;
+  a.diag_slot = c.diag_slot;
+  a.row_ptr = c.csr_row_ptr;
+  a.col = c.csr_col;
+  a.fxy = t->d_fxy;
+  a.wbil = t->d_wbil;
+  a.theta = t->d_theta;
+  a.scale = t->d_gnn;
+  a.features = a.scale + c.R;
+  a.edge = a.features + 13 * (size_t)c.R;
+  a.h0 = a.edge + c.B;
+  a.h1 = a.h0 + 32 * (size_t)c.R;
+  a.msg = a.h1 + 32 * (size_t)c.R;
+  a.w = t->d_w;
+  return gnn_infer(a);
 }
 
 // A_c = P^T A P of the assembled system, its Cholesky factor and the factor's inverse, unless the current
@@ -842,6 +917,7 @@ int two_level_prepare(Context& c, bool* ok) {
     std::chrono::steady_clock::time_point t0;
     ~Clock() { t->factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
   } clock{t, started};
+  if (t->d_theta) TRY_RC(run_model(c));
   HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * TILE * TILE * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
   k_galerkin_band<<<blocks_for(9 * t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,

@@ -227,6 +227,15 @@ int drape_set_preconditioner(drape_ctx* ctx, int32_t kind, const float* weights)
 int drape_two_level_layout(drape_ctx* ctx, int32_t* coarse_rows, int32_t* parent, float* bilinear,
                            int64_t* counters);
 
+/* The learned prolongation: `count` parameters of the graph network (13 input features, width 32, three
+ * message-passing layers, 4 parent-slot outputs: W0 [32x13], b0 [32], then per layer Ws [32x32], Wn [32x32],
+ * b [32], then Wo [4x32], bo [4]). From the next factorisation on, the two-level preconditioner takes its
+ * weights from the network evaluated on each assembled system; NULL returns to the bilinear weights.
+ * No reference counterpart (the learned two-level preconditioner is new). */
+int drape_set_prolongation_model(drape_ctx* ctx, const float* params, int32_t count);
+/* The parent-slot weights [4R] of the current coarse factorisation. */
+int drape_two_level_weights(drape_ctx* ctx, float* weights);
+
 /* The Galerkin coarse operator A_c = P^T A P of the assembled system, row-major [3Nc x 3Nc] doubles (Nc from
  * drape_two_level_layout), for checking; the preconditioner's factorisation is left as it is. */
 int drape_two_level_coarse(drape_ctx* ctx, double* coarse);
@@ -234,6 +243,10 @@ int drape_two_level_coarse(drape_ctx* ctx, double* coarse);
 /* B = L^-1 (A_c = L L^T) of the current coarse factorisation as FP32, row-major [3Nc x 3Nc] (zero above the
  * diagonal), for checking; the coarse solve applies B^T B. */
 int drape_two_level_inverse_factor(drape_ctx* ctx, float* factor);
+
+/* The FP64 iterate [3R] of the last drape_pcg under rules.refine_fp64 (drape_pcg's FP32 solution rounds it).
+ * No reference counterpart (FP64 refinement is new). */
+int drape_solution_fp64(drape_ctx* ctx, double* solution);
 
 /* Device timing of the last drape_pcg (HIP events on the context's stream): average SpMV kernel time
  * and the whole solve, in milliseconds. */

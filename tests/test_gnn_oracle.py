@@ -1,0 +1,94 @@
+"""Pins of the learned-prolongation oracle (oracle/gnn.py): there is no reference counterpart, so the network,
+its loss and its gradient are pinned by mathematics -- the identity at a zero output layer, the loss on
+errors inside and A-orthogonal to the coarse space, and central finite differences of the loss."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from oracle import gnn as G
+from oracle import two_level as TL
+
+
+@pytest.fixture(scope="module")
+def graph17():
+    od = O.Drape(O.reference_scene(17, sphere=False, placed=True))
+    a = od.problem_arrays()
+    rest = od.rest_positions()
+    x_hat = rest.copy()
+    x_hat[2::3] -= 1e-3
+    od.set_time_step(0.1)
+    rhs, vals = od.assemble(rest, x_hat, 0.0, True)
+    return G.Graph(17, a["row_vertex"], a["row_ptr"], a["col_idx"], vals)
+
+
+def test_zero_output_layer_returns_bilinear(graph17):
+    theta = G.init_params(seed=3)
+    w = G.forward(theta, graph17)
+    assert np.allclose(w, graph17.w_bil, rtol=0, atol=1e-15)
+
+
+def test_features_are_scale_invariant(graph17):
+    # the same system scaled by 7: features and edge weights unchanged
+    od = O.Drape(O.reference_scene(17, sphere=False, placed=True))
+    a = od.problem_arrays()
+    blocks = graph17.A
+    vals = np.zeros(9 * len(a["col_idx"]))
+    rows = np.repeat(np.arange(graph17.R), np.diff(a["row_ptr"]))
+    dense = blocks.tocsr()
+    for b, (i, j) in enumerate(zip(rows, a["col_idx"])):
+        vals[9 * b:9 * b + 9] = dense[3 * i:3 * i + 3, 3 * j:3 * j + 3].toarray().ravel()
+    g2 = G.Graph(17, a["row_vertex"], a["row_ptr"], a["col_idx"], 7.0 * vals)
+    assert np.allclose(g2.f, graph17.f, rtol=1e-12, atol=1e-12)
+    assert np.allclose(g2.c, graph17.c, rtol=1e-12, atol=1e-14)
+
+
+def test_loss_is_zero_in_range_and_one_when_orthogonal(graph17):
+    g = graph17
+    w = g.w_bil
+    P = TL.prolongation(g.parent, w, g.coarse_rows)
+    rng = np.random.default_rng(1)
+    E_in = P @ rng.standard_normal((3 * g.coarse_rows, 2))
+    L_in, _ = G.loss_and_weight_gradient(w, g, E_in)
+    assert L_in < 1e-20
+    # A-orthogonal to range(P): e - P A_c^-1 P^T A e
+    E = rng.standard_normal((3 * g.R, 2))
+    Ac = (P.T @ (g.A @ P)).toarray()
+    E_orth = E - P @ np.linalg.solve(Ac, P.T @ (g.A @ E))
+    L_orth, _ = G.loss_and_weight_gradient(w, g, E_orth)
+    assert abs(L_orth - 1.0) < 1e-9
+
+
+def test_weight_gradient_matches_finite_differences(graph17):
+    g = graph17
+    E = G.smooth_vectors(g, 3, seed=2)
+    w = g.w_bil.copy()
+    L0, gw = G.loss_and_weight_gradient(w, g, E)
+    rng = np.random.default_rng(4)
+    for _ in range(6):
+        i, k = rng.integers(g.R), rng.integers(4)
+        if not g.valid[i, k]:
+            continue
+        h = 1e-6
+        wp, wm = w.copy(), w.copy()
+        wp[i, k] += h
+        wm[i, k] -= h
+        fd = (G.loss_and_weight_gradient(wp, g, E)[0] - G.loss_and_weight_gradient(wm, g, E)[0]) / (2 * h)
+        assert fd == pytest.approx(gw[i, k], rel=1e-5, abs=1e-10)
+
+
+def test_parameter_gradient_matches_finite_differences(graph17):
+    g = graph17
+    E = G.smooth_vectors(g, 2, seed=5)
+    theta = G.init_params(seed=7)
+    rng = np.random.default_rng(8)
+    theta = theta + 0.05 * rng.standard_normal(theta.size)   # a non-trivial output layer
+    L, grad = G.backward(theta, g, E)
+    picks = rng.choice(theta.size, 12, replace=False)
+    picks = np.concatenate([picks, [theta.size - 1, theta.size - 5, 0]])
+    for j in picks:
+        h = 1e-6
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd = (G.loss(tp, g, E) - G.loss(tm, g, E)) / (2 * h)
+        assert fd == pytest.approx(grad[j], rel=2e-4, abs=1e-9), j
