@@ -148,7 +148,18 @@ def main() -> None:
     ms_per_newton = 1000.0 * res.seconds / newton
     pcg_per_newton = res.pcg_iterations / newton
     comparison = None
+    learned = None
     if kind != 0 and not args.no_compare:
+        # the same steps with the learned prolongation (trained parameters shipped with the package)
+        from drape_amd.learned import load_model
+        lctx = D.Drape(arrays, D.default_rules(preconditioner=1))
+        lctx.set_prolongation_model(load_model())
+        _, lres, lnewton, _ = timed(lctx, "two_level_learned")
+        learned = {"ms_per_newton": round(1000.0 * lres.seconds / lnewton, 4),
+                   "pcg_per_newton": round(lres.pcg_iterations / lnewton, 2),
+                   "newton_iterations": lres.newton_iterations, "pcg_iterations": lres.pcg_iterations,
+                   "fallbacks": lres.fallbacks, "accepted_steps": lres.accepted_steps}
+        lctx.close()
         bj = D.Drape(arrays, D.default_rules(preconditioner=0))
         _, bres, bnewton, _ = timed(bj, "block_jacobi")
         comparison = {"ms_per_newton": round(1000.0 * bres.seconds / bnewton, 4),
@@ -189,6 +200,7 @@ def main() -> None:
                    "coarse_rows": layout[0] if layout else None,
                    "factorisations": layout[3][0] if layout else None,
                    "factorisation_ms": round(layout[3][3] / 1000.0, 2) if layout else None,
+                   "learned_prolongation": learned,
                    "block_jacobi": comparison},
         "roofline": roofline,
     }

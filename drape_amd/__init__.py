@@ -141,6 +141,8 @@ def lib():
         L.drape_set_prolongation_model.argtypes = [vp, f32p, C.c_int32]
         L.drape_two_level_weights.argtypes = [vp, f32p]
         L.drape_solution_fp64.argtypes = [vp, f64p]
+        L.drape_simulate.argtypes = [vp, C.c_int32, f32p, f32p, f32p, f32p, i32p, f32p, C.POINTER(Record), C.c_int32,
+                                     i32p]
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_problem.restype = C.POINTER(Problem)
@@ -414,6 +416,21 @@ class Drape:
         _check(lib().drape_newton_step(self.h, _ptr(x, f32p), scale, step, _ptr(origin, f32p), recs, cap,
                                        C.byref(n)))
         return x, [recs[i] for i in range(min(n.value, cap))]
+
+    def simulate(self, steps, x0=None, v0=None, keep=20000):
+        """Backward-Euler trajectory (drape.rs simulate): final positions and velocities, per-step Newton
+        updates and final relative residuals, records."""
+        x = np.zeros(3 * self.V, np.float32)
+        v = np.zeros(3 * self.V, np.float32)
+        up = np.zeros(max(steps, 1), np.int32)
+        rel = np.zeros(max(steps, 1), np.float32)
+        recs = (Record * keep)()
+        n = C.c_int32()
+        _check(lib().drape_simulate(self.h, steps, None if x0 is None else _ptr(_f32(x0), f32p),
+                                    None if v0 is None else _ptr(_f32(v0), f32p), _ptr(x, f32p), _ptr(v, f32p),
+                                    _ptr(up, i32p), _ptr(rel, f32p), recs, keep, C.byref(n)))
+        return {"positions": x, "velocities": v, "updates": up[:steps], "relative": rel[:steps],
+                "records": [recs[i] for i in range(min(n.value, keep))]}
 
     def solve_equilibrium(self, start=None, h_start=None, cap=400, budget_s=0.0, keep=20000):
         res = PtcResult()

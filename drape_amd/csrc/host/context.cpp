@@ -40,7 +40,7 @@ static void free_context(Context* c) {
                   c->colliders, c->mu, c->unit_mass_dev, c->org, c->material, c->lgrad, c->lhess, c->bc_ptr,
                   c->bc_idx, c->fc_ptr, c->fc_idx, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->perm, c->diag_slot,
                   c->block_dst, c->csr_row_ptr, c->csr_col, c->b, c->x, c->r, c->z, c->p, c->p2, c->q, c->dinv,
-                  c->partials, c->state, c->refine, c->b_saved, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
+                  c->partials, c->state, c->refine, c->b_saved, c->vel, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->state_host) (void)hipHostFree(c->state_host);
@@ -240,6 +240,7 @@ static int create_mesh_context(const drape_problem* pr, const drape_rules* rules
   }
   for (int v = 0; v < c.V; ++v)
     pos[v] = make_float4(pr->rest_positions[3 * v], pr->rest_positions[3 * v + 1], pr->rest_positions[3 * v + 2], 0.0f);
+  c.rest_host.assign(pr->rest_positions, pr->rest_positions + 3 * (size_t)c.V);
   // scatter maps sorted by row: per CSR block the local 3x3 blocks (units of 9 floats) in the order
   // row term, membranes (9e + 3k + l), hinges (9T + 16h + 4k + l); per row the local gradients
   // (units of 3 floats): row term, membranes (3e + k), hinges (3T + 4h + k).

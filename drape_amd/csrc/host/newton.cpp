@@ -459,6 +459,40 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
   return DRAPE_OK;
 }
 
+// drape.rs:426-439 simulate / time_step: per step x_hat = x + h v + h^2 g, the residual scale |g(x_n)|, the
+// start (x_hat, cut by CCD when colliders exist), one robust Newton solve, v = (x_{n+1} - x_n) / h.
+int simulate(Context& c, int steps, std::vector<int32_t>& updates, std::vector<float>& relative,
+             std::vector<drape_record>& records_out) {
+  Run run(c);
+  const double h = c.problem.time_step;
+  for (int n = 0; n < steps; ++n) {
+    HIP_TRY(hipMemcpyAsync(c.org, c.pos, (size_t)c.V * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+    int rc = launch_dyn_predict(c, c.vel, h);
+    if (rc != DRAPE_OK) return rc;
+    note_shift(c, 0.0f, 0.0);
+    rc = launch_set_float(c, c.mu, 0.0f);
+    if (rc != DRAPE_OK) return rc;
+    Residual r;
+    rc = residual(run, &r);
+    if (rc != DRAPE_OK) return rc;
+    const float scale = r.norm > 0.0f ? r.norm : 1.0f;
+    rc = newton_start(run);
+    if (rc != DRAPE_OK) return rc;
+    const size_t first = run.records.size();
+    rc = solve_step_robust(run, scale, n + 1);
+    if (rc != DRAPE_OK) return rc;
+    int n_up = 0;
+    for (size_t i = first; i < run.records.size(); ++i)
+      if (run.records[i].status == RUNNING && std::isfinite(run.records[i].pcg_iterations)) ++n_up;
+    updates.push_back(n_up);
+    relative.push_back(run.records.empty() ? NAN : (float)run.records.back().relative_residual);
+    rc = launch_dyn_velocity(c, c.vel, h);
+    if (rc != DRAPE_OK) return rc;
+  }
+  records_out = std::move(run.records);
+  return DRAPE_OK;
+}
+
 int newton_step(Context& c, float scale, int step, std::vector<drape_record>& records_out) {
   Run run(c);
   int rc = solve_step_robust(run, scale, step);
@@ -490,6 +524,52 @@ int drape_newton_step(drape_ctx* p, float* x_inout, float scale, int32_t step, c
   HIP_TRY(hipMemcpyAsync(v.data(), c->pos, v.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->V; ++i) { x_inout[3 * i] = v[i].x; x_inout[3 * i + 1] = v[i].y; x_inout[3 * i + 2] = v[i].z; }
+  const int n = (int)recs.size();
+  if (records)
+    for (int i = 0; i < n && i < cap; ++i) records[i] = recs[i];
+  if (count) *count = n;
+  return DRAPE_OK;
+}
+
+int drape_simulate(drape_ctx* p, int32_t steps, const float* x0, const float* v0, float* x_out, float* v_out,
+                   int32_t* updates, float* final_relative, drape_record* records, int32_t cap, int32_t* count) {
+  Context* c = reinterpret_cast<Context*>(p);
+  if (!c || c->linear || steps < 0) {
+    set_error("invalid simulation arguments");
+    return DRAPE_E_INVALID;
+  }
+  if (!c->vel && hipMalloc((void**)&c->vel, (size_t)c->V * sizeof(float4)) != hipSuccess) {
+    set_error("velocity buffer allocation failed");
+    return DRAPE_E_DEVICE;
+  }
+  std::vector<float4> v(c->V, make_float4(0, 0, 0, 0));
+  const float* src = x0 ? x0 : c->rest_host.data();
+  for (int i = 0; i < c->V; ++i) v[i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.0f);
+  HIP_TRY(hipMemcpyAsync(c->pos, v.data(), v.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->V; ++i)
+    v[i] = v0 ? make_float4(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2], 0.0f) : make_float4(0, 0, 0, 0);
+  HIP_TRY(hipMemcpyAsync(c->vel, v.data(), v.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<int32_t> up;
+  std::vector<float> rel;
+  std::vector<drape_record> recs;
+  int rc = simulate(*c, steps, up, rel, recs);
+  if (rc != DRAPE_OK) return rc;
+  auto out3 = [&](const float4* d, float* host) -> int {
+    if (!host) return DRAPE_OK;
+    HIP_TRY(hipMemcpyAsync(v.data(), d, v.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->V; ++i) { host[3 * i] = v[i].x; host[3 * i + 1] = v[i].y; host[3 * i + 2] = v[i].z; }
+    return DRAPE_OK;
+  };
+  rc = out3(c->pos, x_out);
+  if (rc == DRAPE_OK) rc = out3(c->vel, v_out);
+  if (rc != DRAPE_OK) return rc;
+  for (int n = 0; n < steps; ++n) {
+    if (updates) updates[n] = up[n];
+    if (final_relative) final_relative[n] = rel[n];
+  }
   const int n = (int)recs.size();
   if (records)
     for (int i = 0; i < n && i < cap; ++i) records[i] = recs[i];

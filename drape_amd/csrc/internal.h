@@ -42,6 +42,7 @@ struct Context {
   std::vector<uint32_t> row_vertex;
   std::vector<float> unit_mass, colliders_host, row_mass_host;
   std::vector<uint32_t> row_ptr_host, col_idx_host;
+  std::vector<float> rest_host;   // rest positions [3V]
 
   // geometry (device)
   float4* pos = nullptr;      // [V]
@@ -89,6 +90,7 @@ struct Context {
   PcgState* state_host = nullptr; // pinned
   double* refine = nullptr;       // FP64 refinement vectors [3R] x2
   float4* b_saved = nullptr;      // the right-hand side while the refinement loop solves for corrections
+  float4* vel = nullptr;          // [V] velocities of the dynamic loop (allocated on first use)
 
   // reductions for the Newton loop (FP64)
   double* dpart = nullptr;        // [2 * max groups]
@@ -147,6 +149,8 @@ int launch_ptc_prepare(Context& c, const float4* base, double h);  // x_hat, M/h
 int spmv_timing(Context& c, int launches, double* avg_ms);
 int copy_timing(size_t bytes, int launches, double* avg_ms);  // known-size copy kernel
 int launch_set_float(Context& c, float* dst, float v);  // device scalar write by kernel argument
+int launch_dyn_predict(Context& c, const float4* vel, double h);  // x_hat = x + h v + h^2 g on the free rows
+int launch_dyn_velocity(Context& c, float4* vel, double h);       // v = (pos - org) / h
 
 // ---- PCG (pcg.hip) --------------------------------------------------------------------------
 int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats);

@@ -624,6 +624,46 @@ __global__ void k_ptc_fall(const float4* __restrict__ pos, float4* __restrict__ 
   xhat[v] = h;
 }
 
+// drape.rs:869 prediction (x_hat = x + h v + h^2 g on the free vertices, x elsewhere), in FP64 then rounded.
+__global__ void k_dyn_predict(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                              const uint32_t* __restrict__ rvert, int V, int R, double h, double g,
+                              float4* __restrict__ xhat) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < V) xhat[i] = pos[i];
+}
+__global__ void k_dyn_predict_rows(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                   const uint32_t* __restrict__ rvert, int R, double h, double g,
+                                   float4* __restrict__ xhat) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= R) return;
+  const uint32_t v = rvert[row];
+  const float4 x = pos[v], u = vel[v];
+  xhat[v] = make_float4((float)((double)x.x + h * u.x), (float)((double)x.y + h * u.y),
+                        (float)((double)x.z + h * u.z - h * h * g), 0.0f);
+}
+// v = (x_new - x_old) / h for every vertex
+__global__ void k_dyn_velocity(const float4* __restrict__ pos, const float4* __restrict__ org, int V, double h,
+                               float4* __restrict__ vel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  const float4 a = pos[i], b = org[i];
+  vel[i] = make_float4((float)(((double)a.x - b.x) / h), (float)(((double)a.y - b.y) / h),
+                       (float)(((double)a.z - b.z) / h), 0.0f);
+}
+
+int launch_dyn_predict(Context& c, const float4* vel, double h) {
+  k_dyn_predict<<<blocks_for(c.V), 256, 0, c.stream>>>(c.pos, vel, c.rvert, c.V, c.R, h, c.problem.gravity, c.xhat);
+  k_dyn_predict_rows<<<blocks_for(c.R), 256, 0, c.stream>>>(c.pos, vel, c.rvert, c.R, h, c.problem.gravity, c.xhat);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+int launch_dyn_velocity(Context& c, float4* vel, double h) {
+  k_dyn_velocity<<<blocks_for(c.V), 256, 0, c.stream>>>(c.pos, c.org, c.V, h, vel);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
 __global__ void k_set_float(float* dst, float v) {
   if (threadIdx.x == 0) dst[0] = v;
 }

@@ -267,6 +267,7 @@ def main() -> None:
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--budget-s", type=float, default=420.0)
     ap.add_argument("--k", type=int, default=20, help="PCG iterations unrolled in the loss")
+    ap.add_argument("--log-loss", action="store_true", help="minimise the mean log of the A-norm error ratio")
     args = ap.parse_args()
     device = torch.device("cuda" if torch.cuda.is_available() else "cpu")
     t0 = time.time()
@@ -306,7 +307,8 @@ def main() -> None:
 
     def mean_loss(group, th):
         with torch.no_grad():
-            return float(np.mean([pcg_loss(forward(th, s), s, s.b, s.xstar, args.k).item() for s in group]))
+            vals = [pcg_loss(forward(th, s), s, s.b, s.xstar, args.k).item() for s in group]
+            return float(np.exp(np.mean(np.log(vals)))) if args.log_loss else float(np.mean(vals))
 
     zero = torch.zeros_like(theta)
     base_train, base_val = mean_loss(train, zero), mean_loss(val, zero)
@@ -320,6 +322,8 @@ def main() -> None:
             opt.zero_grad()
             try:
                 L = pcg_loss(forward(theta, s), s, s.b, s.xstar, args.k)
+                if args.log_loss:
+                    L = torch.log(L)
             except np.linalg.LinAlgError:
                 continue
             L.backward()
@@ -333,6 +337,7 @@ def main() -> None:
             "params": [float(v) for v in theta.detach().cpu().numpy().astype(np.float32)],
             "training": {"scenes": scenes, "states_per_scene": args.states, "test_vectors": args.vectors,
                          "epochs": len(history), "lr": args.lr, "seed": args.seed, "pcg_iterations_in_loss": args.k,
+                         "log_loss": bool(args.log_loss),
                          "loss_bilinear": {"train": base_train, "validation": base_val},
                          "loss_learned": {"train": history[-1][1], "validation": history[-1][2]}}}
     with open(args.out, "w") as f:

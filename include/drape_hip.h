@@ -196,6 +196,14 @@ int drape_max_step(drape_ctx* ctx, float* alpha, int32_t* vertex, float* min_dis
 int drape_newton_step(drape_ctx* ctx, float* x_inout, float scale, int32_t step, const float* origin,
                       drape_record* records, int32_t cap, int32_t* count);
 
+/* Dynamic trajectory by backward Euler (drape.rs:426-439 simulate; time_step with the prediction
+ * x_hat = x + h v + h^2 g of drape.rs prediction; one robust Newton solve per step as solveStepRobust,
+ * robust-newton.ts:231-250): `steps` steps of problem.time_step from x0 / v0 (NULL: rest, zero). Outputs
+ * (any may be NULL): final positions and velocities [3V], per-step Newton updates and final relative
+ * residual [steps], the records (capacity `cap`). */
+int drape_simulate(drape_ctx* ctx, int32_t steps, const float* x0, const float* v0, float* x_out, float* v_out,
+                   int32_t* updates, float* final_relative, drape_record* records, int32_t cap, int32_t* count);
+
 /* Quasi-static drape by pseudo-transient continuation from `start` with first step size h_start
  * (drape-e7.ts:156-181 solveEquilibrium). `cap` accepted steps, `budget_s` > 0 a wall-clock budget.
  * Any output array may be NULL; final positions go to `final_positions` [3V]. */
