@@ -195,7 +195,7 @@ def main() -> None:
                                "sphere r=0.15 m, dhat=1 mm",
                    "preconditioner": args.preconditioner, "pcg_per_newton": round(pcg_per_newton, 2),
                    "newton_iterations": res.newton_iterations, "pcg_iterations": res.pcg_iterations,
-                   "accepted_steps": res.accepted_steps, "phase_ms": [round(v, 2) for v in res.phase_ms],
+                   "accepted_steps": res.accepted_steps, "phase_ms": {k: round(v, 2) for k, v in zip(D.PHASES, res.phase_ms)},
                    "fallbacks": res.fallbacks, "wall_s": round(wall, 3),
                    "coarse_rows": layout[0] if layout else None,
                    "factorisations": layout[3][0] if layout else None,

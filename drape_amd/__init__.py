@@ -84,7 +84,11 @@ class PtcResult(C.Structure):
     _fields_ = [("stop", C.c_int32), ("accepted_steps", C.c_int32), ("newton_iterations", C.c_int32),
                 ("pcg_iterations", C.c_int64), ("records", C.c_int32), ("attempts", C.c_int32),
                 ("final_h", C.c_double), ("final_residual", C.c_double), ("gravity_load", C.c_double),
-                ("seconds", C.c_double), ("phase_ms", C.c_double * 6), ("fallbacks", C.c_int64)]
+                ("seconds", C.c_double), ("phase_ms", C.c_double * 7), ("fallbacks", C.c_int64),
+                ("read_bytes", C.c_int64)]
+
+
+PHASES = ["assembly", "pcg", "line_search", "ccd", "reduce", "io", "other"]
 
 
 class Scene(C.Structure):
@@ -131,6 +135,13 @@ def lib():
         L.drape_ptc_solve.argtypes = [vp, f32p, C.c_double, C.c_int32, C.c_double, C.POINTER(PtcResult),
                                       C.POINTER(PtcStep), C.c_int32, C.POINTER(PtcAttempt), C.c_int32,
                                       C.POINTER(Record), C.c_int32, f32p]
+        L.drape_set_colliders.argtypes = [vp, C.c_int32]
+        L.drape_set_row_mass.argtypes = [vp, f32p]
+        L.drape_set_tangent.argtypes = [vp, C.c_int32]
+        L.drape_device_name.argtypes = [C.c_char_p, C.c_int32]
+        L.drape_prolong_state.argtypes = [C.c_int32, f32p, f64p, C.c_int32, f32p, i32p, i32p]
+        L.drape_ptc_keep_states.argtypes = [vp, C.c_int32]
+        L.drape_ptc_states.argtypes = [vp, f32p, C.c_int32, i32p]
         L.drape_eval_elements.argtypes = [vp, f32p, f32p, f32p]
         L.drape_get_system.argtypes = [vp, f32p, f32p]
         L.drape_pcg_timing.argtypes = [vp, f64p, f64p, i32p]
@@ -170,6 +181,39 @@ def copy_timing(nbytes: int, launches: int = 10) -> float:
     ms = C.c_double()
     _check(lib().drape_copy_timing(nbytes, launches, C.byref(ms)))
     return ms.value
+
+
+def device_name() -> str:
+    """Name, architecture and compute units of the current device."""
+    buf = C.create_string_buffer(256)
+    _check(lib().drape_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def scene_colliders(scene: Scene) -> np.ndarray:
+    """The scene's colliders packed as [count, dhat, kappa, 0, (kind, a, b, c, w, 0, 0, 0) per collider] in f64."""
+    k = scene.n_colliders
+    out = np.zeros(4 + 8 * k)
+    out[:4] = (k, scene.dhat, scene.kappa, 0.0)
+    for c in range(k):
+        out[4 + 8 * c:4 + 8 * c + 5] = scene.colliders[8 * c:8 * c + 5]
+    return out
+
+
+def prolong_state(coarse, coarse_columns, colliders=None):
+    """A coarse f32 state prolonged to the (2 coarse_columns - 1)^2 grid and projected out of the colliders
+    (e7_prolong.rs:9-47): (fine positions, vertices moved to d = dhat, vertices left infeasible)."""
+    coarse = _f32(coarse).ravel()
+    if coarse.size != 3 * coarse_columns * coarse_columns:
+        raise DrapeError(f"coarse state has {coarse.size // 3} vertices, not {coarse_columns}^2")
+    nf = 2 * (coarse_columns - 1) + 1
+    fine = np.zeros(3 * nf * nf, np.float32)
+    cols = None if colliders is None else np.ascontiguousarray(colliders, np.float64)
+    moved, bad = C.c_int32(), C.c_int32()
+    _check(lib().drape_prolong_state(coarse_columns, _ptr(coarse, f32p), None if cols is None else _ptr(cols, f64p),
+                                     0 if cols is None else cols.size, _ptr(fine, f32p), C.byref(moved),
+                                     C.byref(bad)))
+    return fine, moved.value, bad.value
 
 
 def default_rules(**overrides) -> Rules:
@@ -302,6 +346,16 @@ class Drape:
         _check(lib().drape_get_positions(self.h, _ptr(out, f32p)))
         return out
 
+    def set_colliders(self, enabled: bool):
+        """withoutColliders (contact-host.ts:64-73): False zeroes the collider count the kernels read."""
+        _check(lib().drape_set_colliders(self.h, 1 if enabled else 0))
+
+    def set_row_mass(self, row_mass):
+        _check(lib().drape_set_row_mass(self.h, _ptr(_f32(row_mass), f32p)))
+
+    def set_tangent(self, exact: bool):
+        _check(lib().drape_set_tangent(self.h, 1 if exact else 0))
+
     def set_barrier_curvature(self, drop: bool):
         _check(lib().drape_set_barrier_curvature(self.h, int(drop)))
 
@@ -432,7 +486,10 @@ class Drape:
         return {"positions": x, "velocities": v, "updates": up[:steps], "relative": rel[:steps],
                 "records": [recs[i] for i in range(min(n.value, keep))]}
 
-    def solve_equilibrium(self, start=None, h_start=None, cap=400, budget_s=0.0, keep=20000):
+    def solve_equilibrium(self, start=None, h_start=None, cap=400, budget_s=0.0, keep=20000, keep_states=False):
+        """solveEquilibrium (drape-e7.ts:156-181). "states" holds the start and every accepted state when
+        keep_states, else the start and the last."""
+        _check(lib().drape_ptc_keep_states(self.h, 1 if keep_states else 0))
         res = PtcResult()
         steps = (PtcStep * keep)()
         attempts = (PtcAttempt * keep)()
@@ -449,7 +506,15 @@ class Drape:
             "attempts": [attempts[i] for i in range(min(res.attempts, keep))],
             "records": [recs[i] for i in range(min(res.records, keep))],
             "positions": final,
+            "states": self.kept_states(),
         }
+
+    def kept_states(self):
+        n = C.c_int32()
+        _check(lib().drape_ptc_states(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 3 * self.V), np.float32)
+        _check(lib().drape_ptc_states(self.h, _ptr(out, f32p), n.value, C.byref(n)))
+        return out
 
 
 class LinearSystem:

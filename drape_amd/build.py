@@ -13,6 +13,7 @@ SOURCES = [
     "kernels/elements.hip",
     "kernels/twolevel.hip",
     "kernels/gnn.hip",
+    "kernels/prolong.hip",
     "host/context.cpp",
     "host/newton.cpp",
     "host/problem.cpp",

@@ -493,6 +493,44 @@ int drape_set_barrier_curvature(drape_ctx* p, int32_t drop) {
   return DRAPE_OK;
 }
 
+int drape_set_colliders(drape_ctx* p, int32_t enabled) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
+  if (c->K == 0) return DRAPE_OK;
+  c->colliders_host[0] = enabled ? (float)c->K : 0.0f;
+  HIP_TRY(hipMemcpyAsync(c->colliders, &c->colliders_host[0], sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DRAPE_OK;
+}
+
+int drape_set_row_mass(drape_ctx* p, const float* row_mass) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear || !row_mass) { set_error("invalid argument"); return DRAPE_E_INVALID; }
+  std::copy(row_mass, row_mass + c->R, c->row_mass_host.begin());
+  HIP_TRY(hipMemcpyAsync(c->rmass, c->row_mass_host.data(), c->R * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  ++c->shift_version;
+  c->shift_h = 0.0;
+  return DRAPE_OK;
+}
+
+int drape_set_tangent(drape_ctx* p, int32_t exact) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
+  c->rules.exact_tangent = exact != 0;
+  return DRAPE_OK;
+}
+
+int drape_device_name(char* name, int32_t cap) {
+  if (!name || cap <= 0) { set_error("invalid argument"); return DRAPE_E_INVALID; }
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  std::snprintf(name, (size_t)cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return DRAPE_OK;
+}
+
 int drape_residual(drape_ctx* p, float* norm, double* norm_squared) {
   Context* c = ctx_of(p);
   if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }

@@ -4,6 +4,7 @@
 // newton-ops.ts:122-139, drape-e4.ts:322-330, drape-e7.ts:56-181, ptc.ts:234-268) with the same f32
 // decisions (Math.fround), but with the state resident on the device: the host only reads back the
 // scalars each decision rests on (norms, slopes, energy sums, step bounds).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -16,7 +17,7 @@ namespace drape {
 namespace {
 
 using Clock = std::chrono::steady_clock;
-enum Phase { ASSEMBLY = 0, PCG = 1, LINE_SEARCH = 2, CCD = 3, REDUCE = 4, OTHER = 5 };
+enum Phase { ASSEMBLY = 0, PCG = 1, LINE_SEARCH = 2, CCD = 3, REDUCE = 4, IO = 5, OTHER = 6 };
 enum StepStatusCode { RUNNING = 0, CONVERGED = 1, ITERATION_LIMIT = 2, LINE_SEARCH_FAILED = 3, UPDATE_RULE = 4,
                       REGULARIZATION_CAP = 5 };
 
@@ -26,7 +27,7 @@ struct Run {
   Context& c;
   drape_rules rules;  // with the Newton tolerance / cap of the caller (PTC inner settings)
   std::vector<drape_record> records;
-  double phase_ms[6] = {0, 0, 0, 0, 0, 0};
+  double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   int64_t pcg_iterations = 0;
   explicit Run(Context& ctx) : c(ctx), rules(ctx.rules) {}
 };
@@ -343,13 +344,32 @@ int attempt(Run& run, double h, float scale, int step, bool ready, Attempt* out)
   return DRAPE_OK;
 }
 
+// Appends the positions at `src` [V float4] to the kept states (Float32Array.from(taken.next),
+// drape-e7.ts:175), timed as readback.
+int keep_state(Run& run, const float4* src) {
+  Timer t(run, IO);
+  Context& c = run.c;
+  std::vector<float4> v(c.V);
+  TRY_RC(read_back(c, v.data(), src, v.size() * sizeof(float4)));
+  const size_t at = c.kept_states.size();
+  c.kept_states.resize(at + 3 * (size_t)c.V);
+  for (int i = 0; i < c.V; ++i) {
+    c.kept_states[at + 3 * i] = v[i].x;
+    c.kept_states[at + 3 * i + 1] = v[i].y;
+    c.kept_states[at + 3 * i + 2] = v[i].z;
+  }
+  return DRAPE_OK;
+}
+
 }  // namespace
 
 int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_result* result,
               std::vector<drape_ptc_step>& steps, std::vector<drape_ptc_attempt>& attempts,
               std::vector<drape_record>& records_out) {
   Run run(c);
-  const int64_t fallbacks0 = c.fallbacks;
+  const int64_t fallbacks0 = c.fallbacks, read0 = c.read_bytes;
+  c.kept_states.clear();
+  TRY_RC(keep_state(run, c.org));
   run.rules.newton_tolerance = c.rules.inner_tolerance;
   run.rules.newton_max_iterations = c.rules.inner_max_iterations;
   const drape_rules& r = c.rules;
@@ -420,6 +440,7 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
         steps.push_back({(double)step_index, hh, (double)scale, (double)a.updates, (double)rejected, (double)pcg,
                          min_distance, dropped ? 1.0 : 0.0});
         HIP_TRY(hipMemcpyAsync(c.org, c.pos, (size_t)c.V * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+        if (c.keep_states) TRY_RC(keep_state(run, c.org));
         // ptc.ts:241-244 nextStepSize
         const double factor = a.updates < r.inner_max_iterations ? r.growth : r.shrink;
         h = std::fmin(r.h_max, std::fmax(r.h0 * r.h_min_factor, hh * factor));
@@ -433,6 +454,7 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
     }
   }
   HIP_TRY(hipMemcpyAsync(c.pos, c.org, (size_t)c.V * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+  if (!c.keep_states) TRY_RC(keep_state(run, c.org));  // drape-e7.ts:178: the first and the last state
   HIP_TRY(hipStreamSynchronize(c.stream));
   const double seconds = std::chrono::duration<double>(Clock::now() - started).count();
   result->stop = stop;
@@ -449,12 +471,13 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
   result->gravity_load = r.gravity_load;
   result->seconds = seconds;
   double tracked = 0.0;
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < OTHER; ++i) {
     result->phase_ms[i] = run.phase_ms[i];
     tracked += run.phase_ms[i];
   }
-  result->phase_ms[5] = 1000.0 * seconds - tracked;
+  result->phase_ms[OTHER] = 1000.0 * seconds - tracked;
   result->fallbacks = c.fallbacks - fallbacks0;
+  result->read_bytes = c.read_bytes - read0;
   records_out = std::move(run.records);
   return DRAPE_OK;
 }
@@ -614,6 +637,31 @@ int drape_ptc_solve(drape_ctx* p, const float* start, double h_start, int32_t ca
       final_positions[3 * i] = v[i].x; final_positions[3 * i + 1] = v[i].y; final_positions[3 * i + 2] = v[i].z;
     }
   }
+  return DRAPE_OK;
+}
+
+int drape_ptc_keep_states(drape_ctx* p, int32_t keep) {
+  Context* c = reinterpret_cast<Context*>(p);
+  if (!c || c->linear) {
+    set_error("invalid context");
+    return DRAPE_E_INVALID;
+  }
+  c->keep_states = keep != 0;
+  return DRAPE_OK;
+}
+
+int drape_ptc_states(drape_ctx* p, float* states, int32_t cap, int32_t* count) {
+  Context* c = reinterpret_cast<Context*>(p);
+  if (!c || c->linear || cap < 0) {
+    set_error("invalid context or cap");
+    return DRAPE_E_INVALID;
+  }
+  const size_t per = 3 * (size_t)c->V;
+  const int32_t n = (int32_t)(c->kept_states.size() / per);
+  if (count) *count = n;
+  if (states)
+    for (int32_t i = 0; i < n && i < cap; ++i)
+      std::copy(c->kept_states.begin() + i * per, c->kept_states.begin() + (i + 1) * per, states + i * per);
   return DRAPE_OK;
 }
 

@@ -108,6 +108,11 @@ struct Context {
   float shift_mu = 0.0f;
   double shift_h = 0.0;
   int64_t fallbacks = 0;          // two-level solves redone with block-Jacobi
+  int64_t read_bytes = 0;         // device-to-host bytes read back by the solve loops
+
+  // accepted PTC states kept for the caller: every state when keep_states, else the start and the last
+  bool keep_states = false;
+  std::vector<float> kept_states;  // [count][3V]
 
   // timing of the last PCG solve
   double last_spmv_ms = 0, last_solve_ms = 0;
@@ -131,6 +136,15 @@ void set_error(const std::string& message);
     int _rc = (expr);          \
     if (_rc != DRAPE_OK) return _rc; \
   } while (0)
+
+// Synchronous device-to-host read on the context's stream, counted in read_bytes (the readbacks the
+// reference's timed backend counts, timed-backend.ts).
+inline int read_back(Context& c, void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  c.read_bytes += (int64_t)bytes;
+  return DRAPE_OK;
+}
 
 // ---- kernel launchers (elements.hip) --------------------------------------------------------
 int launch_evaluate(Context& c, bool exact);        // element + row kernels into lgrad/lhess

@@ -702,8 +702,7 @@ static int row_sum(Context& c, const float4* a, const float4* w, double* out) {
   k_row_sums<<<c.G, 256, 0, s>>>(a, w, c.R, c.dpart);
   k_final_sums<<<1, 256, 0, s>>>(c.dpart, c.G, 1, c.dout);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(c.dout_host, c.dout, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  TRY_RC(read_back(c, c.dout_host, c.dout, 2 * sizeof(double)));
   *out = c.dout_host[0];
   return DRAPE_OK;
 }
@@ -737,8 +736,7 @@ int launch_energy_change(Context& c, float alpha, double* sum, double* abs_sum) 
                                                c.R, gt, gh, c.dpart);
   k_final_sums<<<1, 256, 0, s>>>(c.dpart, gt + gh + gr, 2, c.dout);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(c.dout_host, c.dout, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  TRY_RC(read_back(c, c.dout_host, c.dout, 2 * sizeof(double)));
   *sum = c.dout_host[0];
   *abs_sum = c.dout_host[1];
   return DRAPE_OK;
@@ -753,8 +751,7 @@ int launch_max_step(Context& c, float* alpha, int32_t* row, float* min_distance,
   k_ccd<<<c.G, 256, 0, s>>>(c.pos, c.dir, c.rvert, c.colliders, c.R, c.fpart, c.ipart);
   k_ccd_final<<<1, 256, 0, s>>>(c.fpart, c.ipart, c.G, c.dout);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(c.dout_host, c.dout, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  TRY_RC(read_back(c, c.dout_host, c.dout, 4 * sizeof(double)));
   *alpha = (float)c.dout_host[0];
   *row = (int32_t)c.dout_host[1];
   *min_distance = (float)c.dout_host[2];
@@ -767,8 +764,7 @@ int launch_row_maxes(Context& c, const float4* org, float out[4]) {
   k_row_maxes<<<c.G, 256, 0, s>>>(c.x, c.pos, org, c.rvert, c.R, c.fpart);
   k_row_maxes_final<<<1, 256, 0, s>>>(c.fpart, c.G, c.dout);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(c.dout_host, c.dout, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  TRY_RC(read_back(c, c.dout_host, c.dout, 8 * sizeof(double)));
   for (int i = 0; i < 4; ++i) out[i] = (float)c.dout_host[4 + i];
   return DRAPE_OK;
 }

@@ -474,8 +474,7 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
       k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 1, it - 1);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(c.state_host, c.state, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    TRY_RC(read_back(c, c.state_host, c.state, sizeof(PcgState)));
     if (c.state_host->status != 0) {
       stopped = true;
       break;
@@ -488,8 +487,7 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
   k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 0);
   k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 0, 0);
   HIP_TRY(hipEventRecord(c.ev1, s));
-  HIP_TRY(hipMemcpyAsync(c.state_host, c.state, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  TRY_RC(read_back(c, c.state_host, c.state, sizeof(PcgState)));
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, c.ev0, c.ev1));
   c.last_solve_ms = ms;
@@ -631,8 +629,7 @@ static int pcg_refine(Context& c, double tolerance, int max_iterations, drape_pc
     k_residual64<<<G, 256, 0, s>>>(A, x64, c.b_saved, r64, c.dpart);
     k_sum64<<<1, 256, 0, s>>>(c.dpart, G, c.dout);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(c.dout_host, c.dout, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    TRY_RC(read_back(c, c.dout_host, c.dout, sizeof(double)));
     const double rr = c.dout_host[0];
     if (pass == 0) bb = rr;
     rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;

@@ -934,8 +934,7 @@ int two_level_prepare(Context& c, bool* ok) {
   k_band_inverse<<<t->nt * (TILE / INV_COLS), 256, 0, s>>>(t->d_lpanel, t->d_linv, t->nt, t->kb, t->N3, t->N3p, t->d_B,
                                                            t->d_BT, t->d_info);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(t->h_info, t->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  TRY_RC(read_back(c, t->h_info, t->d_info, sizeof(int)));
   t->factorisations++;
   t->ok = *t->h_info == 0;
   if (!t->ok) t->failures++;

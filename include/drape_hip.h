@@ -132,8 +132,9 @@ typedef struct {
   int32_t records, attempts;
   double final_h, final_residual, gravity_load;
   double seconds;          /* wall clock of the solve loop */
-  double phase_ms[6];      /* assembly, pcg, line_search, ccd, reduce, other (timed-backend.ts:3-53) */
+  double phase_ms[7];      /* assembly, pcg, line_search, ccd, reduce, io, other (timed-backend.ts:3-53) */
   int64_t fallbacks;       /* two-level solves that fell back to block-Jacobi */
+  int64_t read_bytes;      /* device-to-host bytes read back by the loop (EquilibriumRun.readBytes) */
 } drape_ptc_result;
 
 /* A sheet scene: the reference's Scene / SceneParams and the material and step constants its Rust
@@ -210,6 +211,30 @@ int drape_simulate(drape_ctx* ctx, int32_t steps, const float* x0, const float* 
 int drape_ptc_solve(drape_ctx* ctx, const float* start, double h_start, int32_t cap, double budget_s,
                     drape_ptc_result* result, drape_ptc_step* steps, int32_t steps_cap, drape_ptc_attempt* attempts,
                     int32_t attempts_cap, drape_record* records, int32_t records_cap, float* final_positions);
+/* States kept by later drape_ptc_solve calls (solveEquilibrium's keepStates, drape-e7.ts:159-178):
+ * keep != 0 keeps the start and every accepted state, keep == 0 the start and the last only. */
+int drape_ptc_keep_states(drape_ctx* ctx, int32_t keep);
+/* The states kept by the last drape_ptc_solve, `count` of them, [count][3V] into `states` (up to `cap`
+ * states; NULL to query the count) -- the contents of gpu-states.bin (run-e7.ts:98). */
+int drape_ptc_states(drape_ctx* ctx, float* states, int32_t cap, int32_t* count);
+
+/* Prolongation of a coarse state to the next grid as a PTC start (e7_prolong.rs:9-47, the oracle's
+ * e7-prolong): `coarse` [3 coarse_columns^2] f32 -> `fine` [3 (2 coarse_columns - 1)^2] f32, computed in f64.
+ * `colliders` [collider_doubles = 4 + 8K] = [K, dhat, kappa, flag, (kind, a, b, c, w, 0, 0, 0) x K] in f64
+ * (NULL: no projection); `moved` counts vertices projected to d = dhat, `infeasible` those left at or inside a
+ * collider (the caller refuses such a start, e7_prolong.rs:60-62). Context-free; coarse_columns <= 8193. */
+int drape_prolong_state(int32_t coarse_columns, const float* coarse, const double* colliders,
+                        int32_t collider_doubles, float* fine, int32_t* moved, int32_t* infeasible);
+
+/* Kernel-check switches (drape-e4.ts:108-120, drape-e7.ts:203-217, contact-host.ts:64-90):
+ * drape_set_colliders(0) zeroes the collider count the kernels read (withoutColliders), 1 restores it;
+ * drape_set_row_mass writes the row masses M/h^2 [R] directly (gpu.rowMass); drape_set_tangent selects the
+ * exact (1) or projected (0) element tangent of later evaluations (exactElements / projectedElements). */
+int drape_set_colliders(drape_ctx* ctx, int32_t enabled);
+int drape_set_row_mass(drape_ctx* ctx, const float* row_mass);
+int drape_set_tangent(drape_ctx* ctx, int32_t exact);
+/* Name, architecture and compute units of the current device (the adapter of run-e7.ts:101). */
+int drape_device_name(char* name, int32_t cap);
 
 /* Kernel check (drape-step.ts evaluateDrapeKernels / drape-e4.ts:376-388): element and row terms at
  * `positions` with the current x_hat, mu and masses: grad [9T + 12H + 3R], hess [81T + 144H + 9R]
