@@ -460,16 +460,32 @@ def run_e7(case_dir, mode="cold", start=None, checks=False, preconditioner="bloc
 
 
 def main(argv=None):
-    ap = argparse.ArgumentParser(description="Quasi-static drape by PTC on a case directory (run-e7.ts)")
-    ap.add_argument("--case", required=True)
+    """`--case DIR [--mode M] [--start PREFIX] [--checks]` runs run-e7.ts; `--coarse PREFIX --coarse-n N
+    --fine-dir DIR --out PREFIX` writes a prolonged start as the oracle's e7-prolong does (main.rs:75-78)."""
+    ap = argparse.ArgumentParser(description="PTC drape on a case directory (run-e7.ts) or a prolonged start "
+                                             "(e7-prolong)")
+    ap.add_argument("--case")
     ap.add_argument("--mode", default="cold", choices=MODES)
     ap.add_argument("--start")
     ap.add_argument("--checks", action="store_true")
     ap.add_argument("--preconditioner", default="block_jacobi", choices=["block_jacobi", "two_level"])
     ap.add_argument("--budget-s", type=float, default=float(os.environ.get("E7_BUDGET_S", "0") or 0))
+    ap.add_argument("--coarse")
+    ap.add_argument("--coarse-n", type=int, default=33)
+    ap.add_argument("--fine-dir")
+    ap.add_argument("--out")
     a = ap.parse_args(argv)
     try:
-        run_e7(a.case, a.mode, a.start, a.checks, a.preconditioner, a.budget_s)
+        if a.coarse is not None:
+            if a.fine_dir is None or a.out is None:
+                raise DrapeError("usage: --coarse <state prefix> --coarse-n <columns> --fine-dir <case dir> "
+                                 "--out <state prefix>")
+            prolong_start(a.coarse, a.coarse_n, a.fine_dir, a.out)
+        elif a.case is not None:
+            run_e7(a.case, a.mode, a.start, a.checks, a.preconditioner, a.budget_s)
+        else:
+            raise DrapeError("usage: --case <case dir> [--mode cold|warm|prolong] [--start <state prefix>] "
+                             "[--checks]  |  --coarse <state prefix> --coarse-n N --fine-dir <dir> --out <prefix>")
     except DrapeError as e:
         print(f"error: {e}", file=sys.stderr)
         return 1

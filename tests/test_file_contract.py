@@ -211,3 +211,10 @@ def test_scorer_replays_accept_a_consistent_run_and_reject_planted_mistakes(tmp_
     groups = S.attempt_groups(its, attempts)
     groups[2][0]["updateInf"] = 1e-9
     assert not S.termination_replay(groups, True)
+
+
+def test_command_line_refuses_incomplete_arguments(tmp_path, capsys):
+    assert FC.main([]) == 1
+    assert FC.main(["--coarse", "x"]) == 1
+    assert FC.main(["--case", str(tmp_path), "--mode", "warm"]) == 1    # warm needs a start
+    assert "error:" in capsys.readouterr().err
