@@ -42,6 +42,7 @@ struct TwoLevel {
   int npairs = 0;    // coarse block pairs
   uint64_t factored = ~0ull;  // system version of the current factorisation (~0: none)
   uint64_t factored_shift = 0;  // mu / step-size version of the current factorisation
+  double factored_h = 0.0;      // step size of M/h^2 at the current factorisation
   bool ok = false;            // the last factorisation succeeded
   bool refresh = false;       // PCG slowed down: refactor at the next new system
   double base_rate = 0.0;     // iterations per decade of the first solve after the factorisation
@@ -900,13 +901,15 @@ int two_level_prepare(Context& c, bool* ok) {
   hipStream_t s = c.stream;
   const bool fresh = t->factored == c.system_version;
   const bool reusable = c.rules.coarse_reuse && t->factored != ~0ull && t->ok && !t->refresh &&
-                        (c.rules.coarse_reuse == 2 || t->factored_shift == c.shift_version);
+                        (c.rules.coarse_reuse == 2 || t->factored_shift == c.shift_version ||
+                         (c.rules.coarse_reuse == 3 && t->factored_h == c.shift_h));
   if (fresh || reusable) {
     *ok = t->ok;
     return DRAPE_OK;
   }
   t->factored = c.system_version;
   t->factored_shift = c.shift_version;
+  t->factored_h = c.shift_h;
   t->refresh = false;
   t->solves = 0;
   t->ok = false;
