@@ -156,6 +156,9 @@ def lib():
                                      i32p]
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
+        L.drape_scene_build_device.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
+        L.drape_scene_contributions.argtypes = [vp, C.POINTER(C.c_int64), u32p, u32p, u32p, u32p, u32p]
+        L.drape_get_contributions.argtypes = [vp, C.POINTER(C.c_int64), u32p, u32p, u32p, u32p]
         L.drape_scene_problem.restype = C.POINTER(Problem)
         L.drape_scene_problem.argtypes = [vp, f64p]
         L.drape_scene_free.argtypes = [vp]
@@ -286,9 +289,11 @@ def drape_config_scene(columns, *, radius=0.15, clearance=None, dhat=1e-3, time_
 class SceneArrays:
     """Problem arrays built by the library from a scene (problem.bin's arrays)."""
 
-    def __init__(self, scene: Scene):
+    def __init__(self, scene: Scene, device: bool = False):
+        """device: build hinges, pattern, geometry, masses and the contribution lists on the GPU."""
         h = C.c_void_p()
-        _check(lib().drape_scene_build(C.byref(scene), C.byref(h)))
+        build = lib().drape_scene_build_device if device else lib().drape_scene_build
+        _check(build(C.byref(scene), C.byref(h)))
         self.h = h
         load = C.c_double()
         self.problem_ptr = lib().drape_scene_problem(self.h, C.byref(load))
@@ -307,6 +312,17 @@ class SceneArrays:
         if n == 0 or not ptr:
             return np.zeros(0, np.float32)
         return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+    def contributions(self):
+        """diag_idx, block_contrib_ptr, block_contrib_idx, force_contrib_ptr, force_contrib_idx of a
+        device-built scene (problem.bin's arrays)."""
+        sizes = (C.c_int64 * 2)()
+        _check(lib().drape_scene_contributions(self.h, sizes, None, None, None, None, None))
+        out = [np.zeros(self.R, np.uint32), np.zeros(self.B + 1, np.uint32), np.zeros(sizes[0], np.uint32),
+               np.zeros(self.R + 1, np.uint32), np.zeros(sizes[1], np.uint32)]
+        _check(lib().drape_scene_contributions(self.h, sizes, *[_ptr(a, u32p) for a in out]))
+        return dict(zip(["diag_idx", "block_contrib_ptr", "block_contrib_idx", "force_contrib_ptr",
+                         "force_contrib_idx"], out))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -345,6 +361,15 @@ class Drape:
         out = np.zeros(3 * self.V, np.float32)
         _check(lib().drape_get_positions(self.h, _ptr(out, f32p)))
         return out
+
+    def contributions(self):
+        """The context's device-built scatter maps: block_contrib_ptr/idx and force_contrib_ptr/idx."""
+        sizes = (C.c_int64 * 2)()
+        _check(lib().drape_get_contributions(self.h, sizes, None, None, None, None))
+        out = [np.zeros(self.B + 1, np.uint32), np.zeros(sizes[0], np.uint32), np.zeros(self.R + 1, np.uint32),
+               np.zeros(sizes[1], np.uint32)]
+        _check(lib().drape_get_contributions(self.h, sizes, *[_ptr(a, u32p) for a in out]))
+        return dict(zip(["block_contrib_ptr", "block_contrib_idx", "force_contrib_ptr", "force_contrib_idx"], out))
 
     def set_colliders(self, enabled: bool):
         """withoutColliders (contact-host.ts:64-73): False zeroes the collider count the kernels read."""

@@ -14,6 +14,7 @@ SOURCES = [
     "kernels/twolevel.hip",
     "kernels/gnn.hip",
     "kernels/prolong.hip",
+    "kernels/builder.hip",
     "host/context.cpp",
     "host/newton.cpp",
     "host/problem.cpp",

@@ -173,13 +173,6 @@ static int alloc_solver(Context& c, const std::vector<uint32_t>& row_ptr, const 
   return DRAPE_OK;
 }
 
-static int position(const std::vector<uint32_t>& row_ptr, const std::vector<uint32_t>& col_idx, int row, int col) {
-  auto b = col_idx.begin() + row_ptr[row], e = col_idx.begin() + row_ptr[row + 1];
-  auto it = std::lower_bound(b, e, (uint32_t)col);
-  if (it == e || *it != (uint32_t)col) return -1;
-  return (int)(it - col_idx.begin());
-}
-
 static int create_mesh_context(const drape_problem* pr, const drape_rules* rules, Context& c) {
   c.V = pr->vertices; c.T = pr->triangles; c.H = pr->hinges; c.R = pr->rows; c.B = pr->blocks;
   if (c.V <= 0 || c.R <= 0 || c.B < c.R || c.T < 0 || c.H < 0) {
@@ -241,59 +234,7 @@ static int create_mesh_context(const drape_problem* pr, const drape_rules* rules
   for (int v = 0; v < c.V; ++v)
     pos[v] = make_float4(pr->rest_positions[3 * v], pr->rest_positions[3 * v + 1], pr->rest_positions[3 * v + 2], 0.0f);
   c.rest_host.assign(pr->rest_positions, pr->rest_positions + 3 * (size_t)c.V);
-  // scatter maps sorted by row: per CSR block the local 3x3 blocks (units of 9 floats) in the order
-  // row term, membranes (9e + 3k + l), hinges (9T + 16h + 4k + l); per row the local gradients
-  // (units of 3 floats): row term, membranes (3e + k), hinges (3T + 4h + k).
   const int T = c.T, H = c.H, R = c.R;
-  std::vector<std::vector<uint32_t>> blocks(c.B), forces(R);
-  for (int row = 0; row < R; ++row) {
-    int d = position(row_ptr, col_idx, row, row);
-    if (d < 0) { set_error("missing diagonal block"); return DRAPE_E_INVALID; }
-    blocks[d].push_back((uint32_t)(9 * T + 16 * H + row));
-    forces[row].push_back((uint32_t)(3 * T + 4 * H + row));
-  }
-  for (int e = 0; e < T; ++e) {
-    const int4 v = tri[e];
-    const int vs[3] = {v.x, v.y, v.z};
-    for (int k = 0; k < 3; ++k) {
-      int r = row_of[vs[k]];
-      if (r < 0) continue;
-      forces[r].push_back((uint32_t)(3 * e + k));
-      for (int l = 0; l < 3; ++l) {
-        int col = row_of[vs[l]];
-        if (col < 0) continue;
-        int b = position(row_ptr, col_idx, r, col);
-        if (b < 0) { set_error("a triangle couples rows outside the pattern"); return DRAPE_E_INVALID; }
-        blocks[b].push_back((uint32_t)(9 * e + 3 * k + l));
-      }
-    }
-  }
-  for (int h = 0; h < H; ++h) {
-    const int4 v = hin[h];
-    const int vs[4] = {v.x, v.y, v.z, v.w};
-    for (int k = 0; k < 4; ++k) {
-      int r = row_of[vs[k]];
-      if (r < 0) continue;
-      forces[r].push_back((uint32_t)(3 * T + 4 * h + k));
-      for (int l = 0; l < 4; ++l) {
-        int col = row_of[vs[l]];
-        if (col < 0) continue;
-        int b = position(row_ptr, col_idx, r, col);
-        if (b < 0) { set_error("a hinge couples rows outside the pattern"); return DRAPE_E_INVALID; }
-        blocks[b].push_back((uint32_t)(9 * T + 16 * h + 4 * k + l));
-      }
-    }
-  }
-  auto flatten = [](const std::vector<std::vector<uint32_t>>& lists, std::vector<uint32_t>& ptr, std::vector<uint32_t>& idx) {
-    ptr.assign(1, 0);
-    for (const auto& l : lists) {
-      idx.insert(idx.end(), l.begin(), l.end());
-      ptr.push_back((uint32_t)idx.size());
-    }
-  };
-  std::vector<uint32_t> bcp, bci, fcp, fci;
-  flatten(blocks, bcp, bci);
-  flatten(forces, fcp, fci);
 
   HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   hipStream_t s = c.stream;
@@ -305,8 +246,6 @@ static int create_mesh_context(const drape_problem* pr, const drape_rules* rules
   TRY(alloc(&c.colliders, 4 + 8 * (size_t)c.K)); TRY(alloc(&c.mu, 1)); TRY(alloc(&c.material, 4));
   TRY(alloc(&c.lgrad, 9 * (size_t)T + 12 * (size_t)H + 3 * (size_t)R));
   TRY(alloc(&c.lhess, 81 * (size_t)T + 144 * (size_t)H + 9 * (size_t)R));
-  TRY(alloc(&c.bc_ptr, bcp.size())); TRY(alloc(&c.bc_idx, bci.size()));
-  TRY(alloc(&c.fc_ptr, fcp.size())); TRY(alloc(&c.fc_idx, fci.size()));
   const int maxg = blocks_for(T) + blocks_for(H) + blocks_for(R) + 4;
   TRY(alloc(&c.dpart, 2 * (size_t)maxg));
   TRY(upload(c.pos, pos.data(), pos.size(), s));
@@ -327,10 +266,10 @@ static int create_mesh_context(const drape_problem* pr, const drape_rules* rules
   const float mat[4] = {(float)pr->wale, (float)pr->course, (float)pr->coupling, (float)pr->shear};
   TRY(upload(c.material, mat, 4, s));
   HIP_TRY(hipStreamSynchronize(s));
-  TRY(upload(c.bc_ptr, bcp.data(), bcp.size(), s));
-  TRY(upload(c.bc_idx, bci.data(), bci.size(), s));
-  TRY(upload(c.fc_ptr, fcp.data(), fcp.size(), s));
-  TRY(upload(c.fc_idx, fci.data(), fci.size(), s));
+  // scatter maps sorted by row: per CSR block the local 3x3 blocks (units of 9 floats) in the order
+  // row term, membranes (9e + 3k + l), hinges (9T + 16h + 4k + l); per row the local gradients
+  // (units of 3 floats): row term, membranes (3e + k), hinges (3T + 4h + k) -- built on the device
+  TRY(build_maps_device(c, row_ptr, col_idx, row_of));
   // M/h^2 of the problem's time step
   c.row_mass_host.resize(R);
   const double h = pr->time_step > 0 ? pr->time_step : 1.0 / 240.0;
@@ -490,6 +429,25 @@ int drape_set_barrier_curvature(drape_ctx* p, int32_t drop) {
   c->curvature_dropped = drop != 0;
   HIP_TRY(hipMemcpyAsync(c->colliders + 3, &c->colliders_host[3], sizeof(float), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return DRAPE_OK;
+}
+
+int drape_get_contributions(drape_ctx* p, int64_t* sizes, uint32_t* block_contrib_ptr, uint32_t* block_contrib_idx,
+                            uint32_t* force_contrib_ptr, uint32_t* force_contrib_idx) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear || !sizes) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
+  sizes[0] = c->block_contribs;
+  sizes[1] = c->force_contribs;
+  hipStream_t s = c->stream;
+  if (block_contrib_ptr)
+    HIP_TRY(hipMemcpyAsync(block_contrib_ptr, c->bc_ptr, ((size_t)c->B + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (block_contrib_idx)
+    HIP_TRY(hipMemcpyAsync(block_contrib_idx, c->bc_idx, (size_t)c->block_contribs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (force_contrib_ptr)
+    HIP_TRY(hipMemcpyAsync(force_contrib_ptr, c->fc_ptr, ((size_t)c->R + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (force_contrib_idx)
+    HIP_TRY(hipMemcpyAsync(force_contrib_idx, c->fc_idx, (size_t)c->force_contribs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return DRAPE_OK;
 }
 

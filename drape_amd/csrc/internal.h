@@ -109,6 +109,8 @@ struct Context {
   double shift_h = 0.0;
   int64_t fallbacks = 0;          // two-level solves redone with block-Jacobi
   int64_t read_bytes = 0;         // device-to-host bytes read back by the solve loops
+  int64_t block_contribs = 0;     // entries of bc_idx and fc_idx
+  int64_t force_contribs = 0;
 
   // accepted PTC states kept for the caller: every state when keep_states, else the start and the last
   bool keep_states = false;
@@ -145,6 +147,24 @@ inline int read_back(Context& c, void* dst, const void* src, size_t bytes) {
   c.read_bytes += (int64_t)bytes;
   return DRAPE_OK;
 }
+
+// ---- device problem builder (builder.hip) -----------------------------------------------------
+// A sheet's topology built on the device from its rest positions (f64 x y per vertex), triangles and
+// free-vertex mask: sorted oriented hinges and their stiffness, element geometry, f64 lumped masses of
+// every vertex (density times area), the rows, the BSR pattern with diagonal positions, and the
+// per-block and per-row contribution lists in the reference's units and order.
+struct MeshTopology {
+  int R = 0, H = 0, B = 0;
+  std::vector<uint32_t> hinge_vertices, row_vertex, row_ptr, col_idx, diag_idx;
+  std::vector<uint32_t> block_contrib_ptr, block_contrib_idx, force_contrib_ptr, force_contrib_idx;
+  std::vector<float> hinge_stiffness, elem_geom;
+  std::vector<double> mass;
+};
+int build_topology_device(int V, const double* rest_xy, int T, const uint32_t* tris, const uint8_t* free_mask,
+                          double density, double bend_wale, double bend_course, MeshTopology& out);
+// The context's scatter maps (bc_ptr/bc_idx, fc_ptr/fc_idx) for its pattern, built on the device.
+int build_maps_device(Context& c, const std::vector<uint32_t>& row_ptr, const std::vector<uint32_t>& col_idx,
+                      const std::vector<int>& row_of);
 
 // ---- kernel launchers (elements.hip) --------------------------------------------------------
 int launch_evaluate(Context& c, bool exact);        // element + row kernels into lgrad/lhess

@@ -161,6 +161,17 @@ typedef struct drape_scene_arrays drape_scene_arrays;
  * e3_generate.rs:84-104). The returned problem points into `arrays`; gravity_load = |m g| over rows
  * (e7_ptc.rs:38-40). */
 int drape_scene_build(const drape_scene* scene, drape_scene_arrays** arrays);
+/* The same arrays with the hinges, pattern, element geometry, masses and the contribution lists built on the
+ * device by stable radix sorts (the reference builds them in its Rust generator: mesh.rs:83-107,
+ * bsr.rs:12-29, e3_generate.rs:84-161); entry for entry equal to drape_scene_build. */
+int drape_scene_build_device(const drape_scene* scene, drape_scene_arrays** arrays);
+/* problem.bin's diag_idx [R] and contribution lists of a device-built scene (e3_generate.rs:111-161):
+ * block_contrib_ptr [B + 1] / block_contrib_idx [sizes[0]] (local 3x3 blocks in units of 9 floats),
+ * force_contrib_ptr [R + 1] / force_contrib_idx [sizes[1]] (local gradients in units of 3 floats). Any
+ * output but `sizes` may be NULL (query the sizes first). */
+int drape_scene_contributions(const drape_scene_arrays* arrays, int64_t* sizes, uint32_t* diag_idx,
+                              uint32_t* block_contrib_ptr, uint32_t* block_contrib_idx, uint32_t* force_contrib_ptr,
+                              uint32_t* force_contrib_idx);
 const drape_problem* drape_scene_problem(const drape_scene_arrays* arrays, double* gravity_load);
 void drape_scene_free(drape_scene_arrays* arrays);
 
@@ -226,6 +237,12 @@ int drape_ptc_states(drape_ctx* ctx, float* states, int32_t cap, int32_t* count)
  * collider (the caller refuses such a start, e7_prolong.rs:60-62). Context-free; coarse_columns <= 8193. */
 int drape_prolong_state(int32_t coarse_columns, const float* coarse, const double* colliders,
                         int32_t collider_doubles, float* fine, int32_t* moved, int32_t* infeasible);
+
+/* The context's assembly scatter maps, built on the device at drape_create: problem.bin's
+ * block_contrib_ptr [B + 1] / block_contrib_idx [sizes[0]] and force_contrib_ptr [R + 1] /
+ * force_contrib_idx [sizes[1]] (e3_generate.rs:111-161). Any output but `sizes` may be NULL. */
+int drape_get_contributions(drape_ctx* ctx, int64_t* sizes, uint32_t* block_contrib_ptr, uint32_t* block_contrib_idx,
+                            uint32_t* force_contrib_ptr, uint32_t* force_contrib_idx);
 
 /* Kernel-check switches (drape-e4.ts:108-120, drape-e7.ts:203-217, contact-host.ts:64-90):
  * drape_set_colliders(0) zeroes the collider count the kernels read (withoutColliders), 1 restores it;
