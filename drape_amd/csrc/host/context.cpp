@@ -44,9 +44,10 @@ static void free_context(Context* c) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->state_host) (void)hipHostFree(c->state_host);
+  if (c->poll_host) (void)hipHostFree(c->poll_host);
   if (c->dout_host) (void)hipHostFree(c->dout_host);
   if (c->two_level) two_level_destroy(c->two_level);
-  for (hipEvent_t e : {c->ev0, c->ev1, c->ev_spmv0, c->ev_spmv1})
+  for (hipEvent_t e : {c->ev0, c->ev1, c->ev_spmv0, c->ev_spmv1, c->ev_poll[0], c->ev_poll[1]})
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -156,6 +157,7 @@ static int alloc_solver(Context& c, const std::vector<uint32_t>& row_ptr, const 
   TRY(alloc(&c.partials, 8 * (size_t)c.G + 16));
   TRY(alloc(&c.state, 1));
   HIP_TRY(hipHostMalloc((void**)&c.state_host, sizeof(PcgState)));
+  HIP_TRY(hipHostMalloc((void**)&c.poll_host, 2 * sizeof(int32_t)));
   HIP_TRY(hipHostMalloc((void**)&c.dout_host, 16 * sizeof(double)));
   TRY(alloc(&c.dout, 16));
   TRY(alloc(&c.fpart, 4 * (size_t)c.G + 16));
@@ -168,6 +170,8 @@ static int alloc_solver(Context& c, const std::vector<uint32_t>& row_ptr, const 
   HIP_TRY(hipEventCreate(&c.ev1));
   HIP_TRY(hipEventCreate(&c.ev_spmv0));
   HIP_TRY(hipEventCreate(&c.ev_spmv1));
+  HIP_TRY(hipEventCreateWithFlags(&c.ev_poll[0], hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&c.ev_poll[1], hipEventDisableTiming));
   // the uploads above read host vectors that go out of scope on return
   HIP_TRY(hipStreamSynchronize(s));
   return DRAPE_OK;
@@ -308,7 +312,7 @@ void drape_default_rules(drape_rules* r) {
   r->exact_tangent = 1;
   r->refine_fp64 = 0;
   r->preconditioner = DRAPE_PRECOND_BLOCK_JACOBI;
-  r->coarse_reuse = 1;
+  r->coarse_reuse = 3;
   r->fallback = 2;
   r->coarse_shift = 0.0;
 }

@@ -88,6 +88,8 @@ struct Context {
   float* partials = nullptr;      // [2 * 3 * G]: pq | (rz, rr) | replace/true
   PcgState* state = nullptr;
   PcgState* state_host = nullptr; // pinned
+  int32_t* poll_host = nullptr;   // pinned [2]: PCG status copies polled while the next iterations run
+  hipEvent_t ev_poll[2] = {nullptr, nullptr};
   double* refine = nullptr;       // FP64 refinement vectors [3R] x2
   float4* b_saved = nullptr;      // the right-hand side while the refinement loop solves for corrections
   float4* vel = nullptr;          // [V] velocities of the dynamic loop (allocated on first use)
@@ -195,6 +197,9 @@ int two_level_create(Context& c, int columns);           // coarse layout of a g
 int two_level_set_weights(Context& c, const float* w);  // [4R] parent-slot weights (NULL: bilinear)
 int two_level_prepare(Context& c, bool* ok);            // Galerkin + factorisation when the system changed
 int two_level_apply(Context& c, float* partials, int guard);  // z = M^-1 r and partial (r.z, r.r)
+// One two-level PCG iteration after its SpMV: alpha from the p.q partials, x += alpha p, r -= alpha q and
+// z = M^-1 r with partial (r.z, r.r), in the restriction, the coarse solve and the prolongation launches.
+int two_level_iterate(Context& c, const float* partial_pq, const float4* p, float* partial_rz, int it);
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host

@@ -82,48 +82,6 @@ __device__ inline float3 precondition(const float* __restrict__ dinv, int row, f
   return make_float3(d[0] * a + d[1] * b + d[2] * c, d[3] * a + d[4] * b + d[5] * c, d[6] * a + d[7] * b + d[8] * c);
 }
 
-// The reference's final reduction pass (reduce.wgsl:33-53, pcg.rs:447-457) inside one wavefront: lane l
-// stands for the four tree lanes l, l+64, l+128, l+192 (each the strided sum of its partials), combines
-// them as the 128- and 64-strides of the tree do, and finishes with shuffles. No LDS, no barrier: every
-// wavefront obtains the identical result.
-__device__ inline float wave_reduce_partials(const float* __restrict__ part, int count) {
-  const int l = threadIdx.x & 63;
-  float v[4];
-#pragma unroll
-  for (int qd = 0; qd < 4; ++qd) {
-    float a = 0.0f;
-    for (int j = l + 64 * qd; j < count; j += 256) a = a + part[j];
-    v[qd] = a;
-  }
-  float s = (v[0] + v[2]) + (v[1] + v[3]);
-#pragma unroll
-  for (int st = 32; st > 0; st >>= 1) s = s + __shfl_down(s, st, 64);
-  return __shfl(s, 0, 64);
-}
-
-__device__ inline float2 wave_reduce_partials2(const float* __restrict__ part, int count) {
-  const int l = threadIdx.x & 63;
-  float va[4], vb[4];
-#pragma unroll
-  for (int qd = 0; qd < 4; ++qd) {
-    float a = 0.0f, b = 0.0f;
-    for (int j = l + 64 * qd; j < count; j += 256) {
-      a = a + part[2 * j];
-      b = b + part[2 * j + 1];
-    }
-    va[qd] = a;
-    vb[qd] = b;
-  }
-  float sa = (va[0] + va[2]) + (va[1] + va[3]);
-  float sb = (vb[0] + vb[2]) + (vb[1] + vb[3]);
-#pragma unroll
-  for (int st = 32; st > 0; st >>= 1) {
-    sa = sa + __shfl_down(sa, st, 64);
-    sb = sb + __shfl_down(sb, st, 64);
-  }
-  return make_float2(__shfl(sa, 0, 64), __shfl(sb, 0, 64));
-}
-
 // Group tree of per-row values held by physical position: the value goes to its logical slot first.
 __device__ inline float tree_logical(float v, uint32_t row, float* lds) {
   const uint32_t t = row == 0xffffffffu ? threadIdx.x : row - blockIdx.x * 256;
@@ -272,9 +230,8 @@ __global__ void __launch_bounds__(256) k_pcg_dir_spmv(SellView A, const float* _
 }
 
 // reduce.wgsl MODE 1 (alpha, breakdown) folded into update_dot.wgsl: x += alpha p, r -= alpha q,
-// z = D^-1 r, partial (r.z, r.r). The row's operands are loaded before the pq reduction. JZ = false (the
-// two-level preconditioner) stops after x and r: z and its dots come from the coarse-correction kernels.
-template <bool JZ>
+// z = D^-1 r, partial (r.z, r.r). The row's operands are loaded before the pq reduction. (The two-level
+// iteration folds the same update into its restriction and prolongation launches.)
 __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ partial_pq, int G, PcgState* st,
                                                     const float4* __restrict__ p, const float4* __restrict__ q,
                                                     const float* __restrict__ dinv, float4* x, float4* r, float4* z,
@@ -290,10 +247,8 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     qr = q[row];
     xr = x[row];
     rv = r[row];
-    if (JZ) {
 #pragma unroll
-      for (int e = 0; e < 9; ++e) d[e] = dinv[9 * row + e];
-    }
+    for (int e = 0; e < 9; ++e) d[e] = dinv[9 * row + e];
   }
   const float rzv = st->rz[it & 1];
   const float pq = wave_reduce_partials(partial_pq, G);
@@ -316,7 +271,6 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     rv.z = rv.z - alpha * qr.z;
     x[row] = xr;
     r[row] = rv;
-    if (!JZ) return;
     const float z0 = d[0] * rv.x + d[1] * rv.y + d[2] * rv.z;
     const float z1 = d[3] * rv.x + d[4] * rv.y + d[5] * rv.z;
     const float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
@@ -324,7 +278,6 @@ __global__ void __launch_bounds__(256) k_pcg_update(const float* __restrict__ pa
     rz = rv.x * z0 + rv.y * z1 + rv.z * z2;
     rr = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
   }
-  if (!JZ) return;
   const float2 sum = tree256x2(rz, rr, lds2);
   if (threadIdx.x == 0) {
     partial_rz[2 * blockIdx.x] = sum.x;
@@ -448,34 +401,54 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
   const int replace_every = c.rules.replace_every;
   const int chunk = replace_every > 0 ? replace_every : CHUNK_PLAIN;
   const int chunk_limit = (max_iterations + chunk - 1) / chunk + 1;
+  const int max_launch = chunk_limit * chunk;
+  // Iterations go out in polled groups: after each group the status is copied to pinned memory behind an
+  // event, and the host waits for a group's status only once the next group is queued, so the device never
+  // idles on the host and at most one group runs past the stop (its kernels return on the status).
+  const int group = coarse ? 5 : 10;
   float4* pbuf[2] = {c.p, c.p2};
   int chunks = 0, it = 0;
   bool stopped = false;
-  for (chunks = 1; chunks <= chunk_limit; ++chunks) {
-    const int first = it;
-    for (int j = 0; j < chunk; ++j, ++it) {
-      float4* pit = pbuf[it & 1];
-      if (it == first)
-        k_pcg_spmv<<<G, 256, 0, s>>>(A, pit, c.q, part_pq, c.state, R, 0);
-      else
-        k_pcg_dir_spmv<<<G, 256, 0, s>>>(A, part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pit, c.q, part_pq, R, it);
-      if (coarse) {
-        k_pcg_update<false><<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
-        TRY_RC(two_level_apply(c, part_rz, 0));
-      } else {
-        k_pcg_update<true><<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
+  int pending[2], npending = 0, slot = 0;
+  for (;;) {
+    if (it < max_launch) {
+      for (int q = 0; q < group && it < max_launch; ++q, ++it) {
+        float4* pit = pbuf[it & 1];
+        if (it % chunk == 0) {
+          ++chunks;
+          k_pcg_spmv<<<G, 256, 0, s>>>(A, pit, c.q, part_pq, c.state, R, 0);
+        } else {
+          k_pcg_dir_spmv<<<G, 256, 0, s>>>(A, part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pit, c.q, part_pq, R, it);
+        }
+        if (coarse) {
+          TRY_RC(two_level_iterate(c, part_pq, pit, part_rz, it));
+        } else {
+          k_pcg_update<<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
+        }
+        if ((it + 1) % chunk == 0) {
+          // direction update after the chunk's last iteration it: p_{it+1} = z + beta p_it into pbuf[(it+1) & 1]
+          k_pcg_direction<<<G, 256, 0, s>>>(part_rz, G, c.state, c.z, pbuf[it & 1], pbuf[(it + 1) & 1], R, it);
+          if (replace_every > 0) {
+            k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 1);
+            if (coarse) TRY_RC(two_level_apply(c, part_res, 1));
+            k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 1, it);
+          }
+        }
       }
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(c.poll_host + slot, &c.state->status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipEventRecord(c.ev_poll[slot], s));
+      pending[npending++] = slot;
+      slot ^= 1;
+      if (npending < 2 && it < max_launch) continue;
     }
-    // direction update after the chunk's last iteration it - 1: p_it = z + beta p_{it-1} into pbuf[it & 1]
-    k_pcg_direction<<<G, 256, 0, s>>>(part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pbuf[it & 1], R, it - 1);
-    if (replace_every > 0) {
-      k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 1);
-      if (coarse) TRY_RC(two_level_apply(c, part_res, 1));
-      k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 1, it - 1);
-    }
-    HIP_TRY(hipGetLastError());
-    TRY_RC(read_back(c, c.state_host, c.state, sizeof(PcgState)));
-    if (c.state_host->status != 0) {
+    if (npending == 0) break;
+    const int oldest = pending[0];
+    HIP_TRY(hipEventSynchronize(c.ev_poll[oldest]));
+    c.read_bytes += (int64_t)sizeof(int32_t);
+    pending[0] = pending[1];
+    --npending;
+    if (c.poll_host[oldest] != 0) {
       stopped = true;
       break;
     }

@@ -99,4 +99,46 @@ __device__ inline double2 tree256d2(double a, double b, double2* lds) {
 
 __device__ inline bool finite32(float v) { return (__float_as_uint(v) & 0x7f800000u) != 0x7f800000u; }
 
+// The reference's final reduction pass (reduce.wgsl:33-53, pcg.rs:447-457) inside one wavefront: lane l
+// stands for the four tree lanes l, l+64, l+128, l+192 (each the strided sum of its partials), combines
+// them as the 128- and 64-strides of the tree do, and finishes with shuffles. No LDS, no barrier: every
+// wavefront obtains the identical result.
+__device__ inline float wave_reduce_partials(const float* __restrict__ part, int count) {
+  const int l = threadIdx.x & 63;
+  float v[4];
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    float a = 0.0f;
+    for (int j = l + 64 * qd; j < count; j += 256) a = a + part[j];
+    v[qd] = a;
+  }
+  float s = (v[0] + v[2]) + (v[1] + v[3]);
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1) s = s + __shfl_down(s, st, 64);
+  return __shfl(s, 0, 64);
+}
+
+__device__ inline float2 wave_reduce_partials2(const float* __restrict__ part, int count) {
+  const int l = threadIdx.x & 63;
+  float va[4], vb[4];
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    float a = 0.0f, b = 0.0f;
+    for (int j = l + 64 * qd; j < count; j += 256) {
+      a = a + part[2 * j];
+      b = b + part[2 * j + 1];
+    }
+    va[qd] = a;
+    vb[qd] = b;
+  }
+  float sa = (va[0] + va[2]) + (va[1] + va[3]);
+  float sb = (vb[0] + vb[2]) + (vb[1] + vb[3]);
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1) {
+    sa = sa + __shfl_down(sa, st, 64);
+    sb = sb + __shfl_down(sb, st, 64);
+  }
+  return make_float2(__shfl(sa, 0, 64), __shfl(sb, 0, 64));
+}
+
 }  // namespace drape

@@ -679,6 +679,109 @@ __global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ par
   }
 }
 
+// One iteration's x / r update (reduce.wgsl MODE 1 with update_dot.wgsl) folded into the preconditioner
+// launches: every wavefront forms alpha = rz / pq from the p.q partials in the fixed order, the restriction
+// gathers the children's updated residual r - alpha q on the fly, and the prolongation writes
+// x += alpha p and r -= alpha q before z = D^-1 r + P y. The arithmetic is that of the separate update,
+// so the iterates are bit-identical to it.
+__global__ void __launch_bounds__(256) k_restrict_update(const uint32_t* __restrict__ child_ptr,
+                                                         const uint32_t* __restrict__ child,
+                                                         const float* __restrict__ w, const float4* __restrict__ r,
+                                                         const float4* __restrict__ q, int Nc, float* __restrict__ rc,
+                                                         const float* __restrict__ partial_pq, int G, PcgState* st,
+                                                         int it) {
+  if (st->status != 0) return;
+  const float rzv = st->rz[it & 1];
+  const float pq = wave_reduce_partials(partial_pq, G);
+  const float alpha = rzv / pq;
+  if (!(pq > 0.0f && finite32(alpha))) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->status = 3;
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pq = pq;
+    st->alpha = alpha;
+  }
+  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (I >= Nc) return;
+  const uint32_t t0 = child_ptr[I], n = child_ptr[I + 1] - t0;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (uint32_t j = lane; j < n; j += 64) {
+    const uint32_t s = child[t0 + j];
+    const float ww = w[s];
+    const float4 v = r[s >> 2], qq = q[s >> 2];
+    const float rx = v.x - alpha * qq.x, ry = v.y - alpha * qq.y, rz = v.z - alpha * qq.z;
+    a0 = a0 + ww * rx;
+    a1 = a1 + ww * ry;
+    a2 = a2 + ww * rz;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 = a0 + __shfl_down(a0, o, 64);
+    a1 = a1 + __shfl_down(a1, o, 64);
+    a2 = a2 + __shfl_down(a2, o, 64);
+  }
+  if (lane == 0) {
+    rc[3 * I] = a0;
+    rc[3 * I + 1] = a1;
+    rc[3 * I + 2] = a2;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restrict__ parent, const float* __restrict__ w,
+                                                        const float* __restrict__ yc, const float* __restrict__ dinv,
+                                                        const float4* __restrict__ p, const float4* __restrict__ q,
+                                                        float4* __restrict__ x, float4* __restrict__ r,
+                                                        float4* __restrict__ z, float* __restrict__ partials, int R,
+                                                        const float* __restrict__ partial_pq, int G,
+                                                        const PcgState* st, int it) {
+  __shared__ float2 lds[256];
+  if (st->status != 0) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  float4 pr = make_float4(0, 0, 0, 0), qr = pr, xr = pr, rv = pr;
+  if (row < R) {
+    pr = p[row];
+    qr = q[row];
+    xr = x[row];
+    rv = r[row];
+  }
+  const float alpha = st->rz[it & 1] / wave_reduce_partials(partial_pq, G);
+  float rzs = 0.0f, rrs = 0.0f;
+  if (row < R) {
+    xr.x = xr.x + alpha * pr.x;
+    xr.y = xr.y + alpha * pr.y;
+    xr.z = xr.z + alpha * pr.z;
+    rv.x = rv.x - alpha * qr.x;
+    rv.y = rv.y - alpha * qr.y;
+    rv.z = rv.z - alpha * qr.z;
+    x[row] = xr;
+    r[row] = rv;
+    const float* d = dinv + 9 * row;
+    float z0 = d[0] * rv.x + d[1] * rv.y + d[2] * rv.z;
+    float z1 = d[3] * rv.x + d[4] * rv.y + d[5] * rv.z;
+    float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int I = parent[4 * row + k];
+      if (I >= 0) {
+        const float ww = w[4 * row + k];
+        z0 = z0 + ww * yc[3 * I];
+        z1 = z1 + ww * yc[3 * I + 1];
+        z2 = z2 + ww * yc[3 * I + 2];
+      }
+    }
+    z[row] = make_float4(z0, z1, z2, 0.0f);
+    rzs = rv.x * z0 + rv.y * z1 + rv.z * z2;
+    rrs = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
+  }
+  const float2 sum = tree256x2(rzs, rrs, lds);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sum.x;
+    partials[2 * blockIdx.x + 1] = sum.y;
+  }
+}
+
 // ---- host ------------------------------------------------------------------------------------
 // Coarse layout of a grid sheet with `columns` vertices per side ((columns - 1) divisible by 4): parents,
 // bilinear weights, children and the Galerkin term lists.
@@ -992,6 +1095,19 @@ int two_level_apply(Context& c, float* partials, int guard) {
   k_tri_gemv<true><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, guard);
   k_tri_gemv<false><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, guard);
   k_prolong<<<c.G, 256, 0, s>>>(t->d_parent, t->d_w, t->d_yc, c.dinv, c.r, c.z, c.p, partials, c.R, c.state, guard);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+int two_level_iterate(Context& c, const float* partial_pq, const float4* p, float* partial_rz, int it) {
+  TwoLevel* t = c.two_level;
+  hipStream_t s = c.stream;
+  k_restrict_update<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, c.r, c.q, t->Nc, t->d_rc,
+                                                    partial_pq, c.G, c.state, it);
+  k_tri_gemv<true><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, 0);
+  k_tri_gemv<false><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, 0);
+  k_prolong_update<<<c.G, 256, 0, s>>>(t->d_parent, t->d_w, t->d_yc, c.dinv, p, c.q, c.x, c.r, c.z, partial_rz, c.R,
+                                       partial_pq, c.G, c.state, it);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }
