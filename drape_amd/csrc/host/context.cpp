@@ -40,7 +40,7 @@ static void free_context(Context* c) {
                   c->colliders, c->mu, c->unit_mass_dev, c->org, c->material, c->lgrad, c->lhess, c->bc_ptr,
                   c->bc_idx, c->fc_ptr, c->fc_idx, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->perm, c->diag_slot,
                   c->block_dst, c->csr_row_ptr, c->csr_col, c->b, c->x, c->r, c->z, c->p, c->p2, c->q, c->dinv,
-                  c->partials, c->state, c->refine, c->b_saved, c->vel, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
+                  c->partials, c->state, c->refine, c->b_saved, c->z1, c->r1, c->vel, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->state_host) (void)hipHostFree(c->state_host);
@@ -164,6 +164,8 @@ static int alloc_solver(Context& c, const std::vector<uint32_t>& row_ptr, const 
   TRY(alloc(&c.ipart, 2 * (size_t)c.G + 16));
   TRY(alloc(&c.refine, 6 * Rp));
   TRY(alloc(&c.b_saved, Rp));
+  TRY(alloc(&c.z1, Rp));
+  TRY(alloc(&c.r1, Rp));
   c.row_ptr_host = row_ptr;
   c.col_idx_host = col_idx;
   HIP_TRY(hipEventCreate(&c.ev0));
@@ -315,6 +317,7 @@ void drape_default_rules(drape_rules* r) {
   r->coarse_reuse = 3;
   r->fallback = 2;
   r->coarse_shift = 0.0;
+  r->smoother = 0.0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }

@@ -92,6 +92,8 @@ struct Context {
   hipEvent_t ev_poll[2] = {nullptr, nullptr};
   double* refine = nullptr;       // FP64 refinement vectors [3R] x2
   float4* b_saved = nullptr;      // the right-hand side while the refinement loop solves for corrections
+  float4* z1 = nullptr;           // V-cycle: pre-smoothed correction [G*256]
+  float4* r1 = nullptr;           // V-cycle: residual after pre-smoothing [G*256]
   float4* vel = nullptr;          // [V] velocities of the dynamic loop (allocated on first use)
 
   // reductions for the Newton loop (FP64)
@@ -200,6 +202,15 @@ int two_level_apply(Context& c, float* partials, int guard);  // z = M^-1 r and 
 // One two-level PCG iteration after its SpMV: alpha from the p.q partials, x += alpha p, r -= alpha q and
 // z = M^-1 r with partial (r.z, r.r), in the restriction, the coarse solve and the prolongation launches.
 int two_level_iterate(Context& c, const float* partial_pq, const float4* p, float* partial_rz, int it);
+// The coarse correction's parts for the V-cycle: yc = B^T B P^T r (guarded like two_level_apply) and the
+// device arrays the prolongation reads.
+struct CoarseView {
+  const int32_t* parent;  // [4R] coarse row of each parent slot, -1 if none
+  const float* w;         // [4R] prolongation weights
+  const float* yc;        // [3Nc] the coarse solution
+};
+CoarseView two_level_view(const Context& c);
+int two_level_restrict_solve(Context& c, const float4* r, int guard);
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host

@@ -375,7 +375,152 @@ __global__ void __launch_bounds__(256) k_pcg_residual_final(const float* __restr
   }
 }
 
+// ---- two-level V-cycle (symmetric multiplicative two-level preconditioner) --------------------------
+// z1 = w D^-1 r;  r1 = r - A z1;  y = B^T B P^T r1;  z2 = z1 + P y;  z = z2 + w D^-1 (r - A z2).
+// Guards as in the coarse correction: 0 runs while the solve runs, 1 with a residual replacement, 2 always.
+__device__ inline bool vc_skip(const PcgState* st, int guard) {
+  return (guard == 0 && st->status != 0) || (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0)));
+}
+
+// z1 = w D^-1 r (setup and replacement)
+__global__ void __launch_bounds__(256) k_vc_smooth(const float* __restrict__ dinv, const float4* __restrict__ r,
+                                                   float4* __restrict__ z1, float omega, int R, const PcgState* st,
+                                                   int guard) {
+  if (vc_skip(st, guard)) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= R) return;
+  const float4 rv = r[row];
+  const float3 d = precondition(dinv, row, rv.x, rv.y, rv.z);
+  z1[row] = make_float4(omega * d.x, omega * d.y, omega * d.z, 0.0f);
+}
+
+// One iteration's update (reduce.wgsl MODE 1 with update_dot.wgsl): alpha from the p.q partials, breakdown,
+// x += alpha p, r -= alpha q; then z1 = w D^-1 r.
+__global__ void __launch_bounds__(256) k_vc_update(const float* __restrict__ partial_pq, int G, PcgState* st,
+                                                   const float4* __restrict__ p, const float4* __restrict__ q,
+                                                   const float* __restrict__ dinv, float4* x, float4* r,
+                                                   float4* __restrict__ z1, float omega, int R, int it) {
+  if (st->status != 0) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  const bool live = row < R;
+  float4 pr = make_float4(0, 0, 0, 0), qr = pr, xr = pr, rv = pr;
+  if (live) {
+    pr = p[row];
+    qr = q[row];
+    xr = x[row];
+    rv = r[row];
+  }
+  const float rzv = st->rz[it & 1];
+  const float pq = wave_reduce_partials(partial_pq, G);
+  const float alpha = rzv / pq;
+  if (!(pq > 0.0f && finite32(alpha))) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->status = 3;
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pq = pq;
+    st->alpha = alpha;
+  }
+  if (!live) return;
+  xr.x = xr.x + alpha * pr.x;
+  xr.y = xr.y + alpha * pr.y;
+  xr.z = xr.z + alpha * pr.z;
+  rv.x = rv.x - alpha * qr.x;
+  rv.y = rv.y - alpha * qr.y;
+  rv.z = rv.z - alpha * qr.z;
+  x[row] = xr;
+  r[row] = rv;
+  const float3 d = precondition(dinv, row, rv.x, rv.y, rv.z);
+  z1[row] = make_float4(omega * d.x, omega * d.y, omega * d.z, 0.0f);
+}
+
+// r1 = r - A z1
+__global__ void __launch_bounds__(256) k_vc_residual1(SellView A, const float4* __restrict__ z1,
+                                                      const float4* __restrict__ r, float4* __restrict__ r1,
+                                                      const PcgState* st, int guard) {
+  if (vc_skip(st, guard)) return;
+  const int phys = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t row = A.perm[phys];
+  if (row == 0xffffffffu) return;
+  const float3 a = row_times<false>(A, phys, z1, nullptr, 0.0f);
+  const float4 rv = r[row];
+  r1[row] = make_float4(rv.x - a.x, rv.y - a.y, rv.z - a.z, 0.0f);
+}
+
+// z2 of a row: z1 + sum over its parent slots of w y_parent
+__device__ inline float4 vc_z2(const float4* __restrict__ z1, const CoarseView& cv, uint32_t row) {
+  float4 v = z1[row];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int I = cv.parent[4 * row + k];
+    if (I >= 0) {
+      const float ww = cv.w[4 * row + k];
+      v.x = v.x + ww * cv.yc[3 * I];
+      v.y = v.y + ww * cv.yc[3 * I + 1];
+      v.z = v.z + ww * cv.yc[3 * I + 2];
+    }
+  }
+  return v;
+}
+
+// A z2 with z2 formed for the gathered columns, then z = z2 + w D^-1 (r - A z2), partial (r.z, r.r) by
+// logical row; with guard 2 also p = z.
+__global__ void __launch_bounds__(256) k_vc_post(SellView A, const float4* __restrict__ z1, CoarseView cv,
+                                                 const float* __restrict__ dinv, const float4* __restrict__ r,
+                                                 float4* __restrict__ z, float4* __restrict__ p, float omega,
+                                                 float* __restrict__ partials, const PcgState* st, int guard) {
+  __shared__ float2 lds[256];
+  if (vc_skip(st, guard)) return;
+  const int phys = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t row = A.perm[phys];
+  float rz = 0.0f, rr = 0.0f;
+  if (row != 0xffffffffu) {
+    const int sl = phys >> 6, l = phys & 63;
+    const uint32_t base = A.slice_ptr[sl], n = A.cnt[phys];
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t q = base + k;
+      const uint32_t col = A.col[q * 64 + l];
+      const float* vp = A.val + (size_t)q * 9 * 64 + l;
+      const float4 w = vc_z2(z1, cv, col);
+      a0 = a0 + (vp[0] * w.x + vp[64] * w.y + vp[128] * w.z);
+      a1 = a1 + (vp[192] * w.x + vp[256] * w.y + vp[320] * w.z);
+      a2 = a2 + (vp[384] * w.x + vp[448] * w.y + vp[512] * w.z);
+    }
+    const float4 rv = r[row];
+    const float4 own = vc_z2(z1, cv, row);
+    const float3 d = precondition(dinv, row, rv.x - a0, rv.y - a1, rv.z - a2);
+    const float4 zz = make_float4(own.x + omega * d.x, own.y + omega * d.y, own.z + omega * d.z, 0.0f);
+    z[row] = zz;
+    if (guard == 2) p[row] = zz;
+    rz = rv.x * zz.x + rv.y * zz.y + rv.z * zz.z;
+    rr = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
+  }
+  const float2 sum = tree_logical2(rz, rr, row, lds);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sum.x;
+    partials[2 * blockIdx.x + 1] = sum.y;
+  }
+}
+
 static SellView view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm}; }
+
+// z = M^-1 r by the V-cycle for the current r (after the caller's z1 = w D^-1 r): guard as above.
+static int vcycle_rest(Context& c, float* partials, int guard) {
+  hipStream_t s = c.stream;
+  const float omega = (float)c.rules.smoother;
+  SellView A = view_of(c);
+  k_vc_residual1<<<c.G, 256, 0, s>>>(A, c.z1, c.r, c.r1, c.state, guard);
+  TRY_RC(two_level_restrict_solve(c, c.r1, guard));
+  k_vc_post<<<c.G, 256, 0, s>>>(A, c.z1, two_level_view(c), c.dinv, c.r, c.z, c.p, omega, partials, c.state, guard);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+static int vcycle_apply(Context& c, float* partials, int guard) {
+  k_vc_smooth<<<c.G, 256, 0, c.stream>>>(c.dinv, c.r, c.z1, (float)c.rules.smoother, c.R, c.state, guard);
+  return vcycle_rest(c, partials, guard);
+}
 
 // One PCG solve; `coarse` adds the two-level coarse correction to block-Jacobi (z = D^-1 r + P A_c^-1 P^T r).
 static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats, bool coarse) {
@@ -390,7 +535,9 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
   hipStream_t s = c.stream;
   HIP_TRY(hipEventRecord(c.ev0, s));
   k_pcg_setup<<<G, 256, 0, s>>>(A, c.diag_slot, c.b, c.dinv, c.x, c.r, c.z, c.p, part_setup, R);
-  if (coarse) TRY_RC(two_level_apply(c, part_setup, 2));
+  const bool vcycle = coarse && c.rules.smoother > 0.0;
+  if (vcycle) TRY_RC(vcycle_apply(c, part_setup, 2));
+  else if (coarse) TRY_RC(two_level_apply(c, part_setup, 2));
   // a two-level solve whose residual grows 1e4-fold has met curvature FP32 cannot resolve: a breakdown
   const float diverge2 = coarse ? 1e8f : INFINITY;
   k_pcg_start<<<1, 256, 0, s>>>(part_setup, G, c.state, tol2, above2, max_iterations, diverge2);
@@ -420,7 +567,11 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
         } else {
           k_pcg_dir_spmv<<<G, 256, 0, s>>>(A, part_rz, G, c.state, c.z, pbuf[(it - 1) & 1], pit, c.q, part_pq, R, it);
         }
-        if (coarse) {
+        if (vcycle) {
+          k_vc_update<<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z1,
+                                        (float)c.rules.smoother, R, it);
+          TRY_RC(vcycle_rest(c, part_rz, 0));
+        } else if (coarse) {
           TRY_RC(two_level_iterate(c, part_pq, pit, part_rz, it));
         } else {
           k_pcg_update<<<G, 256, 0, s>>>(part_pq, G, c.state, pit, c.q, c.dinv, c.x, c.r, c.z, part_rz, R, it);
@@ -430,7 +581,8 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
           k_pcg_direction<<<G, 256, 0, s>>>(part_rz, G, c.state, c.z, pbuf[it & 1], pbuf[(it + 1) & 1], R, it);
           if (replace_every > 0) {
             k_pcg_residual<<<G, 256, 0, s>>>(A, c.b, c.x, c.dinv, c.r, c.z, part_res, c.state, R, 1);
-            if (coarse) TRY_RC(two_level_apply(c, part_res, 1));
+            if (vcycle) TRY_RC(vcycle_apply(c, part_res, 1));
+            else if (coarse) TRY_RC(two_level_apply(c, part_res, 1));
             k_pcg_residual_final<<<1, 256, 0, s>>>(part_res, G, c.state, 1, it);
           }
         }

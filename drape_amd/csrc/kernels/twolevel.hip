@@ -98,6 +98,21 @@ __device__ inline double* band_tile(double* band, int kb, int k, int d) {
   return band + (((size_t)k * (kb + 1) + d) << 12);
 }
 
+// Copies a ROWS x 64 row-major FP64 tile into LDS rows of stride LSTR: every load of the thread is issued
+// before its first store, so the tile costs one memory latency instead of one per element.
+template <int ROWS>
+__device__ inline void tile_to_lds(const double* __restrict__ src, double* dst) {
+  constexpr int PER = ROWS * TILE / 256;
+  double v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) v[i] = src[threadIdx.x + 256 * i];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    dst[(e >> 6) * LSTR + (e & 63)] = v[i];
+  }
+}
+
 // Entry q of the coarse block pair (I, J) into the band where it lies on or below the diagonal: one thread
 // per (pair, entry), the pair's terms in their fixed order.
 __global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restrict__ pair_ptr,
@@ -163,10 +178,8 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ band, d
   if (*info != 0) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double* tile = band_tile(band, kb, k, 0);
-  for (int e = tid; e < TILE * TILE; e += 256) {
-    Ls[(e >> 6) * LSTR + (e & 63)] = tile[e];
-    Xs[(e >> 6) * LSTR + (e & 63)] = 0.0;
-  }
+  tile_to_lds<TILE>(tile, Ls);
+  for (int e = tid; e < TILE * TILE; e += 256) Xs[(e >> 6) * LSTR + (e & 63)] = 0.0;
   if (tid == 0) fail = 0;
   __syncthreads();
   for (int b = 0; b < 4; ++b) {
@@ -352,8 +365,8 @@ __global__ void __launch_bounds__(256) k_chol_trsm(const double* __restrict__ ba
   const int tid = threadIdx.x;
   const double* a = band_tile(const_cast<double*>(band), kb, k, d) + (quarter * 16 << 6);
   const double* inv = linv + ((size_t)k << 12);
-  for (int e = tid; e < TILE * TILE; e += 256) Xs[(e >> 6) * LSTR + (e & 63)] = inv[e];
-  for (int e = tid; e < 16 * TILE; e += 256) As[(e >> 6) * LSTR + (e & 63)] = a[e];
+  tile_to_lds<TILE>(inv, Xs);
+  tile_to_lds<16>(a, As);
   __syncthreads();
   // thread: row r (0..15), columns c0 + 16 v (16 consecutive rows of Xs per load: no bank conflicts)
   const int r = tid >> 4, c0 = tid & 15;
@@ -387,8 +400,8 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ band, 
   const int tid = threadIdx.x;
   const double* p = lpanel + (((size_t)k * kb + (d1 - 1)) << 12) + (quarter * 16 << 6);
   const double* q = lpanel + (((size_t)k * kb + (d2 - 1)) << 12);
-  for (int e = tid; e < TILE * TILE; e += 256) Qs[(e >> 6) * LSTR + (e & 63)] = q[e];
-  for (int e = tid; e < 16 * TILE; e += 256) Ps[(e >> 6) * LSTR + (e & 63)] = p[e];
+  tile_to_lds<TILE>(q, Qs);
+  tile_to_lds<16>(p, Ps);
   __syncthreads();
   const int r = tid >> 4, c0 = tid & 15;
   double o[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1108,6 +1121,21 @@ int two_level_iterate(Context& c, const float* partial_pq, const float4* p, floa
   k_tri_gemv<false><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, 0);
   k_prolong_update<<<c.G, 256, 0, s>>>(t->d_parent, t->d_w, t->d_yc, c.dinv, p, c.q, c.x, c.r, c.z, partial_rz, c.R,
                                        partial_pq, c.G, c.state, it);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+CoarseView two_level_view(const Context& c) {
+  const TwoLevel* t = c.two_level;
+  return {t->d_parent, t->d_w, t->d_yc};
+}
+
+int two_level_restrict_solve(Context& c, const float4* r, int guard) {
+  TwoLevel* t = c.two_level;
+  hipStream_t s = c.stream;
+  k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, r, t->Nc, t->d_rc, c.state, guard);
+  k_tri_gemv<true><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, guard);
+  k_tri_gemv<false><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, guard);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }

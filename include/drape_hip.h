@@ -88,6 +88,10 @@ typedef struct {
   double coarse_shift;    /* two-level: the coarse operator's diagonal is scaled by (1 + coarse_shift) before
                              the factorisation, capping the coarse correction of modes whose curvature is
                              below what FP32 resolves of the stiff membrane */
+  double smoother;        /* two-level: 0 applies D^-1 + P A_c^-1 P^T (additive); w > 0 the symmetric
+                             multiplicative cycle z1 = w D^-1 r, z2 = z1 + P A_c^-1 P^T (r - A z1),
+                             z = z2 + w D^-1 (r - A z2), which is positive definite while
+                             w lambda_max(D^-1 A) < 2 */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0

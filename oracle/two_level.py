@@ -97,11 +97,21 @@ def block_jacobi_inverse(A: sp.csr_matrix) -> sp.csr_matrix:
     return sp.bsr_matrix((inv, np.arange(R), np.arange(R + 1)), shape=A.shape).tocsr()
 
 
+def vcycle(A, Dinv, P, Minv_c, omega, r):
+    """The symmetric multiplicative two-level application (V(1,1) with damped block-Jacobi smoothing):
+    z1 = w D^-1 r; z2 = z1 + P A_c^-1 P^T (r - A z1); z = z2 + w D^-1 (r - A z2). Its error propagation
+    is I - M^-1 A = (I - w D^-1 A)(I - P A_c^-1 P^T A)(I - w D^-1 A)."""
+    z1 = omega * (Dinv @ r)
+    z2 = z1 + P @ (Minv_c @ (P.T @ (r - A @ z1)))
+    return z2 + omega * (Dinv @ (r - A @ z2))
+
+
 def two_level_pcg(A: sp.csr_matrix, b: np.ndarray, P: sp.spmatrix | None, tolerance: float,
-                  max_iterations: int = 20000):
-    """f64 PCG with M^-1 = D^-1 + P A_c^-1 P^T (block-Jacobi alone when P is None), x0 = 0, stop when
-    |r|^2 <= tol^2 |r0|^2 (the recursive residual, pcg.rs:247-259 order of tests), breakdown when
-    p^T A p <= 0 or alpha is not finite. Returns (x, iterations, status, history of |r|/|r0|)."""
+                  max_iterations: int = 20000, omega: float = 0.0):
+    """f64 PCG with M^-1 = D^-1 + P A_c^-1 P^T (block-Jacobi alone when P is None), or with the V-cycle of
+    `vcycle` when omega > 0, x0 = 0, stop when |r|^2 <= tol^2 |r0|^2 (the recursive residual,
+    pcg.rs:247-259 order of tests), breakdown when p^T A p <= 0 or alpha is not finite. Returns (x,
+    iterations, status, history of |r|/|r0|)."""
     Dinv = block_jacobi_inverse(A)
     if P is not None:
         Ac = galerkin(A, P)
@@ -109,8 +119,12 @@ def two_level_pcg(A: sp.csr_matrix, b: np.ndarray, P: sp.spmatrix | None, tolera
         Linv = np.linalg.inv(L)
         Minv_c = Linv.T @ Linv
 
-        def apply(r):
-            return Dinv @ r + P @ (Minv_c @ (P.T @ r))
+        if omega > 0:
+            def apply(r):
+                return vcycle(A, Dinv, P, Minv_c, omega, r)
+        else:
+            def apply(r):
+                return Dinv @ r + P @ (Minv_c @ (P.T @ r))
     else:
         def apply(r):
             return Dinv @ r

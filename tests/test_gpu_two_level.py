@@ -15,11 +15,11 @@ from oracle import two_level as TL
 pytestmark = pytest.mark.gpu
 
 
-def pinned_system(n, h=1.0):
+def pinned_system(n, h=1.0, rules=None):
     """The reference's pinned sheet (drape.rs:53-155) at rest with a sagged x_hat and step size h."""
     ps = D.reference_scene(n, sphere_on_floor=False, placed=True)
     arrays = D.SceneArrays(ps)
-    ctx = D.Drape(arrays)
+    ctx = D.Drape(arrays, rules)
     rest = arrays.array("rest_positions")
     x_hat = rest.copy()
     x_hat[2::3] -= 1e-3
@@ -105,6 +105,31 @@ def test_two_level_pcg_matches_f64_two_level(n, h, tol):
     assert abs(st.iterations - oit) <= max(3, 0.15 * oit)
     assert true_rel <= 1.5 * tol
     assert st.iterations * 2 < st_bj.iterations
+    ctx.close()
+
+
+@pytest.mark.parametrize("n,h,tol", [(65, 0.1, 1e-2), (65, 0.03, 1e-3)])
+def test_vcycle_pcg_matches_f64_vcycle(n, h, tol):
+    """The symmetric multiplicative cycle (rules.smoother = 0.5): the GPU's FP32 PCG takes the iteration count
+    of the f64 V-cycle PCG (oracle/two_level.py, pinned by its error-propagation identity) within 15% (+3),
+    reaches the tolerance in the true residual (1.5x), and needs fewer iterations than the additive form."""
+    arrays, ctx = pinned_system(n, h, D.default_rules(smoother=0.5))
+    rhs, vals = ctx.system()
+    A = TL.bsr_matrix(arrays.array("row_ptr"), arrays.array("col_idx"), vals.astype(np.float64))
+    b = rhs.astype(np.float64)
+    ctx.set_preconditioner("two_level")
+    nc, parent, w, _ = ctx.two_level_layout()
+    st, x = ctx.solve_once(tol, 20000)
+    P = TL.prolongation(parent, w.astype(np.float64), nc)
+    _, oit, ost, _ = TL.two_level_pcg(A, b, P, tol, 20000, omega=0.5)
+    _, ait, _, _ = TL.two_level_pcg(A, b, P, tol, 20000)
+    true_rel = np.linalg.norm(b - A @ x.astype(np.float64)) / np.linalg.norm(b)
+    print(f"[gpu] V-cycle PCG n{n} h={h} tol={tol}: {st.iterations} iterations (f64 V-cycle {oit}, f64 additive "
+          f"{ait}); true relative residual {true_rel:.2e}", flush=True)
+    assert D.PCG_STATUS[st.status] == "converged" and ost == 1
+    assert abs(st.iterations - oit) <= max(3, 0.15 * oit)
+    assert true_rel <= 1.5 * tol
+    assert st.iterations < ait
     ctx.close()
 
 

@@ -103,3 +103,47 @@ def test_without_coarse_space_it_is_the_block_jacobi_pcg_of_the_oracle():
                       single=False)
     assert abs(it1 - so.iterations) <= 1
     assert np.linalg.norm(x1 - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
+def test_vcycle_error_propagation_and_symmetry():
+    """The V-cycle application against the dense product of its three error-propagation factors, built
+    independently from dense D^-1, P and A_c^-1; its operator is symmetric, positive definite for
+    w lambda_max(D^-1 A) < 2 and not for w well above that."""
+    _, a, A, _, _ = scene_system(9)
+    nc, parent, w = TL.coarse_layout(9, a["row_vertex"])
+    P = TL.prolongation(parent, w, nc)
+    Ad = A.toarray()
+    n = Ad.shape[0]
+    Dd = np.zeros_like(Ad)
+    for i in range(0, n, 3):
+        Dd[i:i + 3, i:i + 3] = np.linalg.inv(Ad[i:i + 3, i:i + 3])
+    Pd = P.toarray()
+    Acinv = np.linalg.inv(Pd.T @ Ad @ Pd)
+    lam = np.max(np.real(np.linalg.eigvals(Dd @ Ad)))
+    for omega in (0.5 / lam, 1.0 / lam, 1.9 / lam):
+        S = np.eye(n) - omega * Dd @ Ad
+        E = S @ (np.eye(n) - Pd @ Acinv @ Pd.T @ Ad) @ S
+        Minv = (np.eye(n) - E) @ np.linalg.inv(Ad)
+        Minv_c = np.linalg.inv(np.linalg.cholesky(Pd.T @ Ad @ Pd))
+        Minv_c = Minv_c.T @ Minv_c
+        Dinv = TL.block_jacobi_inverse(A)
+        cols = np.stack([TL.vcycle(A, Dinv, P, Minv_c, omega, e) for e in np.eye(n)], 1)
+        assert np.abs(cols - Minv).max() <= 1e-9 * np.abs(Minv).max()
+        assert np.abs(cols - cols.T).max() <= 1e-9 * np.abs(cols).max()
+        assert np.linalg.eigvalsh(0.5 * (cols + cols.T)).min() > 0
+    S = np.eye(n) - (3.0 / lam) * Dd @ Ad
+    E = S @ (np.eye(n) - Pd @ Acinv @ Pd.T @ Ad) @ S
+    Minv = (np.eye(n) - E) @ np.linalg.inv(Ad)
+    assert np.linalg.eigvalsh(0.5 * (Minv + Minv.T)).min() < 0
+
+
+def test_vcycle_pcg_solves_with_fewer_iterations():
+    _, a, A, rhs, _ = scene_system(33, h=0.05)
+    nc, parent, w = TL.coarse_layout(33, a["row_vertex"])
+    P = TL.prolongation(parent, w, nc)
+    xd = spla.spsolve(A.tocsc(), rhs)
+    x1, it1, st1, _ = TL.two_level_pcg(A, rhs, P, 1e-10, 5000)
+    x2, it2, st2, _ = TL.two_level_pcg(A, rhs, P, 1e-10, 5000, omega=0.5)
+    assert st1 == 1 and st2 == 1
+    assert np.abs(x2 - xd).max() <= 1e-7 * np.abs(xd).max()
+    assert it2 < it1
