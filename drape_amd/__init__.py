@@ -151,6 +151,9 @@ def lib():
         L.drape_copy_timing.argtypes = [C.c_int64, C.c_int32, f64p]
         L.drape_set_prolongation_model.argtypes = [vp, f32p, C.c_int32]
         L.drape_two_level_weights.argtypes = [vp, f32p]
+        L.drape_training_vectors_set.argtypes = [vp, f32p, C.c_int32]
+        L.drape_training_vectors_make.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, f32p]
+        L.drape_prolongation_gradient.argtypes = [vp, f32p, C.c_int32, f64p, f32p]
         L.drape_solution_fp64.argtypes = [vp, f64p]
         L.drape_simulate.argtypes = [vp, C.c_int32, f32p, f32p, f32p, f32p, i32p, f32p, C.POINTER(Record), C.c_int32,
                                      i32p]
@@ -472,6 +475,28 @@ class Drape:
             return
         theta = _f32(params)
         _check(lib().drape_set_prolongation_model(self.h, _ptr(theta, f32p), theta.size))
+
+    def training_vectors(self, vectors=None, count=16, sweeps=5, omega=0.6, seed=0):
+        """The error vectors of the prolongation loss, kept on the device: `vectors` [count, 3R] when given,
+        else made on the device (random starts, damped block-Jacobi sweeps on the assembled system) and
+        returned as float32 [count, 3R]."""
+        if vectors is not None:
+            v = np.ascontiguousarray(np.asarray(vectors, np.float32).reshape(-1, 3 * self.R))
+            _check(lib().drape_training_vectors_set(self.h, _ptr(v, f32p), v.shape[0]))
+            return v
+        out = np.zeros((count, 3 * self.R), np.float32)
+        _check(lib().drape_training_vectors_make(self.h, count, sweeps, omega, seed, _ptr(out, f32p)))
+        return out
+
+    def prolongation_gradient(self, params):
+        """(loss, dloss/dparams) of the learned prolongation at `params` over the training vectors, on the
+        device; `params` becomes the context's prolongation model."""
+        theta = _f32(params)
+        grad = np.zeros(theta.size, np.float32)
+        loss = C.c_double()
+        _check(lib().drape_prolongation_gradient(self.h, _ptr(theta, f32p), theta.size, C.byref(loss),
+                                                 _ptr(grad, f32p)))
+        return loss.value, grad
 
     def two_level_weights(self):
         """Parent-slot weights [R, 4] of the current coarse factorisation."""

@@ -13,6 +13,7 @@ SOURCES = [
     "kernels/elements.hip",
     "kernels/twolevel.hip",
     "kernels/gnn.hip",
+    "kernels/gnn_train.hip",
     "kernels/prolong.hip",
     "kernels/builder.hip",
     "host/context.cpp",

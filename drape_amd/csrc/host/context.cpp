@@ -47,6 +47,7 @@ static void free_context(Context* c) {
   if (c->poll_host) (void)hipHostFree(c->poll_host);
   if (c->dout_host) (void)hipHostFree(c->dout_host);
   if (c->two_level) two_level_destroy(c->two_level);
+  if (c->gnn_train) gnn_train_destroy(c->gnn_train);
   for (hipEvent_t e : {c->ev0, c->ev1, c->ev_spmv0, c->ev_spmv1, c->ev_poll[0], c->ev_poll[1]})
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -652,6 +653,35 @@ int drape_set_prolongation_model(drape_ctx* p, const float* params, int32_t coun
   if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
   if (!c->two_level) TRY(two_level_create(*c, c->problem.grid_columns));
   return two_level_set_model(*c, params, count);
+}
+
+static Context* training_context(drape_ctx* p) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear) {
+    set_error("not a mesh context");
+    return nullptr;
+  }
+  if (!c->two_level && two_level_create(*c, c->problem.grid_columns) != DRAPE_OK) return nullptr;
+  return c;
+}
+
+int drape_training_vectors_set(drape_ctx* p, const float* vectors, int32_t count) {
+  Context* c = training_context(p);
+  if (!c) return DRAPE_E_INVALID;
+  return gnn_train_set_vectors(*c, vectors, count);
+}
+
+int drape_training_vectors_make(drape_ctx* p, int32_t count, int32_t sweeps, float omega, uint64_t seed,
+                                float* out) {
+  Context* c = training_context(p);
+  if (!c) return DRAPE_E_INVALID;
+  return gnn_train_make_vectors(*c, count, sweeps, omega, seed, out);
+}
+
+int drape_prolongation_gradient(drape_ctx* p, const float* params, int32_t count, double* loss, float* grad) {
+  Context* c = training_context(p);
+  if (!c) return DRAPE_E_INVALID;
+  return gnn_train_gradient(*c, params, count, loss, grad);
 }
 
 int drape_two_level_weights(drape_ctx* p, float* weights) {

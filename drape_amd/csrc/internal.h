@@ -26,6 +26,8 @@ struct PcgState {
 
 // Two-level preconditioner data (coarse space) -- see twolevel.hip.
 struct TwoLevel;
+// Training buffers of the learned prolongation -- see gnn_train.hip.
+struct GnnTrain;
 
 struct Context {
   // sizes
@@ -107,6 +109,7 @@ struct Context {
 
   bool curvature_dropped = false;
   TwoLevel* two_level = nullptr;
+  GnnTrain* gnn_train = nullptr;  // training buffers of the learned prolongation (allocated on first use)
   uint64_t system_version = 0;    // bumped by every assembly: the coarse operator follows it
   uint64_t shift_version = 0;     // bumped when mu or the step size of M/h^2 changes
   float shift_mu = 0.0f;
@@ -215,6 +218,9 @@ void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host
 int two_level_set_model(Context& c, const float* theta, int count);  // learned prolongation (NULL: none)
+// Factorises the assembled system with the current weights (the network is not run), unconditionally.
+int two_level_factor_weights(Context& c, bool* ok);
+float* two_level_weights_device(Context& c);  // [4R] current weights on the device
 int two_level_weights(Context& c, float* out);         // [4R] weights of the current factorisation
 int two_level_info(const Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
                    int64_t* failures, double* factor_ms);
@@ -234,8 +240,20 @@ struct GnnArgs {
   float *scale, *features, *edge, *h0, *h1, *msg;  // scratch: [R], [13R], [B], [32R] x 3
   float* w;                 // [4R] output weights
 };
+constexpr int GNN_FEATURES = 13, GNN_WIDTH = 32, GNN_LAYERS = 3;
 int gnn_param_count();
 int gnn_infer(const GnnArgs& a);
+int gnn_features(const GnnArgs& a);  // scale, features and normalised edge weights only
+// gnn_infer with every activation kept: hs [4][32R] (h0..h3), ms [3][32R] (the messages of the three
+// layers); the weights are bit-identical to gnn_infer's.
+int gnn_forward_keep(const GnnArgs& a, float* hs, float* ms);
+GnnArgs two_level_gnn_args(Context& c);  // the context's network inputs, scratch and weight buffers
+
+// ---- training of the learned prolongation (gnn_train.hip) -----------------------------------
+void gnn_train_destroy(GnnTrain* g);
+int gnn_train_set_vectors(Context& c, const float* vectors, int count);
+int gnn_train_make_vectors(Context& c, int count, int sweeps, float omega, uint64_t seed, float* out);
+int gnn_train_gradient(Context& c, const float* params, int count, double* loss, float* grad);
 
 // ---- helpers --------------------------------------------------------------------------------
 inline int blocks_for(int n, int per = WG) { return (n + per - 1) / per; }

@@ -17,7 +17,7 @@ namespace drape {
 
 namespace {
 
-constexpr int F_IN = 13, H = 32;
+constexpr int F_IN = GNN_FEATURES, H = GNN_WIDTH;
 
 // s_i = trace(A_ii) / 3
 __global__ void __launch_bounds__(256) k_gnn_scale(const float* __restrict__ sell_val,
@@ -154,17 +154,25 @@ __global__ void __launch_bounds__(256) k_gnn_head(const float* __restrict__ head
 
 int gnn_param_count() { return H * F_IN + H + 3 * (2 * H * H + H) + 4 * H + 4; }
 
-int gnn_infer(const GnnArgs& a) {
+int gnn_features(const GnnArgs& a) {
   hipStream_t s = a.stream;
   const int R = a.R;
   k_gnn_scale<<<blocks_for(R), 256, 0, s>>>(a.sell_val, a.diag_slot, R, a.scale);
   k_gnn_features<<<blocks_for(R), 256, 0, s>>>(a.sell_val, a.block_dst, a.diag_slot, a.row_ptr, a.col, a.scale, a.fxy,
                                                a.wbil, R, a.features, a.edge);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+int gnn_infer(const GnnArgs& a) {
+  hipStream_t s = a.stream;
+  const int R = a.R;
+  TRY_RC(gnn_features(a));
   k_gnn_input<<<blocks_for(R * H), 256, 0, s>>>(a.features, a.theta, R, a.h0);
   const float* layer = a.theta + H * F_IN + H;
   float* h = a.h0;
   float* out = a.h1;
-  for (int l = 0; l < 3; ++l) {
+  for (int l = 0; l < GNN_LAYERS; ++l) {
     k_gnn_message<<<blocks_for(R * H), 256, 0, s>>>(a.row_ptr, a.col, a.edge, h, R, a.msg);
     k_gnn_dense<<<blocks_for(R * H), 256, 0, s>>>(layer, h, a.msg, R, out);
     layer += 2 * H * H + H;
@@ -173,6 +181,23 @@ int gnn_infer(const GnnArgs& a) {
     out = tmp;
   }
   k_gnn_head<<<blocks_for(R), 256, 0, s>>>(layer, h, a.wbil, R, a.w);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+int gnn_forward_keep(const GnnArgs& a, float* hs, float* ms) {
+  hipStream_t s = a.stream;
+  const int R = a.R;
+  const size_t plane = (size_t)H * R;
+  TRY_RC(gnn_features(a));
+  k_gnn_input<<<blocks_for(R * H), 256, 0, s>>>(a.features, a.theta, R, hs);
+  const float* layer = a.theta + H * F_IN + H;
+  for (int l = 0; l < GNN_LAYERS; ++l) {
+    k_gnn_message<<<blocks_for(R * H), 256, 0, s>>>(a.row_ptr, a.col, a.edge, hs + l * plane, R, ms + l * plane);
+    k_gnn_dense<<<blocks_for(R * H), 256, 0, s>>>(layer, hs + l * plane, ms + l * plane, R, hs + (l + 1) * plane);
+    layer += 2 * H * H + H;
+  }
+  k_gnn_head<<<blocks_for(R), 256, 0, s>>>(layer, hs + GNN_LAYERS * plane, a.wbil, R, a.w);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }

@@ -13,74 +13,9 @@
 // iteration is three launches (spmv+dot, update+dots, direction).
 #include "../internal.h"
 #include "reduce.hpp"
+#include "sell.hpp"
 
 namespace drape {
-
-struct SellView {
-  const uint32_t* slice_ptr;  // per physical slice: first block slot
-  const uint32_t* col;        // [slots * 64] logical column of (slot, lane)
-  const float* val;           // [slots * 9 * 64] entry e of (slot q, lane l) at (9q + e) * 64 + l
-  const uint32_t* cnt;        // blocks of the row at each physical position (0 past the last row)
-  const uint32_t* perm;       // physical position -> logical row (same 256-row group); ~0u past the last row
-};
-
-// Blocks of a row processed per chunk: every load of a chunk is issued before the first use, so a row
-// costs one column-load and one gather latency per chunk instead of one per block.
-constexpr int CHUNK = 8;
-
-// acc[r] = acc[r] + (v[3r] p0 + v[3r+1] p1 + v[3r+2] p2) over the row's blocks in column order (the
-// reference's per-row order, pcg.rs:359-369): contributions are formed independently, the sum stays
-// sequential. `dir` selects the fused direction update: the gathered value is z[c] + beta p[c].
-template <bool DIR>
-__device__ inline float3 row_times(const SellView& A, int phys, const float4* __restrict__ p,
-                                   const float4* __restrict__ z, float beta) {
-  const int s = phys >> 6, l = phys & 63;
-  const uint32_t base = A.slice_ptr[s];
-  const uint32_t n = A.cnt[phys];
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (uint32_t k0 = 0; k0 < n; k0 += CHUNK) {
-    uint32_t c[CHUNK];
-    float v[CHUNK][9];
-    float4 pc[CHUNK], zc[CHUNK];
-#pragma unroll
-    for (int j = 0; j < CHUNK; ++j) {
-      if (k0 + j < n) {
-        const uint32_t q = base + k0 + j;
-        c[j] = A.col[q * 64 + l];
-        const float* vp = A.val + (size_t)q * 9 * 64 + l;
-#pragma unroll
-        for (int e = 0; e < 9; ++e) v[j][e] = vp[64 * e];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < CHUNK; ++j) {
-      if (k0 + j < n) {
-        pc[j] = p[c[j]];
-        if (DIR) zc[j] = z[c[j]];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < CHUNK; ++j) {
-      if (k0 + j < n) {
-        float4 w = pc[j];
-        if (DIR) {
-          w.x = zc[j].x + beta * w.x;
-          w.y = zc[j].y + beta * w.y;
-          w.z = zc[j].z + beta * w.z;
-        }
-        a0 = a0 + (v[j][0] * w.x + v[j][1] * w.y + v[j][2] * w.z);
-        a1 = a1 + (v[j][3] * w.x + v[j][4] * w.y + v[j][5] * w.z);
-        a2 = a2 + (v[j][6] * w.x + v[j][7] * w.y + v[j][8] * w.z);
-      }
-    }
-  }
-  return make_float3(a0, a1, a2);
-}
-
-__device__ inline float3 precondition(const float* __restrict__ dinv, int row, float a, float b, float c) {
-  const float* d = dinv + 9 * row;
-  return make_float3(d[0] * a + d[1] * b + d[2] * c, d[3] * a + d[4] * b + d[5] * c, d[6] * a + d[7] * b + d[8] * c);
-}
 
 // Group tree of per-row values held by physical position: the value goes to its logical slot first.
 __device__ inline float tree_logical(float v, uint32_t row, float* lds) {
@@ -100,17 +35,6 @@ __device__ inline float2 tree_logical2(float a, float b, uint32_t row, float2* l
   return tree256x2(own.x, own.y, lds);
 }
 
-// pcg.rs:268-281 invert3 (adjugate / determinant), same operation order.
-__device__ inline void invert3(const float m[9], float* out) {
-  float a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7], i = m[8];
-  float c00 = e * i - f * h, c01 = c * h - b * i, c02 = b * f - c * e;
-  float c10 = f * g - d * i, c11 = a * i - c * g, c12 = c * d - a * f;
-  float c20 = d * h - e * g, c21 = b * g - a * h, c22 = a * e - b * d;
-  float det = a * c00 + b * c10 + c * c20;
-  out[0] = c00 / det; out[1] = c01 / det; out[2] = c02 / det;
-  out[3] = c10 / det; out[4] = c11 / det; out[5] = c12 / det;
-  out[6] = c20 / det; out[7] = c21 / det; out[8] = c22 / det;
-}
 
 // jacobi_invert + cg_init: D^-1, x = 0, r = b, z = D^-1 r, p = z, partial (r.z, r.r).
 __global__ void __launch_bounds__(256) k_pcg_setup(SellView A, const uint32_t* __restrict__ diag_slot,

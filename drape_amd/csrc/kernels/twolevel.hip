@@ -986,7 +986,7 @@ int two_level_weights(Context& c, float* out) {
   return DRAPE_OK;
 }
 
-static int run_model(Context& c) {
+GnnArgs two_level_gnn_args(Context& c) {
   TwoLevel* t = c.two_level;
   GnnArgs a;
   a.stream = c.stream;
@@ -1006,15 +1006,22 @@ static int run_model(Context& c) {
   a.h1 = a.h0 + 32 * (size_t)c.R;
   a.msg = a.h1 + 32 * (size_t)c.R;
   a.w = t->d_w;
-  return gnn_infer(a);
+  return a;
 }
+
+static int run_model(Context& c) { return gnn_infer(two_level_gnn_args(c)); }
+
+static int factorise(Context& c, bool model, bool* ok);
+
+int two_level_factor_weights(Context& c, bool* ok) { return factorise(c, false, ok); }
+
+float* two_level_weights_device(Context& c) { return c.two_level->d_w; }
 
 // A_c = P^T A P of the assembled system, its Cholesky factor and the factor's inverse, unless the current
 // factorisation may serve this system (same system, or coarse_reuse with unchanged mu and step size and no
 // slowdown). *ok is false when A_c is not positive definite.
 int two_level_prepare(Context& c, bool* ok) {
   TwoLevel* t = c.two_level;
-  hipStream_t s = c.stream;
   const bool fresh = t->factored == c.system_version;
   const bool reusable = c.rules.coarse_reuse && t->factored != ~0ull && t->ok && !t->refresh &&
                         (c.rules.coarse_reuse == 2 || t->factored_shift == c.shift_version ||
@@ -1023,6 +1030,14 @@ int two_level_prepare(Context& c, bool* ok) {
     *ok = t->ok;
     return DRAPE_OK;
   }
+  return factorise(c, true, ok);
+}
+
+// The factorisation itself for the assembled system (with the network's weights first when `model` and a
+// network is set, else with the current weights); it becomes the current one.
+static int factorise(Context& c, bool model, bool* ok) {
+  TwoLevel* t = c.two_level;
+  hipStream_t s = c.stream;
   t->factored = c.system_version;
   t->factored_shift = c.shift_version;
   t->factored_h = c.shift_h;
@@ -1036,7 +1051,7 @@ int two_level_prepare(Context& c, bool* ok) {
     std::chrono::steady_clock::time_point t0;
     ~Clock() { t->factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
   } clock{t, started};
-  if (t->d_theta) TRY_RC(run_model(c));
+  if (model && t->d_theta) TRY_RC(run_model(c));
   HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * TILE * TILE * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
   k_galerkin_band<<<blocks_for(9 * t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,

@@ -291,6 +291,22 @@ int drape_set_prolongation_model(drape_ctx* ctx, const float* params, int32_t co
 /* The parent-slot weights [4R] of the current coarse factorisation. */
 int drape_two_level_weights(drape_ctx* ctx, float* weights);
 
+/* Training of the learned prolongation on the device (no reference counterpart; the reference has no
+ * two-level preconditioner). The error vectors the loss is taken over, kept on the device:
+ * drape_training_vectors_set copies `count` vectors [count][3R] (row-major per vector);
+ * drape_training_vectors_make makes them from uniform random starts in [-1, 1) (seeded) by `sweeps` damped
+ * block-Jacobi sweeps e <- e - omega D^-1 A e on the assembled system, and copies them to `out`
+ * [count][3R] unless it is NULL. */
+int drape_training_vectors_set(drape_ctx* ctx, const float* vectors, int32_t count);
+int drape_training_vectors_make(drape_ctx* ctx, int32_t count, int32_t sweeps, float omega, uint64_t seed,
+                                float* out);
+/* The projection loss L = mean_e |e - P c_e|_A^2 / |e|_A^2 (c_e = A_c^-1 P^T A e, A_c = P^T A P, P the
+ * prolongation the network at `params` gives on the assembled system) into *loss and dL/dparams [count]
+ * into `grad`, by the network's forward and backward kernels; the coarse solve is the preconditioner's own
+ * factorisation. `params` becomes the context's prolongation model (as drape_set_prolongation_model).
+ * DRAPE_E_INVALID when no vectors are set, `count` is not the model's or A_c is not positive definite. */
+int drape_prolongation_gradient(drape_ctx* ctx, const float* params, int32_t count, double* loss, float* grad);
+
 /* The Galerkin coarse operator A_c = P^T A P of the assembled system, row-major [3Nc x 3Nc] doubles (Nc from
  * drape_two_level_layout), for checking; the preconditioner's factorisation is left as it is. */
 int drape_two_level_coarse(drape_ctx* ctx, double* coarse);
