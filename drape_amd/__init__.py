@@ -51,7 +51,8 @@ class Rules(C.Structure):
                 ("floor_stop", C.c_int32), ("forcing", C.c_int32), ("regularize", C.c_int32),
                 ("noise_aware", C.c_int32), ("exact_tangent", C.c_int32), ("refine_fp64", C.c_int32),
                 ("preconditioner", C.c_int32), ("coarse_reuse", C.c_int32),
-                ("fallback", C.c_int32), ("coarse_shift", C.c_double), ("smoother", C.c_double)]
+                ("fallback", C.c_int32), ("coarse_shift", C.c_double), ("smoother", C.c_double),
+                ("coarse_half", C.c_int32)]
 
 
 class PcgStats(C.Structure):

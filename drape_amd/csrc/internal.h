@@ -213,7 +213,7 @@ struct CoarseView {
   const float* yc;        // [3Nc] the coarse solution
 };
 CoarseView two_level_view(const Context& c);
-int two_level_restrict_solve(Context& c, const float4* r, int guard);
+int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32 = false);  // fp32: FP32 B always
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host

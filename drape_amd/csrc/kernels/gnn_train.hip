@@ -9,7 +9,8 @@
 // Galerkin orthogonality, P^T A (e - P c_e) = 0, removes the derivative through c_e:
 //   dL/dw_ik = -(2/n) sum_e (A eps_e)_i . c_e[parent(i, k)] / |e|_A^2,   eps_e = e - P c_e.
 // A_c^-1 is the preconditioner's own factorisation (FP64 banded Cholesky, applied as B^T B with the FP32
-// inverse factor B = L^-1); the products with A are the solver's SELL-64 rows.
+// inverse factor B = L^-1, also where the preconditioner applies the FP16 copy); the products with A are the
+// solver's SELL-64 rows.
 //
 // Error vectors: given by the caller, or made on the device from uniform random starts in [-1, 1) (SplitMix64
 // of the seed and the entry index, Steele, Lea and Flood 2014) by damped block-Jacobi sweeps
@@ -487,7 +488,7 @@ int gnn_train_gradient(Context& c, const float* params, int count, double* loss,
   for (int e = 0; e < n; ++e) {
     const float4* E = g->d_E + (size_t)e * R;
     k_spmv_dot<<<c.G, 256, 0, s>>>(A, E, g->d_q, g->d_partial);
-    TRY_RC(two_level_restrict_solve(c, g->d_q, 2));
+    TRY_RC(two_level_restrict_solve(c, g->d_q, 2, true));
     k_eps<<<blocks_for(R), 256, 0, s>>>(cv.parent, cv.w, cv.yc, E, g->d_eps, R);
     k_spmv_dot<<<c.G, 256, 0, s>>>(A, g->d_eps, g->d_q, g->d_partial + c.G);
     k_sum_pair<<<1, 256, 0, s>>>(g->d_partial, c.G, g->d_sums + 2 * e);

@@ -68,6 +68,8 @@ struct TwoLevel {
   double* d_Ac = nullptr;           // [N3 * N3] dense A_c for the check entry point (allocated on demand)
   float* d_B = nullptr;             // [N3 * N3p] row-major B = L^-1 (zero above the diagonal)
   float* d_BT = nullptr;            // [N3 * N3p] row-major B^T (zero below the diagonal)
+  _Float16* d_Bh = nullptr;         // [N3 * N3p] FP16 B~ = S^-1 B, S = diag(B) (zero above the diagonal)
+  _Float16* d_BTh = nullptr;        // [N3 * N3p] its transpose (the same values)
   float* d_t = nullptr;             // [N3p] B r_c
   float* d_rc = nullptr;            // [N3p]
   float* d_yc = nullptr;            // [N3p]
@@ -81,7 +83,7 @@ void two_level_destroy(TwoLevel* t) {
   if (!t) return;
   for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_wbil, (void*)t->d_fxy, (void*)t->d_theta,
                   (void*)t->d_gnn, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
-                  (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT,
+                  (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT, (void*)t->d_Bh, (void*)t->d_BTh,
                   (void*)t->d_t, (void*)t->d_rc, (void*)t->d_yc, (void*)t->d_info})
     if (p) (void)hipFree(p);
   if (t->h_info) (void)hipHostFree(t->h_info);
@@ -421,11 +423,14 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ band, 
 // tiles of X kept in LDS. The factor tiles stream through one LDS tile buffer, the next two prefetched into
 // registers while the current one is used. Each product splits its 64-long sums over the four wavefronts
 // (16 terms each, a 4 x 4 block of outputs per lane), and the partial sums meet in LDS in a fixed order.
-// FP32 B (row-major) and B^T (row-major) receive the rows and columns below N3.
+// FP32 B (row-major) and B^T (row-major) receive the rows and columns below N3, and so do the FP16 copies
+// B~ = S^-1 B and its transpose, S = diag(B) = the reciprocal factor diagonal (the diagonal of linv), with
+// |B~| clamped to the FP16 range.
 constexpr int INV_COLS = 16;
 __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__ lpanel,
                                                       const double* __restrict__ linv, int nt, int kb, int N3,
                                                       int N3p, float* __restrict__ B, float* __restrict__ BT,
+                                                      _Float16* __restrict__ Bh, _Float16* __restrict__ BTh,
                                                       const int* info) {
   __shared__ double Xh[(MAX_KB + 1) * TILE * INV_COLS];
   __shared__ double T[TILE * INV_COLS];
@@ -545,6 +550,10 @@ __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__
         if (R < N3 && C < N3) {
           B[(size_t)R * N3p + C] = (float)xv;
           BT[(size_t)C * N3p + R] = (float)xv;
+          const double sr = linv[((size_t)(R >> 6) << 12) + (R & 63) * 65];
+          const _Float16 h = (_Float16)fmin(fmax(xv / sr, -65504.0), 65504.0);
+          Bh[(size_t)R * N3p + C] = h;
+          BTh[(size_t)C * N3p + R] = h;
         }
       }
 #pragma unroll
@@ -650,6 +659,59 @@ __global__ void __launch_bounds__(256) k_tri_gemv(const float* __restrict__ T, c
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) acc = acc + __shfl_down(acc, s, 64);
   if (lane == 0) y[row] = acc;
+}
+
+// The FP16 form of k_tri_gemv: T holds B~ (LOWER) or its transpose; eight halves per lane and step, FP32
+// products and sums in the same order as k_tri_gemv. LOWER also applies S^2 (the square of the row's
+// diagonal of B, read from linv), so that the pair computes (S B~)^T (S B~) x.
+template <bool LOWER>
+__global__ void __launch_bounds__(256) k_tri_gemv_h(const _Float16* __restrict__ T, const float* __restrict__ x,
+                                                    const double* __restrict__ linv, int N3, int N3p,
+                                                    float* __restrict__ y, const PcgState* st, int guard) {
+  if (guard == 0 && st->status != 0) return;
+  if (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= N3) return;
+  const uint4* m = reinterpret_cast<const uint4*>(T + (size_t)row * N3p);
+  const float4* v = reinterpret_cast<const float4*>(x);
+  const int lo = LOWER ? 0 : row / 8, hi = LOWER ? row / 8 + 1 : N3p / 8;
+  float acc = 0.0f;
+  for (int j0 = lo + lane; j0 < hi; j0 += 4 * 64) {
+    uint4 a[4];
+    float4 b0[4], b1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + 64 * u;
+      if (j < hi) {
+        a[u] = m[j];
+        b0[u] = v[2 * j];
+        b1[u] = v[2 * j + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (j0 + 64 * u < hi) {
+        union {
+          uint4 q;
+          _Float16 h[8];
+        } w;
+        w.q = a[u];
+        acc = acc + ((float)w.h[0] * b0[u].x + (float)w.h[1] * b0[u].y + (float)w.h[2] * b0[u].z +
+                     (float)w.h[3] * b0[u].w);
+        acc = acc + ((float)w.h[4] * b1[u].x + (float)w.h[5] * b1[u].y + (float)w.h[6] * b1[u].z +
+                     (float)w.h[7] * b1[u].w);
+      }
+  }
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) acc = acc + __shfl_down(acc, s, 64);
+  if (lane == 0) {
+    if (LOWER) {
+      const float sr = (float)linv[((size_t)(row >> 6) << 12) + (row & 63) * 65];
+      acc = acc * (sr * sr);
+    }
+    y[row] = acc;
+  }
 }
 
 // z = D^-1 r + sum_k w_ik y_c[parent(i, k)], partial (r.z, r.r); mode 2 (setup) also sets p = z.
@@ -907,6 +969,8 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_lpanel, (size_t)t->nt * std::max(t->kb, 1) * TILE * TILE);
   TL_ALLOC(t->d_B, (size_t)t->N3 * t->N3p);
   TL_ALLOC(t->d_BT, (size_t)t->N3 * t->N3p);
+  TL_ALLOC(t->d_Bh, (size_t)t->N3 * t->N3p);
+  TL_ALLOC(t->d_BTh, (size_t)t->N3 * t->N3p);
   TL_ALLOC(t->d_t, (size_t)t->N3p);
   TL_ALLOC(t->d_rc, (size_t)t->N3p);
   TL_ALLOC(t->d_yc, (size_t)t->N3p);
@@ -930,6 +994,8 @@ int two_level_create(Context& c, int columns) {
       (e = hipMemsetAsync(t->d_t, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_B, 0, (size_t)t->N3 * t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_BT, 0, (size_t)t->N3 * t->N3p * 4, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_Bh, 0, (size_t)t->N3 * t->N3p * 2, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_BTh, 0, (size_t)t->N3 * t->N3p * 2, s)) != hipSuccess ||
       (e = hipStreamSynchronize(s)) != hipSuccess)
     return fail(e);
   if (c.two_level) two_level_destroy(c.two_level);
@@ -1066,7 +1132,7 @@ static int factorise(Context& c, bool model, bool* ok) {
     }
   }
   k_band_inverse<<<t->nt * (TILE / INV_COLS), 256, 0, s>>>(t->d_lpanel, t->d_linv, t->nt, t->kb, t->N3, t->N3p, t->d_B,
-                                                           t->d_BT, t->d_info);
+                                                           t->d_BT, t->d_Bh, t->d_BTh, t->d_info);
   HIP_TRY(hipGetLastError());
   TRY_RC(read_back(c, t->h_info, t->d_info, sizeof(int)));
   t->factorisations++;
@@ -1115,13 +1181,25 @@ void two_level_observe(Context& c, int iterations, double relative_residual) {
   else if (rate > 2.0 * t->base_rate + 4.0) t->refresh = true;
 }
 
+// y_c = B^T B r_c: from the FP16 scaled copies under rules.coarse_half unless `fp32`, else from FP32 B.
+static void coarse_solve(Context& c, const TwoLevel* t, int guard, bool fp32) {
+  hipStream_t s = c.stream;
+  const int blocks = (t->N3 + 3) / 4;
+  if (c.rules.coarse_half && !fp32) {
+    k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, t->N3, t->N3p, t->d_t, c.state, guard);
+    k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, t->N3, t->N3p, t->d_yc, c.state, guard);
+  } else {
+    k_tri_gemv<true><<<blocks, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, guard);
+    k_tri_gemv<false><<<blocks, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, guard);
+  }
+}
+
 // z = D^-1 r + P B^T B P^T r for the current r (guard: 0 running, 1 replacement gate, 2 setup with p = z).
 int two_level_apply(Context& c, float* partials, int guard) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
   k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, c.r, t->Nc, t->d_rc, c.state, guard);
-  k_tri_gemv<true><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, guard);
-  k_tri_gemv<false><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, guard);
+  coarse_solve(c, t, guard, false);
   k_prolong<<<c.G, 256, 0, s>>>(t->d_parent, t->d_w, t->d_yc, c.dinv, c.r, c.z, c.p, partials, c.R, c.state, guard);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
@@ -1132,8 +1210,7 @@ int two_level_iterate(Context& c, const float* partial_pq, const float4* p, floa
   hipStream_t s = c.stream;
   k_restrict_update<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, c.r, c.q, t->Nc, t->d_rc,
                                                     partial_pq, c.G, c.state, it);
-  k_tri_gemv<true><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, 0);
-  k_tri_gemv<false><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, 0);
+  coarse_solve(c, t, 0, false);
   k_prolong_update<<<c.G, 256, 0, s>>>(t->d_parent, t->d_w, t->d_yc, c.dinv, p, c.q, c.x, c.r, c.z, partial_rz, c.R,
                                        partial_pq, c.G, c.state, it);
   HIP_TRY(hipGetLastError());
@@ -1145,12 +1222,11 @@ CoarseView two_level_view(const Context& c) {
   return {t->d_parent, t->d_w, t->d_yc};
 }
 
-int two_level_restrict_solve(Context& c, const float4* r, int guard) {
+int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
   k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, r, t->Nc, t->d_rc, c.state, guard);
-  k_tri_gemv<true><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, guard);
-  k_tri_gemv<false><<<(t->N3 + 3) / 4, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, guard);
+  coarse_solve(c, t, guard, fp32);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }

@@ -92,6 +92,10 @@ typedef struct {
                              multiplicative cycle z1 = w D^-1 r, z2 = z1 + P A_c^-1 P^T (r - A z1),
                              z = z2 + w D^-1 (r - A z2), which is positive definite while
                              w lambda_max(D^-1 A) < 2 */
+  int32_t coarse_half;    /* two-level: 1 applies the coarse inverse from FP16 storage, B~ = fp16(S^-1 B) with
+                             S = diag(B) (the reciprocal factor diagonal), as (S B~)^T (S B~): half the bytes of
+                             the FP32 copy and still exactly a Gram form, so symmetric positive definite;
+                             0 (default) applies the FP32 B = L^-1 */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0

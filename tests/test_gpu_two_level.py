@@ -83,13 +83,15 @@ def test_coarse_inverse_factor_matches_f64_cholesky(n):
     ctx.close()
 
 
+@pytest.mark.parametrize("half", [0, 1])
 @pytest.mark.parametrize("n,h,tol", [(65, 0.1, 1e-2), (65, 0.03, 1e-3)])
-def test_two_level_pcg_matches_f64_two_level(n, h, tol):
+def test_two_level_pcg_matches_f64_two_level(n, h, tol, half):
     """On the pinned n65 sheet at step sizes where the stiffness dominates the mass term, the GPU's FP32
     two-level PCG takes the f64 two-level iteration count within 15% (+3) and reaches the tolerance in the
     true residual (1.5x: FP32 round-off near the tolerance), with at least 2x fewer iterations than
-    block-Jacobi on the same system."""
-    arrays, ctx = pinned_system(n, h)
+    block-Jacobi on the same system; with the coarse inverse applied from FP32 and from the scaled FP16
+    copies (rules.coarse_half)."""
+    arrays, ctx = pinned_system(n, h, D.default_rules(coarse_half=half))
     rhs, vals = ctx.system()
     A = TL.bsr_matrix(arrays.array("row_ptr"), arrays.array("col_idx"), vals.astype(np.float64))
     b = rhs.astype(np.float64)
@@ -99,7 +101,7 @@ def test_two_level_pcg_matches_f64_two_level(n, h, tol):
     st, x = ctx.solve_once(tol, 20000)
     _, oit, ost, _ = TL.two_level_pcg(A, b, TL.prolongation(parent, w.astype(np.float64), nc), tol, 20000)
     true_rel = np.linalg.norm(b - A @ x.astype(np.float64)) / np.linalg.norm(b)
-    print(f"[gpu] two-level PCG n{n} h={h} tol={tol}: {st.iterations} iterations (f64 two-level {oit}, GPU "
+    print(f"[gpu] two-level PCG n{n} h={h} tol={tol} half={half}: {st.iterations} iterations (f64 two-level {oit}, GPU "
           f"block-Jacobi {st_bj.iterations}); true relative residual {true_rel:.2e}", flush=True)
     assert D.PCG_STATUS[st.status] == "converged" and ost == 1
     assert abs(st.iterations - oit) <= max(3, 0.15 * oit)
