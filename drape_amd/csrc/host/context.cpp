@@ -319,7 +319,7 @@ void drape_default_rules(drape_rules* r) {
   r->fallback = 2;
   r->coarse_shift = 0.0;
   r->smoother = 0.0;
-  r->coarse_half = 0;
+  r->coarse_half = 2;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }

@@ -1181,11 +1181,15 @@ void two_level_observe(Context& c, int iterations, double relative_residual) {
   else if (rate > 2.0 * t->base_rate + 4.0) t->refresh = true;
 }
 
-// y_c = B^T B r_c: from the FP16 scaled copies under rules.coarse_half unless `fp32`, else from FP32 B.
+// y_c = B^T B r_c: from the FP16 scaled copies when rules.coarse_half selects them and not `fp32`, else from
+// FP32 B.
 static void coarse_solve(Context& c, const TwoLevel* t, int guard, bool fp32) {
   hipStream_t s = c.stream;
   const int blocks = (t->N3 + 3) / 4;
-  if (c.rules.coarse_half && !fp32) {
+  // auto (2): FP16 once the FP32 pair B, B^T outgrows half of the 256 MB Infinity Cache
+  const bool big = (size_t)8 * t->N3 * t->N3p > ((size_t)128 << 20);
+  const bool half = c.rules.coarse_half == 1 || (c.rules.coarse_half == 2 && big);
+  if (half && !fp32) {
     k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, t->N3, t->N3p, t->d_t, c.state, guard);
     k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, t->N3, t->N3p, t->d_yc, c.state, guard);
   } else {

@@ -95,7 +95,8 @@ typedef struct {
   int32_t coarse_half;    /* two-level: 1 applies the coarse inverse from FP16 storage, B~ = fp16(S^-1 B) with
                              S = diag(B) (the reciprocal factor diagonal), as (S B~)^T (S B~): half the bytes of
                              the FP32 copy and still exactly a Gram form, so symmetric positive definite;
-                             0 (default) applies the FP32 B = L^-1 */
+                             0 applies the FP32 B = L^-1; 2 (default) chooses FP16 when the FP32 B and B^T
+                             together exceed 128 MiB (3 Nc above about 4,000: n257 and larger) */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0
