@@ -70,6 +70,7 @@ struct TwoLevel {
   float* d_BT = nullptr;            // [N3 * N3p] row-major B^T (zero below the diagonal)
   _Float16* d_Bh = nullptr;         // [N3 * N3p] FP16 B~ = S^-1 B, S = diag(B) (zero above the diagonal)
   _Float16* d_BTh = nullptr;        // [N3 * N3p] its transpose (the same values)
+  bool half_valid = false;          // the FP16 copies belong to the current factorisation
   float* d_t = nullptr;             // [N3p] B r_c
   float* d_rc = nullptr;            // [N3p]
   float* d_yc = nullptr;            // [N3p]
@@ -424,8 +425,8 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ band, 
 // registers while the current one is used. Each product splits its 64-long sums over the four wavefronts
 // (16 terms each, a 4 x 4 block of outputs per lane), and the partial sums meet in LDS in a fixed order.
 // FP32 B (row-major) and B^T (row-major) receive the rows and columns below N3, and so do the FP16 copies
-// B~ = S^-1 B and its transpose, S = diag(B) = the reciprocal factor diagonal (the diagonal of linv), with
-// |B~| clamped to the FP16 range.
+// B~ = S^-1 B and its transpose when Bh is not null, S = diag(B) = the reciprocal factor diagonal (the
+// diagonal of linv), with |B~| clamped to the FP16 range.
 constexpr int INV_COLS = 16;
 __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__ lpanel,
                                                       const double* __restrict__ linv, int nt, int kb, int N3,
@@ -550,10 +551,12 @@ __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__
         if (R < N3 && C < N3) {
           B[(size_t)R * N3p + C] = (float)xv;
           BT[(size_t)C * N3p + R] = (float)xv;
-          const double sr = linv[((size_t)(R >> 6) << 12) + (R & 63) * 65];
-          const _Float16 h = (_Float16)fmin(fmax(xv / sr, -65504.0), 65504.0);
-          Bh[(size_t)R * N3p + C] = h;
-          BTh[(size_t)C * N3p + R] = h;
+          if (Bh) {
+            const double sr = linv[((size_t)(R >> 6) << 12) + (R & 63) * 65];
+            const _Float16 h = (_Float16)fmin(fmax(xv / sr, -65504.0), 65504.0);
+            Bh[(size_t)R * N3p + C] = h;
+            BTh[(size_t)C * N3p + R] = h;
+          }
         }
       }
 #pragma unroll
@@ -1077,6 +1080,13 @@ GnnArgs two_level_gnn_args(Context& c) {
 
 static int run_model(Context& c) { return gnn_infer(two_level_gnn_args(c)); }
 
+// The FP16 copies are formed and applied under rules.coarse_half = 1, and under 2 (auto) once the FP32 pair
+// B, B^T outgrows half of the 256 MB Infinity Cache.
+static bool use_half(const Context& c, const TwoLevel* t) {
+  const bool big = (size_t)8 * t->N3 * t->N3p > ((size_t)128 << 20);
+  return c.rules.coarse_half == 1 || (c.rules.coarse_half == 2 && big);
+}
+
 static int factorise(Context& c, bool model, bool* ok);
 
 int two_level_factor_weights(Context& c, bool* ok) { return factorise(c, false, ok); }
@@ -1131,8 +1141,10 @@ static int factorise(Context& c, bool model, bool* ok) {
       k_chol_update<<<4 * updates, 256, 0, s>>>(t->d_band, t->d_lpanel, k, t->nt, t->kb, t->d_info);
     }
   }
+  t->half_valid = use_half(c, t);
   k_band_inverse<<<t->nt * (TILE / INV_COLS), 256, 0, s>>>(t->d_lpanel, t->d_linv, t->nt, t->kb, t->N3, t->N3p, t->d_B,
-                                                           t->d_BT, t->d_Bh, t->d_BTh, t->d_info);
+                                                           t->d_BT, t->half_valid ? t->d_Bh : nullptr,
+                                                           t->half_valid ? t->d_BTh : nullptr, t->d_info);
   HIP_TRY(hipGetLastError());
   TRY_RC(read_back(c, t->h_info, t->d_info, sizeof(int)));
   t->factorisations++;
@@ -1181,15 +1193,12 @@ void two_level_observe(Context& c, int iterations, double relative_residual) {
   else if (rate > 2.0 * t->base_rate + 4.0) t->refresh = true;
 }
 
-// y_c = B^T B r_c: from the FP16 scaled copies when rules.coarse_half selects them and not `fp32`, else from
-// FP32 B.
+// y_c = B^T B r_c: from the FP16 scaled copies when they are selected and were formed with the current
+// factorisation (and not `fp32`), else from FP32 B.
 static void coarse_solve(Context& c, const TwoLevel* t, int guard, bool fp32) {
   hipStream_t s = c.stream;
   const int blocks = (t->N3 + 3) / 4;
-  // auto (2): FP16 once the FP32 pair B, B^T outgrows half of the 256 MB Infinity Cache
-  const bool big = (size_t)8 * t->N3 * t->N3p > ((size_t)128 << 20);
-  const bool half = c.rules.coarse_half == 1 || (c.rules.coarse_half == 2 && big);
-  if (half && !fp32) {
+  if (t->half_valid && use_half(c, t) && !fp32) {
     k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, t->N3, t->N3p, t->d_t, c.state, guard);
     k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, t->N3, t->N3p, t->d_yc, c.state, guard);
   } else {
