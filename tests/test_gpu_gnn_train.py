@@ -125,3 +125,28 @@ def test_device_training_gain():
         assert 5 * sum(learned) <= sum(bj)
     for s in train + held:
         s.close()
+
+
+def test_training_entry_points_reject_bad_calls():
+    """The error paths of the training entry points: a gradient before any vectors, a parameter count that is
+    not the model's, a vector count below one, and a context without a mesh each return DRAPE_E_INVALID with
+    a message, and the context stays usable."""
+    ctx, _ = assembled(33)
+    with pytest.raises(D.DrapeError, match="no training vectors"):
+        ctx.prolongation_gradient(random_params(5))
+    ctx.training_vectors(count=2, sweeps=1, seed=1)
+    with pytest.raises(D.DrapeError, match="parameters"):
+        ctx.prolongation_gradient(np.zeros(G.param_count() - 1, np.float32))
+    with pytest.raises(D.DrapeError):
+        ctx.training_vectors(np.zeros((0, 3 * ctx.R), np.float32))
+    loss, grad = ctx.prolongation_gradient(random_params(5))
+    assert np.isfinite(loss) and np.all(np.isfinite(grad))
+    ctx.close()
+    import oracle_lib as O
+
+    c = O.Case("dynamic", 33)
+    lin = D.LinearSystem(c.row_ptr.astype(np.uint32), c.col_idx.astype(np.uint32), c.vals.astype(np.float32),
+                         c.rhs.astype(np.float32))
+    rc = D.lib().drape_training_vectors_make(lin.h, 2, 1, 0.6, 0, None)
+    assert rc == -1 and b"mesh" in D.lib().drape_last_error()  # DRAPE_E_INVALID
+    lin.close()
