@@ -52,7 +52,7 @@ class Rules(C.Structure):
                 ("noise_aware", C.c_int32), ("exact_tangent", C.c_int32), ("refine_fp64", C.c_int32),
                 ("preconditioner", C.c_int32), ("coarse_reuse", C.c_int32),
                 ("fallback", C.c_int32), ("coarse_shift", C.c_double), ("smoother", C.c_double),
-                ("coarse_half", C.c_int32)]
+                ("coarse_half", C.c_int32), ("self_contact", C.c_int32)]
 
 
 class PcgStats(C.Structure):
@@ -152,6 +152,7 @@ def lib():
         L.drape_copy_timing.argtypes = [C.c_int64, C.c_int32, f64p]
         L.drape_set_prolongation_model.argtypes = [vp, f32p, C.c_int32]
         L.drape_two_level_weights.argtypes = [vp, f32p]
+        L.drape_self_contact_pairs.argtypes = [vp, i32p, i32p, i32p, f64p]
         L.drape_training_vectors_set.argtypes = [vp, f32p, C.c_int32]
         L.drape_training_vectors_make.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, f32p]
         L.drape_prolongation_gradient.argtypes = [vp, f32p, C.c_int32, f64p, f32p]
@@ -476,6 +477,17 @@ class Drape:
             return
         theta = _f32(params)
         _check(lib().drape_set_prolongation_model(self.h, _ptr(theta, f32p), theta.size))
+
+    def self_contact_pairs(self):
+        """Self-contact pairs of the last evaluation: (point-triangle count, vertices [n, 4] int32, FP64
+        distances [n]); the point-triangle pairs come first."""
+        npt, nee = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        _check(lib().drape_self_contact_pairs(self.h, _ptr(npt, i32p), _ptr(nee, i32p), None, None))
+        n = int(npt[0] + nee[0])
+        verts = np.zeros((max(n, 1), 4), np.int32)
+        dist = np.zeros(max(n, 1), np.float64)
+        _check(lib().drape_self_contact_pairs(self.h, None, None, _ptr(verts, i32p), _ptr(dist, f64p)))
+        return int(npt[0]), verts[:n], dist[:n]
 
     def training_vectors(self, vectors=None, count=16, sweeps=5, omega=0.6, seed=0):
         """The error vectors of the prolongation loss, kept on the device: `vectors` [count, 3R] when given,

@@ -14,6 +14,7 @@ SOURCES = [
     "kernels/twolevel.hip",
     "kernels/gnn.hip",
     "kernels/gnn_train.hip",
+    "kernels/selfcontact.hip",
     "kernels/prolong.hip",
     "kernels/builder.hip",
     "host/context.cpp",

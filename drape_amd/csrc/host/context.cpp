@@ -48,6 +48,7 @@ static void free_context(Context* c) {
   if (c->dout_host) (void)hipHostFree(c->dout_host);
   if (c->two_level) two_level_destroy(c->two_level);
   if (c->gnn_train) gnn_train_destroy(c->gnn_train);
+  if (c->self_contact) self_contact_destroy(c->self_contact);
   for (hipEvent_t e : {c->ev0, c->ev1, c->ev_spmv0, c->ev_spmv1, c->ev_poll[0], c->ev_poll[1]})
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -320,6 +321,7 @@ void drape_default_rules(drape_rules* r) {
   r->coarse_shift = 0.0;
   r->smoother = 0.0;
   r->coarse_half = 2;
+  r->self_contact = 0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }
@@ -409,6 +411,7 @@ int drape_set_state(drape_ctx* p, const float* positions, const float* x_hat, do
   Context* c = ctx_of(p);
   if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
   if (positions) TRY(upload_positions(*c, c->pos, positions));
+  if (positions) self_contact_moved(*c);
   if (x_hat) TRY(upload_positions(*c, c->xhat, x_hat));
   if (h > 0) {
     for (int row = 0; row < c->R; ++row) c->row_mass_host[row] = (float)((double)c->unit_mass[row] / (h * h));
@@ -654,6 +657,12 @@ int drape_set_prolongation_model(drape_ctx* p, const float* params, int32_t coun
   if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
   if (!c->two_level) TRY(two_level_create(*c, c->problem.grid_columns));
   return two_level_set_model(*c, params, count);
+}
+
+int drape_self_contact_pairs(drape_ctx* p, int32_t* n_pt, int32_t* n_ee, int32_t* vertices, double* distances) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
+  return self_contact_pairs(*c, n_pt, n_ee, vertices, distances);
 }
 
 static Context* training_context(drape_ctx* p) {

@@ -28,6 +28,8 @@ struct PcgState {
 struct TwoLevel;
 // Training buffers of the learned prolongation -- see gnn_train.hip.
 struct GnnTrain;
+// Self-contact broad phase, candidates and pair terms -- see selfcontact.hip.
+struct SelfContact;
 
 struct Context {
   // sizes
@@ -110,6 +112,7 @@ struct Context {
   bool curvature_dropped = false;
   TwoLevel* two_level = nullptr;
   GnnTrain* gnn_train = nullptr;  // training buffers of the learned prolongation (allocated on first use)
+  SelfContact* self_contact = nullptr;  // self-contact state (allocated on first use under rules.self_contact)
   uint64_t system_version = 0;    // bumped by every assembly: the coarse operator follows it
   uint64_t shift_version = 0;     // bumped when mu or the step size of M/h^2 changes
   float shift_mu = 0.0f;
@@ -192,6 +195,22 @@ int copy_timing(size_t bytes, int launches, double* avg_ms);  // known-size copy
 int launch_set_float(Context& c, float* dst, float v);  // device scalar write by kernel argument
 int launch_dyn_predict(Context& c, const float4* vel, double h);  // x_hat = x + h v + h^2 g on the free rows
 int launch_dyn_velocity(Context& c, float4* vel, double h);       // v = (pos - org) / h
+
+// ---- self-contact (selfcontact.hip) ---------------------------------------------------------
+// All no-ops unless rules.self_contact. evaluate: broad phase within dhat at the current positions and the
+// pair terms; assemble: their gradient into b and diagonal blocks into the matrix (after launch_assemble's
+// kernels); view: the coupling the SpMV adds (left null without terms); step_bound / energy_change: over
+// the pairs that can come within dhat along the current direction (found once per direction; moved() marks
+// positions or direction as changed).
+void self_contact_destroy(SelfContact* sc);
+int self_contact_evaluate(Context& c);
+int self_contact_assemble(Context& c);
+void self_contact_view(const Context& c, const uint32_t** cptr, const uint32_t** cinc, const int4** crow,
+                       const float4** cdat, const float4** ccoef);
+void self_contact_moved(Context& c);
+int self_contact_step_bound(Context& c, float* alpha, int32_t* row);
+int self_contact_energy_change(Context& c, float alpha, double* sum, double* abs_sum);
+int self_contact_pairs(Context& c, int32_t* n_pt, int32_t* n_ee, int32_t* vertices, double* distances);
 
 // ---- PCG (pcg.hip) --------------------------------------------------------------------------
 int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats);

@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "../internal.h"
+#include "barrier.hpp"
 #include "reduce.hpp"
 
 namespace drape {
@@ -203,20 +204,6 @@ __global__ void __launch_bounds__(128) k_hinge(const int4* __restrict__ hin, con
 }
 
 // ---- rows: inertia and vertex-collider barrier ----------------------------------------------
-// b(d) = -kappa (d - dhat)^2 ln(d/dhat) and its derivatives (contact.rs:81-108), in FP64.
-struct BarrierTerms {
-  double value, first, second;
-};
-__device__ inline BarrierTerms barrier_terms(double d, double dhat, double kappa) {
-  BarrierTerms b{0.0, 0.0, 0.0};
-  if (d >= dhat) return b;
-  const double t = d - dhat;
-  const double ln = log(d / dhat);
-  b.value = d <= 0.0 ? INFINITY : -kappa * (t * t) * ln;
-  b.first = -kappa * (2.0 * t * ln + t * t / d);
-  b.second = -kappa * (2.0 * ln + 4.0 * t / d - t * t / (d * d));
-  return b;
-}
 
 struct Proximity {
   double distance;
@@ -685,7 +672,7 @@ int launch_evaluate(Context& c, bool exact) {
                                          c.lgrad + 9 * (size_t)c.T + 12 * (size_t)c.H,
                                          c.lhess + 81 * (size_t)c.T + 144 * (size_t)c.H);
   HIP_TRY(hipGetLastError());
-  return DRAPE_OK;
+  return self_contact_evaluate(c);
 }
 
 int launch_assemble(Context& c) {
@@ -694,7 +681,7 @@ int launch_assemble(Context& c) {
   k_assemble_blocks<<<blocks_for(c.B), 256, 0, s>>>(c.bc_ptr, c.bc_idx, c.lhess, c.block_dst, c.B, c.sell_val);
   k_assemble_rhs<<<blocks_for(c.R), 256, 0, s>>>(c.fc_ptr, c.fc_idx, c.lgrad, c.R, c.b);
   HIP_TRY(hipGetLastError());
-  return DRAPE_OK;
+  return self_contact_assemble(c);
 }
 
 static int row_sum(Context& c, const float4* a, const float4* w, double* out) {
@@ -719,12 +706,14 @@ int launch_slope(Context& c, double* slope) {
 int launch_scatter_direction(Context& c, const float4* rows_src) {
   k_scatter_direction<<<blocks_for(c.R), 256, 0, c.stream>>>(rows_src, c.rvert, c.R, c.dir);
   HIP_TRY(hipGetLastError());
+  self_contact_moved(c);
   return DRAPE_OK;
 }
 
 int launch_advance(Context& c, float alpha) {
   k_advance<<<blocks_for(c.R), 256, 0, c.stream>>>(c.pos, c.dir, c.rvert, c.R, alpha);
   HIP_TRY(hipGetLastError());
+  self_contact_moved(c);
   return DRAPE_OK;
 }
 
@@ -739,6 +728,12 @@ int launch_energy_change(Context& c, float alpha, double* sum, double* abs_sum) 
   TRY_RC(read_back(c, c.dout_host, c.dout, 2 * sizeof(double)));
   *sum = c.dout_host[0];
   *abs_sum = c.dout_host[1];
+  if (c.rules.self_contact) {
+    double cs = 0.0, ca = 0.0;
+    TRY_RC(self_contact_energy_change(c, alpha, &cs, &ca));
+    *sum += cs;
+    *abs_sum += ca;
+  }
   return DRAPE_OK;
 }
 
@@ -756,6 +751,15 @@ int launch_max_step(Context& c, float* alpha, int32_t* row, float* min_distance,
   *row = (int32_t)c.dout_host[1];
   *min_distance = (float)c.dout_host[2];
   *min_row = (int32_t)c.dout_host[3];
+  if (c.rules.self_contact) {
+    float a = 1.0f;
+    int32_t r = -1;
+    TRY_RC(self_contact_step_bound(c, &a, &r));
+    if (a < *alpha) {
+      *alpha = a;
+      *row = r;
+    }
+  }
   return DRAPE_OK;
 }
 
@@ -778,6 +782,7 @@ int launch_towards(Context& c) {
 int launch_step_all(Context& c, float alpha) {
   k_step_all<<<blocks_for(c.V), 256, 0, c.stream>>>(c.pos, c.dir, c.V, alpha);
   HIP_TRY(hipGetLastError());
+  self_contact_moved(c);
   return DRAPE_OK;
 }
 

@@ -98,7 +98,8 @@ __global__ void __launch_bounds__(256) k_pcg_spmv(SellView A, const float4* __re
   const uint32_t row = A.perm[phys];
   float pq = 0.0f;
   if (row != 0xffffffffu) {
-    const float3 a = row_times<false>(A, phys, p, nullptr, 0.0f);
+    float3 a = row_times<false>(A, phys, p, nullptr, 0.0f);
+    if (A.cptr) a = add3(a, contact_times<false>(A, row, p, nullptr, 0.0f));
     q[row] = make_float4(a.x, a.y, a.z, 0.0f);
     const float4 pr = p[row];
     pq = pr.x * a.x + pr.y * a.y + pr.z * a.z;
@@ -138,7 +139,8 @@ __global__ void __launch_bounds__(256) k_pcg_dir_spmv(SellView A, const float* _
   const uint32_t row = A.perm[phys];
   float pq = 0.0f;
   if (row != 0xffffffffu) {
-    const float3 a = row_times<true>(A, phys, p_old, z, beta);
+    float3 a = row_times<true>(A, phys, p_old, z, beta);
+    if (A.cptr) a = add3(a, contact_times<true>(A, row, p_old, z, beta));
     q[row] = make_float4(a.x, a.y, a.z, 0.0f);
     const float4 zr = z[row], po = p_old[row];
     float4 pr;
@@ -259,7 +261,8 @@ __global__ void __launch_bounds__(256) k_pcg_residual(SellView A, const float4* 
   const uint32_t row = A.perm[phys];
   float a = 0.0f, c = 0.0f;
   if (row != 0xffffffffu) {
-    const float3 ax = row_times<false>(A, phys, x, nullptr, 0.0f);
+    float3 ax = row_times<false>(A, phys, x, nullptr, 0.0f);
+    if (A.cptr) ax = add3(ax, contact_times<false>(A, row, x, nullptr, 0.0f));
     const float4 bb = b[row];
     const float s0 = bb.x - ax.x, s1 = bb.y - ax.y, s2 = bb.z - ax.z;
     const float ss = s0 * s0 + s1 * s1 + s2 * s2;
@@ -427,13 +430,21 @@ __global__ void __launch_bounds__(256) k_vc_post(SellView A, const float4* __res
   }
 }
 
-static SellView view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm}; }
+// The system's view, with the self-contact coupling when self-contact terms are active. The V-cycle's
+// smoothing residuals use the view without it (the mesh operator with the contact diagonal blocks): a valid
+// symmetric preconditioner of the full operator, which PCG's own products apply.
+static SellView view_of(Context& c) {
+  SellView v{c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm};
+  self_contact_view(c, &v.cptr, &v.cinc, &v.crow, &v.cdat, &v.ccoef);
+  return v;
+}
+static SellView mesh_view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm}; }
 
 // z = M^-1 r by the V-cycle for the current r (after the caller's z1 = w D^-1 r): guard as above.
 static int vcycle_rest(Context& c, float* partials, int guard) {
   hipStream_t s = c.stream;
   const float omega = (float)c.rules.smoother;
-  SellView A = view_of(c);
+  SellView A = mesh_view_of(c);
   k_vc_residual1<<<c.G, 256, 0, s>>>(A, c.z1, c.r, c.r1, c.state, guard);
   TRY_RC(two_level_restrict_solve(c, c.r1, guard));
   k_vc_post<<<c.G, 256, 0, s>>>(A, c.z1, two_level_view(c), c.dinv, c.r, c.z, c.p, omega, partials, c.state, guard);
@@ -600,6 +611,13 @@ __global__ void __launch_bounds__(256) k_residual64(SellView A, const double* __
       a0 += (double)v[0] * x0 + (double)v[64] * x1 + (double)v[128] * x2;
       a1 += (double)v[192] * x0 + (double)v[256] * x1 + (double)v[320] * x2;
       a2 += (double)v[384] * x0 + (double)v[448] * x1 + (double)v[512] * x2;
+    }
+    if (A.cptr) {
+      double ct[3];
+      contact_times64(A, row, x, ct);
+      a0 += ct[0];
+      a1 += ct[1];
+      a2 += ct[2];
     }
     const float4 bb = b[row];
     const double r0 = (double)bb.x - a0, r1 = (double)bb.y - a1, r2 = (double)bb.z - a2;

@@ -1,0 +1,892 @@
+// Self-contact of the sheet (no reference counterpart: the reference's contact is vertex-collider only): the
+// IPC log barrier b(d) (the colliders' dhat and kappa) over every point-triangle and edge-edge pair of the
+// mesh closer than dhat, in FP64.
+//
+// Broad phase: a uniform grid of cell size h = (longest edge) + (inflation), hashed (Teschner et al. 2003:
+// 73856093 x ^ 19349663 y ^ 83492791 z) and sorted by key (hipCUB radix sort, stable). A vertex within the
+// inflation distance of a triangle lies within h of its centroid, and two edges within it have midpoints
+// within h, so each triangle queries the vertices, and each edge the edges, of the 27 cells around its
+// centroid (midpoint) cell; the true cell of every visited primitive is checked, so hash collisions neither
+// add nor duplicate pairs. Counting pass, scan, then the emitting pass in the same order: the candidate list
+// is deterministic (point-triangle pairs by triangle, then edge-edge pairs by first edge).
+//
+// Narrow phase: closest points by Voronoi-region classification (point-triangle) and clamped parameters
+// (segment-segment), following Ericson, Real-Time Collision Detection, 5.1.5 and 5.1.9. The distance
+// gradient comes from the envelope theorem: with closest points x1 = sum a_k p_k on one primitive and
+// x2 on the other and n = (x1 - x2)/d, grad_k d = c_k n with c_k the barycentric weight of vertex k (negated
+// on the second primitive). A pair contributes b(d) to the energy, b'(d) c_k n to the gradient of its vertex
+// k and the Gauss-Newton block b''(d) c_k c_l n n^T to the Hessian (positive semidefinite).
+//
+// Step bound: conservative advancement per pair (Mirtich 1996; as in additive CCD, Li et al. 2021): the
+// distance of a pair moves at most lp t along t -> x + t d, lp = the largest displacement on each primitive
+// summed, so steps of 0.9 (d - g)/lp keep it above g = 0.1 d(x), the reference's collider rule.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../internal.h"
+#include "barrier.hpp"
+#include "reduce.hpp"
+#include "sell.hpp"
+
+namespace drape {
+
+struct SelfContact {
+  int V = 0, T = 0, E = 0;
+  int2* d_edges = nullptr;       // [E] unique mesh edges (a < b)
+  int32_t* d_row_of = nullptr;   // [V] row of each vertex, -1 when pinned
+  // broad phase
+  uint32_t *d_vkey = nullptr, *d_vid = nullptr, *d_vkey_s = nullptr, *d_vid_s = nullptr;  // [V]
+  uint32_t *d_ekey = nullptr, *d_eid = nullptr, *d_ekey_s = nullptr, *d_eid_s = nullptr;  // [E]
+  uint32_t *d_tcount = nullptr, *d_toff = nullptr;  // [T + 1]
+  uint32_t *d_ecount = nullptr, *d_eoff = nullptr;  // [E + 1]
+  float* d_part = nullptr;       // [2 * groups] block maxima (edge length, displacement)
+  float* d_grid = nullptr;       // [2] cell size, inflation
+  void* d_tmp = nullptr;         // hipCUB scratch
+  size_t tmp_bytes = 0;
+  uint32_t* h_totals = nullptr;  // pinned [4]
+  // candidates of the evaluated state (their pair terms below) and of the current step (the line-search set)
+  struct Set {
+    int cap = 0, npt = 0, nee = 0;
+    int4* d_cand = nullptr;      // [cap] PT: (p, a, b, c); EE: (p1, q1, p2, q2)
+  } eval, step;
+  bool step_ready = false;       // `step` belongs to the current positions and direction
+  int tcap = 0;                  // pairs the term arrays hold
+  // pair terms of the last evaluation (active pairs only contribute)
+  int4* d_crow = nullptr;        // [cap] rows of the pair's vertices (-1: pinned)
+  float4* d_cdat = nullptr;      // [cap] n, b''
+  float4* d_ccoef = nullptr;     // [cap] c_0..c_3
+  float* d_cfirst = nullptr;     // [cap] b'
+  double* d_dist = nullptr;      // [cap] distance
+  uint32_t *d_ikey = nullptr, *d_ival = nullptr, *d_ikey_s = nullptr, *d_ival_s = nullptr;  // [4 cap]
+  uint32_t* d_iptr = nullptr;    // [R + 1]
+  bool terms = false;            // pair terms and incidence lists belong to the current assembly
+  double dhat = 0.0, kappa = 0.0;
+  // line-search sums
+  double* d_epart = nullptr;     // [2 * groups]
+  double* d_eout = nullptr;      // [2]
+  float* d_apart = nullptr;      // [groups] step bound per group
+  int32_t* d_aidx = nullptr;     // [groups] pair of the group's bound
+  int parts = 0;                 // groups the partial buffers hold
+};
+
+void self_contact_destroy(SelfContact* sc) {
+  for (void* p : {(void*)sc->d_edges, (void*)sc->d_row_of, (void*)sc->d_vkey, (void*)sc->d_vid, (void*)sc->d_vkey_s,
+                  (void*)sc->d_vid_s, (void*)sc->d_ekey, (void*)sc->d_eid, (void*)sc->d_ekey_s, (void*)sc->d_eid_s,
+                  (void*)sc->d_tcount, (void*)sc->d_toff, (void*)sc->d_ecount, (void*)sc->d_eoff, (void*)sc->d_part,
+                  (void*)sc->d_grid, sc->d_tmp, (void*)sc->eval.d_cand, (void*)sc->step.d_cand, (void*)sc->d_crow, (void*)sc->d_cdat,
+                  (void*)sc->d_ccoef, (void*)sc->d_cfirst, (void*)sc->d_dist, (void*)sc->d_ikey, (void*)sc->d_ival,
+                  (void*)sc->d_ikey_s, (void*)sc->d_ival_s, (void*)sc->d_iptr, (void*)sc->d_epart,
+                  (void*)sc->d_eout, (void*)sc->d_apart, (void*)sc->d_aidx})
+    if (p) (void)hipFree(p);
+  if (sc->h_totals) (void)hipHostFree(sc->h_totals);
+  delete sc;
+}
+
+namespace {
+
+struct P3 {
+  double x, y, z;
+};
+__device__ inline P3 p3(const float4& v) { return {v.x, v.y, v.z}; }
+__device__ inline P3 sub(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline P3 axpy(P3 a, double s, P3 b) { return {a.x + s * b.x, a.y + s * b.y, a.z + s * b.z}; }
+__device__ inline double dot(P3 a, P3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Closest point of triangle (a, b, c) to p as barycentric weights w (Voronoi regions: vertices, edges,
+// interior).
+__device__ inline void closest_on_triangle(P3 p, P3 a, P3 b, P3 c, double w[3]) {
+  const P3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+  const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) { w[0] = 1.0; w[1] = 0.0; w[2] = 0.0; return; }
+  const P3 bp = sub(p, b);
+  const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0 && d4 <= d3) { w[0] = 0.0; w[1] = 1.0; w[2] = 0.0; return; }
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+    const double v = d1 / (d1 - d3);
+    w[0] = 1.0 - v; w[1] = v; w[2] = 0.0;
+    return;
+  }
+  const P3 cp = sub(p, c);
+  const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) { w[0] = 0.0; w[1] = 0.0; w[2] = 1.0; return; }
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+    const double t = d2 / (d2 - d6);
+    w[0] = 1.0 - t; w[1] = 0.0; w[2] = t;
+    return;
+  }
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+    const double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    w[0] = 0.0; w[1] = 1.0 - t; w[2] = t;
+    return;
+  }
+  const double den = 1.0 / (va + vb + vc);
+  const double v = vb * den, t = vc * den;
+  w[0] = 1.0 - v - t; w[1] = v; w[2] = t;
+}
+
+// Closest points of segments p1 + s (q1 - p1) and p2 + t (q2 - p2), s, t in [0, 1].
+__device__ inline void closest_on_segments(P3 p1, P3 q1, P3 p2, P3 q2, double* s_out, double* t_out) {
+  const P3 d1 = sub(q1, p1), d2 = sub(q2, p2), r = sub(p1, p2);
+  const double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  const double tiny = 1e-30;
+  double s = 0.0, t = 0.0;
+  if (a <= tiny && e <= tiny) {
+    s = t = 0.0;
+  } else if (a <= tiny) {
+    s = 0.0;
+    t = fmin(fmax(f / e, 0.0), 1.0);
+  } else {
+    const double c = dot(d1, r);
+    if (e <= tiny) {
+      t = 0.0;
+      s = fmin(fmax(-c / a, 0.0), 1.0);
+    } else {
+      const double b = dot(d1, d2), den = a * e - b * b;
+      s = den > 0.0 ? fmin(fmax((b * f - c * e) / den, 0.0), 1.0) : 0.0;
+      t = (b * s + f) / e;
+      if (t < 0.0) {
+        t = 0.0;
+        s = fmin(fmax(-c / a, 0.0), 1.0);
+      } else if (t > 1.0) {
+        t = 1.0;
+        s = fmin(fmax((b - c) / a, 0.0), 1.0);
+      }
+    }
+  }
+  *s_out = s;
+  *t_out = t;
+}
+
+// Distance of a pair, its unit direction n and vertex coefficients c (grad_k d = c_k n).
+struct PairGeom {
+  double d;
+  P3 n;
+  double c[4];
+};
+__device__ inline PairGeom pair_geometry(bool pt, P3 x0, P3 x1, P3 x2, P3 x3) {
+  PairGeom g;
+  P3 diff;
+  if (pt) {
+    double w[3];
+    closest_on_triangle(x0, x1, x2, x3, w);
+    const P3 q = {w[0] * x1.x + w[1] * x2.x + w[2] * x3.x, w[0] * x1.y + w[1] * x2.y + w[2] * x3.y,
+                  w[0] * x1.z + w[1] * x2.z + w[2] * x3.z};
+    diff = sub(x0, q);
+    g.c[0] = 1.0; g.c[1] = -w[0]; g.c[2] = -w[1]; g.c[3] = -w[2];
+  } else {
+    double s, t;
+    closest_on_segments(x0, x1, x2, x3, &s, &t);
+    const P3 a = axpy(x0, s, sub(x1, x0)), b = axpy(x2, t, sub(x3, x2));
+    diff = sub(a, b);
+    g.c[0] = 1.0 - s; g.c[1] = s; g.c[2] = -(1.0 - t); g.c[3] = -t;
+  }
+  g.d = sqrt(dot(diff, diff));
+  const double inv = g.d > 0.0 ? 1.0 / g.d : 0.0;
+  g.n = {diff.x * inv, diff.y * inv, diff.z * inv};
+  return g;
+}
+
+__device__ inline int3 cell_of(P3 x, double h) {
+  return make_int3((int)floor(x.x / h), (int)floor(x.y / h), (int)floor(x.z / h));
+}
+__device__ inline uint32_t cell_key(int3 c) {
+  return ((uint32_t)c.x * 73856093u) ^ ((uint32_t)c.y * 19349663u) ^ ((uint32_t)c.z * 83492791u);
+}
+__device__ inline bool same_cell(int3 a, int3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// ---- broad phase -------------------------------------------------------------------------------
+
+// block maxima of the edge lengths and of the displacement |dir| (dir may be null)
+__global__ void __launch_bounds__(256) k_extent(const int2* __restrict__ edges, int E, const float4* __restrict__ pos,
+                                                const float4* __restrict__ dir, const int32_t* __restrict__ row_of,
+                                                int V, float* __restrict__ part) {
+  __shared__ float la[256], lb[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  float len = 0.0f, mv = 0.0f;
+  if (i < E) {
+    const int2 e = edges[i];
+    const float4 a = pos[e.x], b = pos[e.y];
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    len = sqrtf(dx * dx + dy * dy + dz * dz);
+  }
+  if (dir && i < V && row_of[i] >= 0) {
+    const float4 d = dir[i];
+    mv = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+  }
+  la[threadIdx.x] = len;
+  lb[threadIdx.x] = mv;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      la[threadIdx.x] = fmaxf(la[threadIdx.x], la[threadIdx.x + s]);
+      lb[threadIdx.x] = fmaxf(lb[threadIdx.x], lb[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = la[0];
+    part[2 * blockIdx.x + 1] = lb[0];
+  }
+}
+
+// grid[0] = cell size = longest edge + inflation, grid[1] = inflation = dhat + 2 |dir|max (the most two
+// primitives can approach along the step), both rounded up slightly
+__global__ void k_grid(const float* __restrict__ part, int groups, float dhat, float* __restrict__ grid) {
+  __shared__ float la[256], lb[256];
+  float a = 0.0f, b = 0.0f;
+  for (int j = threadIdx.x; j < groups; j += 256) {
+    a = fmaxf(a, part[2 * j]);
+    b = fmaxf(b, part[2 * j + 1]);
+  }
+  la[threadIdx.x] = a;
+  lb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      la[threadIdx.x] = fmaxf(la[threadIdx.x], la[threadIdx.x + s]);
+      lb[threadIdx.x] = fmaxf(lb[threadIdx.x], lb[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float infl = (dhat + 2.0f * lb[0]) * 1.0001f;
+    grid[1] = infl;
+    grid[0] = (la[0] + infl) * 1.0001f;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_vertex_keys(const float4* __restrict__ pos, int V,
+                                                     const float* __restrict__ grid, uint32_t* __restrict__ key,
+                                                     uint32_t* __restrict__ id) {
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= V) return;
+  key[v] = cell_key(cell_of(p3(pos[v]), grid[0]));
+  id[v] = (uint32_t)v;
+}
+
+__device__ inline P3 midpoint(const float4* pos, int2 e) {
+  const P3 a = p3(pos[e.x]), b = p3(pos[e.y]);
+  return {0.5 * (a.x + b.x), 0.5 * (a.y + b.y), 0.5 * (a.z + b.z)};
+}
+
+__global__ void __launch_bounds__(256) k_edge_keys(const int2* __restrict__ edges, int E,
+                                                   const float4* __restrict__ pos, const float* __restrict__ grid,
+                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ id) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= E) return;
+  key[e] = cell_key(cell_of(midpoint(pos, edges[e]), grid[0]));
+  id[e] = (uint32_t)e;
+}
+
+// point-triangle candidates of triangle t: EMIT 0 counts, 1 writes at out + offset[t]
+template <int EMIT>
+__global__ void __launch_bounds__(256) k_pt(const int4* __restrict__ tri, int T, const float4* __restrict__ pos,
+                                            int V, const float* __restrict__ grid, double limit,
+                                            const uint32_t* __restrict__ vkey, const uint32_t* __restrict__ vid,
+                                            uint32_t* __restrict__ count, const uint32_t* __restrict__ offset,
+                                            int4* __restrict__ out) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= T) return;
+  const double h = grid[0], infl = limit > 0.0 ? limit : grid[1];
+  const int4 tv = tri[t];
+  const P3 a = p3(pos[tv.x]), b = p3(pos[tv.y]), c = p3(pos[tv.z]);
+  const P3 cen = {(a.x + b.x + c.x) / 3.0, (a.y + b.y + c.y) / 3.0, (a.z + b.z + c.z) / 3.0};
+  const int3 cc = cell_of(cen, h);
+  uint32_t n = 0, at = EMIT ? offset[t] : 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int3 q = make_int3(cc.x + dx, cc.y + dy, cc.z + dz);
+        const uint32_t key = cell_key(q);
+        for (uint32_t i = lower_bound_u32(vkey, V, key); i < (uint32_t)V && vkey[i] == key; ++i) {
+          const int v = (int)vid[i];
+          if (v == tv.x || v == tv.y || v == tv.z) continue;
+          const P3 x = p3(pos[v]);
+          if (!same_cell(cell_of(x, h), q)) continue;
+          if (pair_geometry(true, x, a, b, c).d >= infl) continue;
+          if (EMIT) out[at++] = make_int4(v, tv.x, tv.y, tv.z);
+          ++n;
+        }
+      }
+  if (!EMIT) count[t] = n;
+}
+
+// edge-edge candidates (e, f) with f > e and no shared vertex
+template <int EMIT>
+__global__ void __launch_bounds__(256) k_ee(const int2* __restrict__ edges, int E, const float4* __restrict__ pos,
+                                            const float* __restrict__ grid, double limit,
+                                            const uint32_t* __restrict__ ekey,
+                                            const uint32_t* __restrict__ eid, uint32_t* __restrict__ count,
+                                            const uint32_t* __restrict__ offset, int4* __restrict__ out) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= E) return;
+  const double h = grid[0], infl = limit > 0.0 ? limit : grid[1];
+  const int2 ev = edges[e];
+  const P3 p1 = p3(pos[ev.x]), q1 = p3(pos[ev.y]);
+  const int3 cc = cell_of(midpoint(pos, ev), h);
+  uint32_t n = 0, at = EMIT ? offset[e] : 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int3 q = make_int3(cc.x + dx, cc.y + dy, cc.z + dz);
+        const uint32_t key = cell_key(q);
+        for (uint32_t i = lower_bound_u32(ekey, E, key); i < (uint32_t)E && ekey[i] == key; ++i) {
+          const int f = (int)eid[i];
+          if (f <= e) continue;
+          const int2 fv = edges[f];
+          if (fv.x == ev.x || fv.x == ev.y || fv.y == ev.x || fv.y == ev.y) continue;
+          if (!same_cell(cell_of(midpoint(pos, fv), h), q)) continue;
+          if (pair_geometry(false, p1, q1, p3(pos[fv.x]), p3(pos[fv.y])).d >= infl) continue;
+          if (EMIT) out[at++] = make_int4(ev.x, ev.y, fv.x, fv.y);
+          ++n;
+        }
+      }
+  if (!EMIT) count[e] = n;
+}
+
+// ---- pair terms ----------------------------------------------------------------------------------
+
+// distance, direction, coefficients and barrier derivatives of each candidate; the incidence entries
+// (row, 4 pair + k) of the active pairs' free vertices, others keyed past every row
+__global__ void __launch_bounds__(256) k_pair_terms(const int4* __restrict__ cand, int n, int npt,
+                                                    const float4* __restrict__ pos, const int32_t* __restrict__ row_of,
+                                                    double dhat, double kappa, int4* __restrict__ crow,
+                                                    float4* __restrict__ cdat, float4* __restrict__ ccoef,
+                                                    float* __restrict__ cfirst, double* __restrict__ dist,
+                                                    uint32_t* __restrict__ ikey, uint32_t* __restrict__ ival) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int4 v = cand[c];
+  const PairGeom g = pair_geometry(c < npt, p3(pos[v.x]), p3(pos[v.y]), p3(pos[v.z]), p3(pos[v.w]));
+  const bool active = g.d < dhat && g.d > 0.0;
+  const BarrierTerms b = active ? barrier_terms(g.d, dhat, kappa) : BarrierTerms{0.0, 0.0, 0.0};
+  const int4 rows = make_int4(row_of[v.x], row_of[v.y], row_of[v.z], row_of[v.w]);
+  crow[c] = rows;
+  cdat[c] = make_float4((float)g.n.x, (float)g.n.y, (float)g.n.z, (float)b.second);
+  ccoef[c] = make_float4((float)g.c[0], (float)g.c[1], (float)g.c[2], (float)g.c[3]);
+  cfirst[c] = (float)b.first;
+  dist[c] = g.d;
+  const int r[4] = {rows.x, rows.y, rows.z, rows.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ikey[4 * c + k] = (active && r[k] >= 0) ? (uint32_t)r[k] : 0xffffffffu;
+    ival[4 * c + k] = 4u * (uint32_t)c + (uint32_t)k;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_row_ptr(const uint32_t* __restrict__ keys, uint32_t n, int R,
+                                                 uint32_t* __restrict__ ptr) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r > R) return;
+  ptr[r] = lower_bound_u32(keys, n, (uint32_t)r);
+}
+
+// per row, in its list order: b -= b' c_k n and the diagonal block += b'' c_k^2 n n^T
+__global__ void __launch_bounds__(256) k_contact_assemble(const uint32_t* __restrict__ iptr,
+                                                          const uint32_t* __restrict__ ival,
+                                                          const float4* __restrict__ cdat,
+                                                          const float4* __restrict__ ccoef,
+                                                          const float* __restrict__ cfirst,
+                                                          const uint32_t* __restrict__ diag_slot, int R,
+                                                          float* __restrict__ sell_val, float4* __restrict__ b) {
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= R) return;
+  const uint32_t t0 = iptr[row], t1 = iptr[row + 1];
+  if (t0 == t1) return;
+  float4 rhs = b[row];
+  float* d = sell_val + diag_slot[row];
+  float blk[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) blk[e] = d[64 * e];
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t e = ival[t];
+    const uint32_t pair = e >> 2;
+    const int k = (int)(e & 3u);
+    const float4 dn = cdat[pair];
+    const float ck = coef_of(ccoef[pair], k);
+    const float g = cfirst[pair] * ck;
+    rhs.x = rhs.x - g * dn.x;
+    rhs.y = rhs.y - g * dn.y;
+    rhs.z = rhs.z - g * dn.z;
+    const float s = dn.w * ck * ck;
+    const float n3[3] = {dn.x, dn.y, dn.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) blk[3 * i + j] = blk[3 * i + j] + s * n3[i] * n3[j];
+  }
+  b[row] = rhs;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) d[64 * e] = blk[e];
+}
+
+// ---- line search ---------------------------------------------------------------------------------
+
+// FP64 partial sums of (b(d(x + alpha dir)) - b(d(x)), |same|) over the candidates
+__global__ void __launch_bounds__(256) k_contact_energy(const int4* __restrict__ cand, int n, int npt,
+                                                        const float4* __restrict__ pos,
+                                                        const float4* __restrict__ dir,
+                                                        const int32_t* __restrict__ row_of, float alpha, double dhat,
+                                                        double kappa, double* __restrict__ part) {
+  __shared__ double2 lds[256];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  double delta = 0.0;
+  if (c < n) {
+    const int4 v = cand[c];
+    const bool pt = c < npt;
+    const int ids[4] = {v.x, v.y, v.z, v.w};
+    P3 x[4], y[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float4 xp = pos[ids[k]];
+      x[k] = p3(xp);
+      // the FP32 step the advance applies to free vertices: pos + alpha dir
+      const float4 d = row_of[ids[k]] >= 0 ? dir[ids[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      y[k] = {(double)(xp.x + alpha * d.x), (double)(xp.y + alpha * d.y), (double)(xp.z + alpha * d.z)};
+    }
+    const double before = pair_geometry(pt, x[0], x[1], x[2], x[3]).d;
+    const double after = pair_geometry(pt, y[0], y[1], y[2], y[3]).d;
+    delta = barrier_terms(after, dhat, kappa).value - barrier_terms(before, dhat, kappa).value;
+  }
+  const double2 t = tree256d2(delta, fabs(delta), lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t.x;
+    part[2 * blockIdx.x + 1] = t.y;
+  }
+}
+
+__global__ void k_sum_pairs64(const double* __restrict__ part, int count, double* __restrict__ out) {
+  __shared__ double2 lds[256];
+  double a = 0.0, b = 0.0;
+  for (int j = threadIdx.x; j < count; j += 256) {
+    a += part[2 * j];
+    b += part[2 * j + 1];
+  }
+  const double2 t = tree256d2(a, b, lds);
+  if (threadIdx.x == 0) {
+    out[0] = t.x;
+    out[1] = t.y;
+  }
+}
+
+// conservative advancement of each candidate along x + t dir, t in [0, 1]; group minima (lowest pair on
+// ties)
+__global__ void __launch_bounds__(256) k_contact_ccd(const int4* __restrict__ cand, int n, int npt,
+                                                     const float4* __restrict__ pos, const float4* __restrict__ dir,
+                                                     const int32_t* __restrict__ row_of, float* __restrict__ apart,
+                                                     int32_t* __restrict__ aidx) {
+  __shared__ float sa[256];
+  __shared__ int ia[256];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  float alpha = 1.0f;
+  if (c < n) {
+    const int4 v = cand[c];
+    const bool pt = c < npt;
+    const int ids[4] = {v.x, v.y, v.z, v.w};
+    P3 x[4], d[4];
+    double m[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      x[k] = p3(pos[ids[k]]);
+      d[k] = row_of[ids[k]] >= 0 ? p3(dir[ids[k]]) : P3{0.0, 0.0, 0.0};
+      m[k] = sqrt(dot(d[k], d[k]));
+    }
+    const double lp = pt ? m[0] + fmax(m[1], fmax(m[2], m[3])) : fmax(m[0], m[1]) + fmax(m[2], m[3]);
+    const double d0 = pair_geometry(pt, x[0], x[1], x[2], x[3]).d;
+    if (lp > 0.0 && d0 > 0.0) {
+      const double target = 0.1 * d0;
+      double t = 0.0, dist = d0;
+      bool done = false;
+      for (int it = 0; it < 64 && !done; ++it) {
+        const double dt = 0.9 * (dist - target) / lp;
+        if (t + dt >= 1.0) {
+          t = 1.0;
+          done = true;
+        } else if (dt <= 1e-9 * (t + dt)) {
+          done = true;
+        } else {
+          t += dt;
+          P3 y[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) y[k] = axpy(x[k], t, d[k]);
+          dist = pair_geometry(pt, y[0], y[1], y[2], y[3]).d;
+        }
+      }
+      alpha = (float)t;
+      if ((double)alpha > t) alpha = nextafterf(alpha, 0.0f);  // never round the bound up
+    } else if (d0 <= 0.0) {
+      alpha = 0.0f;
+    }
+  }
+  sa[threadIdx.x] = alpha;
+  ia[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const int t = threadIdx.x, u = t + s;
+      if (sa[u] < sa[t] || (sa[u] == sa[t] && ia[u] < ia[t])) {
+        sa[t] = sa[u];
+        ia[t] = ia[u];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    apart[blockIdx.x] = sa[0];
+    aidx[blockIdx.x] = ia[0];
+  }
+}
+
+__global__ void k_min_final(const float* __restrict__ apart, const int32_t* __restrict__ aidx, int count,
+                            const int4* __restrict__ cand, const int32_t* __restrict__ row_of,
+                            double* __restrict__ out) {
+  __shared__ float sa[256];
+  __shared__ int ia[256];
+  float a = 1.0f;
+  int ai = 0x7fffffff;
+  for (int j = threadIdx.x; j < count; j += 256)
+    if (apart[j] < a || (apart[j] == a && aidx[j] < ai)) {
+      a = apart[j];
+      ai = aidx[j];
+    }
+  sa[threadIdx.x] = a;
+  ia[threadIdx.x] = ai;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const int t = threadIdx.x, u = t + s;
+      if (sa[u] < sa[t] || (sa[u] == sa[t] && ia[u] < ia[t])) {
+        sa[t] = sa[u];
+        ia[t] = ia[u];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sa[0];
+    int row = -1;
+    if (ia[0] != 0x7fffffff) {
+      const int4 v = cand[ia[0]];
+      const int ids[4] = {v.x, v.y, v.z, v.w};
+      for (int k = 0; k < 4 && row < 0; ++k) row = row_of[ids[k]];
+    }
+    out[1] = row;
+  }
+}
+
+template <typename T>
+int grow(T** p, size_t n) {
+  if (*p) HIP_TRY(hipFree(*p));
+  *p = nullptr;
+  HIP_TRY(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+  return DRAPE_OK;
+}
+
+int setup(Context& c) {
+  if (c.self_contact) return DRAPE_OK;
+  if (c.K <= 0) {
+    set_error("self-contact takes its barrier (dhat, kappa) from the colliders; the scene has none");
+    return DRAPE_E_INVALID;
+  }
+  SelfContact* sc = new SelfContact();
+  c.self_contact = sc;
+  sc->V = c.V;
+  sc->T = c.T;
+  hipStream_t s = c.stream;
+  // unique edges of the triangles, and the row of each vertex
+  std::vector<int4> tri(c.T);
+  HIP_TRY(hipMemcpyAsync(tri.data(), c.tri, (size_t)c.T * sizeof(int4), hipMemcpyDeviceToHost, s));
+  float hdr[4];
+  HIP_TRY(hipMemcpyAsync(hdr, c.colliders, sizeof(hdr), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  sc->dhat = hdr[1];
+  sc->kappa = hdr[2];
+  std::vector<uint64_t> keys;
+  keys.reserve(3 * (size_t)c.T);
+  for (const int4& t : tri) {
+    const int v[3] = {t.x, t.y, t.z};
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t a = (uint32_t)std::min(v[k], v[(k + 1) % 3]), b = (uint32_t)std::max(v[k], v[(k + 1) % 3]);
+      keys.push_back(((uint64_t)a << 32) | b);
+    }
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  std::vector<int2> edges(keys.size());
+  for (size_t i = 0; i < keys.size(); ++i) edges[i] = make_int2((int)(keys[i] >> 32), (int)(keys[i] & 0xffffffffu));
+  sc->E = (int)edges.size();
+  std::vector<int32_t> row_of(c.V, -1);
+  for (int r = 0; r < c.R; ++r) row_of[c.row_vertex[r]] = r;
+  const int V = c.V, T = c.T, E = sc->E;
+  TRY_RC(grow(&sc->d_edges, E));
+  TRY_RC(grow(&sc->d_row_of, V));
+  for (uint32_t** p : {&sc->d_vkey, &sc->d_vid, &sc->d_vkey_s, &sc->d_vid_s}) TRY_RC(grow(p, V));
+  for (uint32_t** p : {&sc->d_ekey, &sc->d_eid, &sc->d_ekey_s, &sc->d_eid_s}) TRY_RC(grow(p, E));
+  TRY_RC(grow(&sc->d_tcount, T + 1));
+  TRY_RC(grow(&sc->d_toff, T + 1));
+  TRY_RC(grow(&sc->d_ecount, E + 1));
+  TRY_RC(grow(&sc->d_eoff, E + 1));
+  TRY_RC(grow(&sc->d_part, 2 * (size_t)blocks_for(std::max(V, E))));
+  TRY_RC(grow(&sc->d_grid, 2));
+  TRY_RC(grow(&sc->d_iptr, (size_t)c.R + 1));
+  TRY_RC(grow(&sc->d_eout, 2));
+  HIP_TRY(hipHostMalloc((void**)&sc->h_totals, 4 * sizeof(uint32_t)));
+  // scratch for the largest sort and scans
+  size_t need = 0, b = 0;
+  const int most = std::max(std::max(V, E), T + 1);
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, most));
+  need = std::max(need, b);
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr, most));
+  need = std::max(need, b);
+  sc->tmp_bytes = need;
+  HIP_TRY(hipMalloc(&sc->d_tmp, std::max<size_t>(need, 1)));
+  HIP_TRY(hipMemcpyAsync(sc->d_edges, edges.data(), (size_t)E * sizeof(int2), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(sc->d_row_of, row_of.data(), (size_t)V * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return DRAPE_OK;
+}
+
+int ensure_tmp(SelfContact* sc, size_t bytes) {
+  if (bytes <= sc->tmp_bytes) return DRAPE_OK;
+  HIP_TRY(hipFree(sc->d_tmp));
+  sc->d_tmp = nullptr;
+  HIP_TRY(hipMalloc(&sc->d_tmp, bytes));
+  sc->tmp_bytes = bytes;
+  return DRAPE_OK;
+}
+
+int sort_pairs(SelfContact* sc, const uint32_t* k_in, uint32_t* k_out, const uint32_t* v_in, uint32_t* v_out, int n,
+               hipStream_t s) {
+  if (n <= 0) return DRAPE_OK;
+  size_t b = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, k_in, k_out, v_in, v_out, n, 0, 32, s));
+  TRY_RC(ensure_tmp(sc, b));
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(sc->d_tmp, b, k_in, k_out, v_in, v_out, n, 0, 32, s));
+  return DRAPE_OK;
+}
+
+int scan(SelfContact* sc, const uint32_t* in, uint32_t* out, int n, hipStream_t s) {
+  size_t b = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, s));
+  TRY_RC(ensure_tmp(sc, b));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(sc->d_tmp, b, in, out, n, s));
+  return DRAPE_OK;
+}
+
+// Candidates at the current positions closer than dhat, or (dir not null) than dhat + 2 |dir|max (rounded
+// up), into `set`.
+int broad_phase(Context& c, const float4* dir, SelfContact::Set& set) {
+  SelfContact* sc = c.self_contact;
+  const double limit = dir ? 0.0 : sc->dhat;
+  hipStream_t s = c.stream;
+  const int V = sc->V, T = sc->T, E = sc->E;
+  const int groups = blocks_for(std::max(V, E));
+  k_extent<<<groups, 256, 0, s>>>(sc->d_edges, E, c.pos, dir, sc->d_row_of, V, sc->d_part);
+  k_grid<<<1, 256, 0, s>>>(sc->d_part, groups, (float)sc->dhat, sc->d_grid);
+  k_vertex_keys<<<blocks_for(V), 256, 0, s>>>(c.pos, V, sc->d_grid, sc->d_vkey, sc->d_vid);
+  k_edge_keys<<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, sc->d_ekey, sc->d_eid);
+  HIP_TRY(hipGetLastError());
+  TRY_RC(sort_pairs(sc, sc->d_vkey, sc->d_vkey_s, sc->d_vid, sc->d_vid_s, V, s));
+  TRY_RC(sort_pairs(sc, sc->d_ekey, sc->d_ekey_s, sc->d_eid, sc->d_eid_s, E, s));
+  k_pt<0><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->d_vkey_s, sc->d_vid_s, sc->d_tcount,
+                                        nullptr, nullptr);
+  k_ee<0><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->d_ekey_s, sc->d_eid_s, sc->d_ecount,
+                                        nullptr, nullptr);
+  HIP_TRY(hipGetLastError());
+  // totals: the scans' last entries (the counts' last slots are zero)
+  HIP_TRY(hipMemsetAsync(sc->d_tcount + T, 0, sizeof(uint32_t), s));
+  HIP_TRY(hipMemsetAsync(sc->d_ecount + E, 0, sizeof(uint32_t), s));
+  TRY_RC(scan(sc, sc->d_tcount, sc->d_toff, T + 1, s));
+  TRY_RC(scan(sc, sc->d_ecount, sc->d_eoff, E + 1, s));
+  HIP_TRY(hipMemcpyAsync(sc->h_totals, sc->d_toff + T, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(sc->h_totals + 1, sc->d_eoff + E, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c.read_bytes += 2 * sizeof(uint32_t);
+  const int npt = (int)sc->h_totals[0], nee = (int)sc->h_totals[1], n = npt + nee;
+  if (n > set.cap) {
+    const int cap = std::max(n, 2 * set.cap);
+    TRY_RC(grow(&set.d_cand, cap));
+    set.cap = cap;
+  }
+  set.npt = npt;
+  set.nee = nee;
+  if (npt > 0)
+    k_pt<1><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->d_vkey_s, sc->d_vid_s, nullptr,
+                                          sc->d_toff, set.d_cand);
+  if (nee > 0)
+    k_ee<1><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->d_ekey_s, sc->d_eid_s, nullptr,
+                                          sc->d_eoff, set.d_cand + npt);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+// Term arrays for n pairs and partial buffers for n candidates.
+int ensure_terms(SelfContact* sc, int n) {
+  if (n > sc->tcap) {
+    const int cap = std::max(n, 2 * sc->tcap);
+    TRY_RC(grow(&sc->d_crow, cap));
+    TRY_RC(grow(&sc->d_cdat, cap));
+    TRY_RC(grow(&sc->d_ccoef, cap));
+    TRY_RC(grow(&sc->d_cfirst, cap));
+    TRY_RC(grow(&sc->d_dist, cap));
+    for (uint32_t** p : {&sc->d_ikey, &sc->d_ival, &sc->d_ikey_s, &sc->d_ival_s}) TRY_RC(grow(p, 4 * (size_t)cap));
+    sc->tcap = cap;
+  }
+  return DRAPE_OK;
+}
+
+int ensure_parts(SelfContact* sc, int n) {
+  const int parts = blocks_for(n);
+  if (parts > sc->parts) {
+    TRY_RC(grow(&sc->d_epart, 2 * (size_t)parts));
+    TRY_RC(grow(&sc->d_apart, parts));
+    TRY_RC(grow(&sc->d_aidx, parts));
+    sc->parts = parts;
+  }
+  return DRAPE_OK;
+}
+
+}  // namespace
+
+int self_contact_evaluate(Context& c) {
+  if (!c.rules.self_contact) return DRAPE_OK;
+  TRY_RC(setup(c));
+  SelfContact* sc = c.self_contact;
+  hipStream_t s = c.stream;
+  sc->step_ready = false;
+  TRY_RC(broad_phase(c, nullptr, sc->eval));
+  const int n = sc->eval.npt + sc->eval.nee;
+  sc->terms = true;
+  if (n == 0) return DRAPE_OK;
+  TRY_RC(ensure_terms(sc, n));
+  k_pair_terms<<<blocks_for(n), 256, 0, s>>>(sc->eval.d_cand, n, sc->eval.npt, c.pos, sc->d_row_of, sc->dhat,
+                                             sc->kappa, sc->d_crow, sc->d_cdat, sc->d_ccoef, sc->d_cfirst,
+                                             sc->d_dist, sc->d_ikey, sc->d_ival);
+  HIP_TRY(hipGetLastError());
+  TRY_RC(sort_pairs(sc, sc->d_ikey, sc->d_ikey_s, sc->d_ival, sc->d_ival_s, 4 * n, s));
+  k_row_ptr<<<blocks_for(c.R + 1), 256, 0, s>>>(sc->d_ikey_s, 4u * (uint32_t)n, c.R, sc->d_iptr);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+static bool has_terms(const Context& c) {
+  const SelfContact* sc = c.self_contact;
+  return c.rules.self_contact && sc && sc->terms && sc->eval.npt + sc->eval.nee > 0;
+}
+
+int self_contact_assemble(Context& c) {
+  if (!has_terms(c)) return DRAPE_OK;
+  SelfContact* sc = c.self_contact;
+  k_contact_assemble<<<blocks_for(c.R), 256, 0, c.stream>>>(sc->d_iptr, sc->d_ival_s, sc->d_cdat, sc->d_ccoef,
+                                                             sc->d_cfirst, c.diag_slot, c.R, c.sell_val, c.b);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+void self_contact_view(const Context& c, const uint32_t** cptr, const uint32_t** cinc, const int4** crow,
+                       const float4** cdat, const float4** ccoef) {
+  if (!has_terms(c)) return;
+  const SelfContact* sc = c.self_contact;
+  *cptr = sc->d_iptr;
+  *cinc = sc->d_ival_s;
+  *crow = sc->d_crow;
+  *cdat = sc->d_cdat;
+  *ccoef = sc->d_ccoef;
+}
+
+void self_contact_moved(Context& c) {
+  if (c.self_contact) c.self_contact->step_ready = false;
+}
+
+static int step_set(Context& c) {
+  SelfContact* sc = c.self_contact;
+  if (!sc->step_ready) {
+    TRY_RC(broad_phase(c, c.dir, sc->step));
+    TRY_RC(ensure_parts(sc, sc->step.npt + sc->step.nee));
+    sc->step_ready = true;
+  }
+  return DRAPE_OK;
+}
+
+int self_contact_step_bound(Context& c, float* alpha, int32_t* row) {
+  *alpha = 1.0f;
+  *row = -1;
+  if (!c.rules.self_contact) return DRAPE_OK;
+  TRY_RC(setup(c));
+  SelfContact* sc = c.self_contact;
+  hipStream_t s = c.stream;
+  sc->step_ready = false;  // a new direction: a new line-search set
+  TRY_RC(step_set(c));
+  const int n = sc->step.npt + sc->step.nee;
+  if (n == 0) return DRAPE_OK;
+  const int groups = blocks_for(n);
+  k_contact_ccd<<<groups, 256, 0, s>>>(sc->step.d_cand, n, sc->step.npt, c.pos, c.dir, sc->d_row_of, sc->d_apart,
+                                       sc->d_aidx);
+  k_min_final<<<1, 256, 0, s>>>(sc->d_apart, sc->d_aidx, groups, sc->step.d_cand, sc->d_row_of, sc->d_eout);
+  HIP_TRY(hipGetLastError());
+  double out[2];
+  HIP_TRY(hipMemcpyAsync(out, sc->d_eout, sizeof(out), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c.read_bytes += sizeof(out);
+  *alpha = (float)out[0];
+  *row = (int32_t)out[1];
+  return DRAPE_OK;
+}
+
+int self_contact_energy_change(Context& c, float alpha, double* sum, double* abs_sum) {
+  *sum = 0.0;
+  *abs_sum = 0.0;
+  if (!c.rules.self_contact) return DRAPE_OK;
+  TRY_RC(setup(c));
+  SelfContact* sc = c.self_contact;
+  hipStream_t s = c.stream;
+  TRY_RC(step_set(c));
+  const int n = sc->step.npt + sc->step.nee;
+  if (n == 0) return DRAPE_OK;
+  const int groups = blocks_for(n);
+  k_contact_energy<<<groups, 256, 0, s>>>(sc->step.d_cand, n, sc->step.npt, c.pos, c.dir, sc->d_row_of, alpha,
+                                          sc->dhat, sc->kappa, sc->d_epart);
+  k_sum_pairs64<<<1, 256, 0, s>>>(sc->d_epart, groups, sc->d_eout);
+  HIP_TRY(hipGetLastError());
+  double out[2];
+  HIP_TRY(hipMemcpyAsync(out, sc->d_eout, sizeof(out), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c.read_bytes += sizeof(out);
+  *sum = out[0];
+  *abs_sum = out[1];
+  return DRAPE_OK;
+}
+
+int self_contact_pairs(Context& c, int32_t* n_pt, int32_t* n_ee, int32_t* vertices, double* distances) {
+  const SelfContact* sc = c.self_contact;
+  const bool have = c.rules.self_contact && sc && sc->terms;
+  const int npt = have ? sc->eval.npt : 0, nee = have ? sc->eval.nee : 0;
+  if (n_pt) *n_pt = npt;
+  if (n_ee) *n_ee = nee;
+  const int n = npt + nee;
+  if (n == 0) return DRAPE_OK;
+  if (vertices)
+    HIP_TRY(hipMemcpyAsync(vertices, sc->eval.d_cand, (size_t)n * sizeof(int4), hipMemcpyDeviceToHost, c.stream));
+  if (distances)
+    HIP_TRY(hipMemcpyAsync(distances, sc->d_dist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  return DRAPE_OK;
+}
+
+}  // namespace drape

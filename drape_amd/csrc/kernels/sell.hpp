@@ -13,6 +13,14 @@ struct SellView {
   const float* val;           // [slots * 9 * 64] entry e of (slot q, lane l) at (9q + e) * 64 + l
   const uint32_t* cnt;        // blocks of the row at each physical position (0 past the last row)
   const uint32_t* perm;       // physical position -> logical row (same 256-row group); ~0u past the last row
+  // self-contact coupling outside the block pattern (null: none). Pair c has rows crow[c] (-1: pinned vertex)
+  // and Hessian b'' (c_k n)(c_l n)^T in block (k, l), cdat[c] = (n, b'') and ccoef[c] = c_0..c_3; the rows'
+  // pair slots are cinc[cptr[row] .. cptr[row + 1]) = 4 pair + k.
+  const uint32_t* cptr = nullptr;
+  const uint32_t* cinc = nullptr;
+  const int4* crow = nullptr;
+  const float4* cdat = nullptr;
+  const float4* ccoef = nullptr;
 };
 
 // Blocks of a row processed per chunk: every load of a chunk is issued before the first use, so a row
@@ -68,6 +76,73 @@ __device__ inline float3 row_times(const SellView& A, int phys, const float4* __
     }
   }
   return make_float3(a0, a1, a2);
+}
+
+__device__ inline float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+
+// The self-contact coupling of a logical row: for each contact pair it belongs to (slot k), in the row's
+// list order, b'' c_k n (sum over the pair's other free vertices l of c_l n.w_l), w = p or, with DIR,
+// z + beta p. The diagonal blocks b'' c_k^2 n n^T are part of the assembled matrix.
+__device__ inline float coef_of(const float4& c, int k) { return k == 0 ? c.x : k == 1 ? c.y : k == 2 ? c.z : c.w; }
+__device__ inline int row_of(const int4& r, int k) { return k == 0 ? r.x : k == 1 ? r.y : k == 2 ? r.z : r.w; }
+
+template <bool DIR>
+__device__ inline float3 contact_times(const SellView& A, uint32_t row, const float4* __restrict__ p,
+                                       const float4* __restrict__ z, float beta) {
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  const uint32_t t1 = A.cptr[row + 1];
+  for (uint32_t t = A.cptr[row]; t < t1; ++t) {
+    const uint32_t e = A.cinc[t];
+    const uint32_t pair = e >> 2;
+    const int k = (int)(e & 3u);
+    const int4 rows = A.crow[pair];
+    const float4 d = A.cdat[pair], c = A.ccoef[pair];
+    float s = 0.0f;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int rl = row_of(rows, l);
+      if (l == k || rl < 0) continue;
+      float4 w = p[rl];
+      if (DIR) {
+        const float4 zz = z[rl];
+        w.x = zz.x + beta * w.x;
+        w.y = zz.y + beta * w.y;
+        w.z = zz.z + beta * w.z;
+      }
+      s = s + coef_of(c, l) * (d.x * w.x + d.y * w.y + d.z * w.z);
+    }
+    const float f = d.w * coef_of(c, k) * s;
+    a0 = a0 + f * d.x;
+    a1 = a1 + f * d.y;
+    a2 = a2 + f * d.z;
+  }
+  return make_float3(a0, a1, a2);
+}
+
+// The same in FP64 over an FP64 vector x [3R] (the refinement's true residual).
+__device__ inline void contact_times64(const SellView& A, uint32_t row, const double* __restrict__ x, double out[3]) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  const uint32_t t1 = A.cptr[row + 1];
+  for (uint32_t t = A.cptr[row]; t < t1; ++t) {
+    const uint32_t e = A.cinc[t];
+    const uint32_t pair = e >> 2;
+    const int k = (int)(e & 3u);
+    const int4 rows = A.crow[pair];
+    const float4 d = A.cdat[pair], c = A.ccoef[pair];
+    double s = 0.0;
+    for (int l = 0; l < 4; ++l) {
+      const int rl = row_of(rows, l);
+      if (l == k || rl < 0) continue;
+      s += (double)coef_of(c, l) * ((double)d.x * x[3 * rl] + (double)d.y * x[3 * rl + 1] + (double)d.z * x[3 * rl + 2]);
+    }
+    const double f = (double)d.w * (double)coef_of(c, k) * s;
+    a0 += f * d.x;
+    a1 += f * d.y;
+    a2 += f * d.z;
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
 }
 
 __device__ inline float3 precondition(const float* __restrict__ dinv, int row, float a, float b, float c) {

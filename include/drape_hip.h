@@ -97,6 +97,11 @@ typedef struct {
                              the FP32 copy and still exactly a Gram form, so symmetric positive definite;
                              0 applies the FP32 B = L^-1; 2 (default) chooses FP16 when the FP32 B and B^T
                              together exceed 128 MiB (3 Nc above about 4,000: n257 and larger) */
+  int32_t self_contact;   /* 1 adds the sheet's self-contact: the IPC barrier b(d) of every point-triangle and
+                             edge-edge pair closer than dhat (the colliders' dhat and kappa), found by a
+                             spatial-hash broad phase, with its Gauss-Newton Hessian b'' grad d grad d^T; the
+                             step bound then also keeps every such pair above 0.1 of its distance. No reference
+                             counterpart (the reference's contact is vertex-collider only). Default 0 */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0
@@ -311,6 +316,12 @@ int drape_training_vectors_make(drape_ctx* ctx, int32_t count, int32_t sweeps, f
  * factorisation. `params` becomes the context's prolongation model (as drape_set_prolongation_model).
  * DRAPE_E_INVALID when no vectors are set, `count` is not the model's or A_c is not positive definite. */
 int drape_prolongation_gradient(drape_ctx* ctx, const float* params, int32_t count, double* loss, float* grad);
+
+/* The self-contact pairs of the last evaluation under rules.self_contact (no reference counterpart): the
+ * numbers of point-triangle and edge-edge pairs closer than dhat at the evaluated positions, their vertices
+ * [4 (n_pt + n_ee)] (the point, then the triangle's three; or the first edge's two, then the second's) and
+ * their FP64 distances. Any output may be NULL (a first call with NULL arrays gives the counts). */
+int drape_self_contact_pairs(drape_ctx* ctx, int32_t* n_pt, int32_t* n_ee, int32_t* vertices, double* distances);
 
 /* The Galerkin coarse operator A_c = P^T A P of the assembled system, row-major [3Nc x 3Nc] doubles (Nc from
  * drape_two_level_layout), for checking; the preconditioner's factorisation is left as it is. */

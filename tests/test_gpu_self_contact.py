@@ -1,0 +1,189 @@
+"""GPU tests of the sheet's self-contact (drape_amd/csrc/kernels/selfcontact.hip, rules.self_contact) through the
+C ABI, against the f64 oracle (oracle/selfcontact.py, pinned by dense sampling and finite differences) on a
+folded n17 sheet whose upper half lies over its lower half:
+  - the point-triangle and edge-edge pairs closer than dhat equal the oracle's brute-force sets, with the
+    same distances (1e-12 relative);
+  - the assembled right-hand side changes by the oracle's barrier gradient (FP32: 2e-5 of its largest entry);
+  - PCG solves the full operator, the contact coupling outside the block pattern included: the true residual
+    of the oracle's matrix (assembled blocks + Gauss-Newton couplings) is at the tolerance;
+  - the step bound keeps every pair that can come within dhat above 0.1 of its distance, and the line
+    search's energy change includes the pairs' barrier change (1e-6 relative);
+  - dynamics: under gravity the upper half comes to rest on the lower half with self-contact and never
+    crosses it, while without self-contact it falls through to the floor.
+With rules.self_contact = 0 (the default) every other test runs the unchanged path."""
+import numpy as np
+import pytest
+
+import drape_amd as D
+from oracle import selfcontact as S
+
+pytestmark = pytest.mark.gpu
+
+N = 17
+DHAT = 1e-3
+
+
+def folded(gap):
+    """Positions [V, 3] of the folded sheet: lower half resting at dhat/2 above the floor (z = 0), rows past
+    the fold mirrored onto it `gap` higher; and the scene (floor collider, negligible bending)."""
+    scene = D.make_scene(N, pinned=False, colliders=[(1, 0.0, 0.0, 1.0, 0.0)], dhat=DHAT, kappa=0.0, lift=0.0,
+                         bend=(1e-9, 1e-9), time_step=1.0 / 240.0)
+    arrays = D.SceneArrays(scene)
+    x = arrays.array("rest_positions").astype(np.float64).reshape(-1, 3).copy()
+    L = x[:, 1].max()
+    top = x[:, 1] > 0.5 * L + 1e-9
+    x[:, 2] = 0.5 * DHAT
+    x[top, 1] = L - x[top, 1]
+    x[top, 2] += gap
+    return arrays, x.astype(np.float32).astype(np.float64), top
+
+
+def context(arrays, on):
+    return D.Drape(arrays, D.default_rules(self_contact=1 if on else 0))
+
+
+def assembled(arrays, x, on, h=1.0 / 240.0):
+    ctx = context(arrays, on)
+    xf = x.astype(np.float32).ravel()
+    ctx.set_state(xf, xf, h, 0.0)
+    ctx.residual()
+    return ctx
+
+
+def kappa_of(arrays):
+    return float(arrays.array("colliders")[2])
+
+
+def tri_of(arrays):
+    return arrays.array("elem_vertices").reshape(-1, 3)
+
+
+def test_pairs_match_brute_force():
+    arrays, x, _ = folded(0.5 * DHAT)
+    ctx = assembled(arrays, x, True)
+    npt, verts, dist = ctx.self_contact_pairs()
+    quads, d_ref, kinds = S.candidates(x, tri_of(arrays), DHAT)
+    print(f"[gpu] self-contact pairs: {npt} point-triangle, {len(verts) - npt} edge-edge "
+          f"(oracle {int((kinds == 0).sum())}, {int((kinds == 1).sum())})", flush=True)
+    got = {(0 if i < npt else 1, *map(int, v)): d for i, (v, d) in enumerate(zip(verts, dist))}
+    want = {(int(k), *map(int, q)): d for q, d, k in zip(quads, d_ref, kinds)}
+    # a pair within 1e-12 of dhat may fall either side
+    edge = {k for k, d in want.items() if abs(d - DHAT) < 1e-12 * DHAT}
+    assert set(got) - edge == set(want) - edge
+    err = max(abs(got[k] - want[k]) / want[k] for k in got)
+    assert err <= 1e-12
+    assert len(got) > 100
+    ctx.close()
+
+
+def test_rhs_change_is_the_barrier_gradient():
+    arrays, x, _ = folded(0.5 * DHAT)
+    on, off = assembled(arrays, x, True), assembled(arrays, x, False)
+    rhs_on, _ = on.system()
+    rhs_off, _ = off.system()
+    quads, _, kinds = S.candidates(x, tri_of(arrays), DHAT)
+    g = S.gradient(x, quads, kinds, DHAT, kappa_of(arrays))
+    rows = arrays.array("row_vertex")
+    want = -g[rows].ravel()
+    got = rhs_on.astype(np.float64) - rhs_off.astype(np.float64)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"[gpu] self-contact gradient: max |g| {np.abs(want).max():.3e}, max error / max {err:.2e}", flush=True)
+    assert err <= 2e-5
+    on.close()
+    off.close()
+
+
+def test_pcg_solves_the_full_operator():
+    arrays, x, _ = folded(0.5 * DHAT)
+    ctx = assembled(arrays, x, True)
+    rhs, vals = ctx.system()
+    st, sol = ctx.solve_once(1e-6, 20000)
+    assert D.PCG_STATUS[st.status] == "converged"
+    R = arrays.R
+    row_ptr, col = arrays.array("row_ptr"), arrays.array("col_idx")
+    A = np.zeros((3 * R, 3 * R))
+    blocks = vals.astype(np.float64).reshape(-1, 3, 3)
+    for i in range(R):
+        for b in range(row_ptr[i], row_ptr[i + 1]):
+            j = col[b]
+            A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = blocks[b]
+    quads, _, kinds = S.candidates(x, tri_of(arrays), DHAT)
+    H = S.gauss_newton(x, quads, kinds, DHAT, kappa_of(arrays), arrays.V)
+    rows = arrays.array("row_vertex")
+    idx = (3 * rows[:, None] + np.arange(3)[None, :]).ravel()
+    Hr = H[np.ix_(idx, idx)]
+    # the assembled diagonal blocks already hold the pairs' diagonal blocks
+    for i in range(R):
+        Hr[3 * i:3 * i + 3, 3 * i:3 * i + 3] = 0.0
+    full = A + Hr
+    b = rhs.astype(np.float64)
+    true_rel = np.linalg.norm(b - full @ sol.astype(np.float64)) / np.linalg.norm(b)
+    without = np.linalg.norm(b - A @ sol.astype(np.float64)) / np.linalg.norm(b)
+    print(f"[gpu] self-contact PCG: {st.iterations} iterations; true relative residual {true_rel:.2e} "
+          f"(against the block pattern alone {without:.2e})", flush=True)
+    assert true_rel <= 3e-6
+    assert without > 100 * true_rel
+    ctx.close()
+
+
+def test_step_bound_and_energy_change():
+    arrays, x, top = folded(2.0 * DHAT)
+    ctx = assembled(arrays, x, True)
+    off = assembled(arrays, x, False)
+    rows = arrays.array("row_vertex")
+    # the upper half moves down by 4 dhat: through the lower half without a bound
+    d = np.zeros((arrays.V, 3))
+    d[top, 2] = -4.0 * DHAT
+    d[:, 0] += 1e-4 * np.sin(np.arange(arrays.V))
+    dr = d[rows].astype(np.float32).ravel()
+    for c in (ctx, off):
+        c.set_direction(dr)
+    alpha, row, _ = ctx.max_step()
+    alpha_off, _, _ = off.max_step()
+    xs = x + alpha * d.astype(np.float32).astype(np.float64)
+    quads, d0, kinds = S.candidates(x, tri_of(arrays), DHAT + 2.0 * np.abs(d).max() * 1.01)
+    d1 = np.array([S.pair_geometry(xs, q, k)[0] for q, k in zip(quads, kinds)])
+    ratio = (d1 / d0).min()
+    print(f"[gpu] self-contact step bound: alpha {alpha:.4f} (colliders alone {alpha_off:.4f}), row {row}, "
+          f"{len(quads)} pairs, min d(x + alpha p)/d(x) {ratio:.3f}", flush=True)
+    # the floor alone allows 0.5625 (2.5 dhat above it, 0.1 kept); the lower half stops the upper half first
+    assert alpha < alpha_off < 1.0
+    assert ratio >= 0.1 * (1 - 1e-6)
+    assert alpha > 0.2  # the pairs start 2 dhat apart and close at 4 dhat per unit step
+    # the line search's energy change adds the pairs' barrier change
+    a = 0.9 * alpha  # the upper half ends 0.38 dhat above the lower
+    s_on, _ = ctx.energy_change(a)
+    s_off, _ = off.energy_change(a)
+    # the trial positions as the device forms them: FP32 pos + alpha dir
+    xa = (x.astype(np.float32) + np.float32(a) * d.astype(np.float32)).astype(np.float64)
+    kap = kappa_of(arrays)
+    want = S.energy(xa, quads, kinds, DHAT, kap) - S.energy(x, quads, kinds, DHAT, kap)
+    assert want > 0
+    assert abs((s_on - s_off) - want) <= 1e-6 * abs(want)
+    ctx.close()
+    off.close()
+
+
+def test_fold_comes_to_rest_without_crossing():
+    arrays, x, top = folded(3.0 * DHAT)
+    tri = tri_of(arrays)
+    gaps = {}
+    for on in (True, False):
+        ctx = context(arrays, on)
+        out = ctx.simulate(24, x0=x.astype(np.float32).ravel())
+        xe = out["positions"].astype(np.float64).reshape(-1, 3)
+        # the vertex of the lower half under each vertex of the upper half (mirror rows)
+        cols = N
+        idx = np.arange(arrays.V)
+        r, c = idx // cols, idx % cols
+        under = (N - 1 - r) * cols + c
+        gaps[on] = (xe[top, 2] - xe[under[top], 2]).min()
+        crossing = S.crossings(xe, tri)
+        print(f"[gpu] fold, self-contact {'on' if on else 'off'}: min height of the upper half over the lower "
+              f"{gaps[on] / DHAT:.3f} dhat, {crossing} edge-triangle crossings, updates "
+              f"{int(out['updates'].sum())}", flush=True)
+        if on:
+            assert crossing == 0
+        ctx.close()
+    assert gaps[True] > 0.05 * DHAT
+    assert gaps[False] < 0.01 * DHAT
