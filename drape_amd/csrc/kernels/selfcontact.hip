@@ -18,8 +18,10 @@
 // k and the Gauss-Newton block b''(d) c_k c_l n n^T to the Hessian (positive semidefinite).
 //
 // Step bound: conservative advancement per pair (Mirtich 1996; as in additive CCD, Li et al. 2021): the
-// distance of a pair moves at most lp t along t -> x + t d, lp = the largest displacement on each primitive
-// summed, so steps of 0.9 (d - g)/lp keep it above g = 0.1 d(x), the reference's collider rule.
+// distance of a pair shrinks at most lp t along t -> x + t d, lp = the largest displacement of a vertex of one
+// primitive relative to a vertex of the other, so steps of 0.9 (d - g)/lp keep it above g = 0.1 d(x), the
+// reference's collider rule. The line-search set holds the pairs closer than dhat + lp; the grid covers them
+// with an inflation of dhat + 2 max |displacement - mean displacement|.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -42,8 +44,8 @@ struct SelfContact {
   uint32_t *d_ekey = nullptr, *d_eid = nullptr, *d_ekey_s = nullptr, *d_eid_s = nullptr;  // [E]
   uint32_t *d_tcount = nullptr, *d_toff = nullptr;  // [T + 1]
   uint32_t *d_ecount = nullptr, *d_eoff = nullptr;  // [E + 1]
-  float* d_part = nullptr;       // [2 * groups] block maxima (edge length, displacement)
-  float* d_grid = nullptr;       // [2] cell size, inflation
+  float* d_part = nullptr;       // [5 * groups] block edge maxima and displacement sums, then deviations
+  float* d_grid = nullptr;       // [5] cell size, inflation, mean displacement
   void* d_tmp = nullptr;         // hipCUB scratch
   size_t tmp_bytes = 0;
   uint32_t* h_totals = nullptr;  // pinned [4]
@@ -211,13 +213,30 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32
 
 // ---- broad phase -------------------------------------------------------------------------------
 
-// block maxima of the edge lengths and of the displacement |dir| (dir may be null)
+// Displacement of vertex v along the step (free vertices only; pinned ones stay).
+__device__ inline P3 disp(const float4* __restrict__ dir, const int32_t* __restrict__ row_of, int v) {
+  return row_of[v] >= 0 ? p3(dir[v]) : P3{0.0, 0.0, 0.0};
+}
+__device__ inline double dist3(P3 a, P3 b) { return sqrt(dot(sub(a, b), sub(a, b))); }
+
+// The most a pair's distance can shrink along x + t dir, t in [0, 1]: the largest relative displacement of a
+// vertex of one primitive against one of the other (the closest points are convex combinations).
+__device__ inline double approach(const float4* dir, const int32_t* row_of, bool pt, int4 v) {
+  const P3 a0 = disp(dir, row_of, v.x), a1 = disp(dir, row_of, v.y), b0 = disp(dir, row_of, v.z),
+           b1 = disp(dir, row_of, v.w);
+  if (pt) return fmax(dist3(a0, a1), fmax(dist3(a0, b0), dist3(a0, b1)));
+  return fmax(fmax(dist3(a0, b0), dist3(a0, b1)), fmax(dist3(a1, b0), dist3(a1, b1)));
+}
+
+// block maxima of the edge lengths and block sums of the free vertices' displacements (dir may be null)
 __global__ void __launch_bounds__(256) k_extent(const int2* __restrict__ edges, int E, const float4* __restrict__ pos,
                                                 const float4* __restrict__ dir, const int32_t* __restrict__ row_of,
                                                 int V, float* __restrict__ part) {
-  __shared__ float la[256], lb[256];
+  __shared__ float la[256];
+  __shared__ float4 ls[256];
   const int i = blockIdx.x * 256 + threadIdx.x;
-  float len = 0.0f, mv = 0.0f;
+  float len = 0.0f;
+  float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (i < E) {
     const int2 e = edges[i];
     const float4 a = pos[e.x], b = pos[e.y];
@@ -226,47 +245,95 @@ __global__ void __launch_bounds__(256) k_extent(const int2* __restrict__ edges, 
   }
   if (dir && i < V && row_of[i] >= 0) {
     const float4 d = dir[i];
-    mv = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    sum = make_float4(d.x, d.y, d.z, 1.0f);
   }
   la[threadIdx.x] = len;
-  lb[threadIdx.x] = mv;
+  ls[threadIdx.x] = sum;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       la[threadIdx.x] = fmaxf(la[threadIdx.x], la[threadIdx.x + s]);
-      lb[threadIdx.x] = fmaxf(lb[threadIdx.x], lb[threadIdx.x + s]);
+      const float4 u = ls[threadIdx.x], w = ls[threadIdx.x + s];
+      ls[threadIdx.x] = make_float4(u.x + w.x, u.y + w.y, u.z + w.z, u.w + w.w);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = la[0];
-    part[2 * blockIdx.x + 1] = lb[0];
+    part[5 * blockIdx.x] = la[0];
+    part[5 * blockIdx.x + 1] = ls[0].x;
+    part[5 * blockIdx.x + 2] = ls[0].y;
+    part[5 * blockIdx.x + 3] = ls[0].z;
+    part[5 * blockIdx.x + 4] = ls[0].w;
   }
 }
 
-// grid[0] = cell size = longest edge + inflation, grid[1] = inflation = dhat + 2 |dir|max (the most two
-// primitives can approach along the step), both rounded up slightly
-__global__ void k_grid(const float* __restrict__ part, int groups, float dhat, float* __restrict__ grid) {
-  __shared__ float la[256], lb[256];
-  float a = 0.0f, b = 0.0f;
+// grid[0] = longest edge, grid[2..4] = mean displacement of the free vertices (fixed-order sums)
+__global__ void k_mean(const float* __restrict__ part, int groups, float* __restrict__ grid) {
+  __shared__ float la[256];
+  __shared__ float4 ls[256];
+  float a = 0.0f;
+  float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   for (int j = threadIdx.x; j < groups; j += 256) {
-    a = fmaxf(a, part[2 * j]);
-    b = fmaxf(b, part[2 * j + 1]);
+    a = fmaxf(a, part[5 * j]);
+    u = make_float4(u.x + part[5 * j + 1], u.y + part[5 * j + 2], u.z + part[5 * j + 3], u.w + part[5 * j + 4]);
   }
   la[threadIdx.x] = a;
-  lb[threadIdx.x] = b;
+  ls[threadIdx.x] = u;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       la[threadIdx.x] = fmaxf(la[threadIdx.x], la[threadIdx.x + s]);
-      lb[threadIdx.x] = fmaxf(lb[threadIdx.x], lb[threadIdx.x + s]);
+      const float4 p = ls[threadIdx.x], q = ls[threadIdx.x + s];
+      ls[threadIdx.x] = make_float4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w);
     }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float n = fmaxf(ls[0].w, 1.0f);
+    grid[0] = la[0];
+    grid[2] = ls[0].x / n;
+    grid[3] = ls[0].y / n;
+    grid[4] = ls[0].z / n;
+  }
+}
+
+// block maxima of |displacement - mean| over the free vertices
+__global__ void __launch_bounds__(256) k_deviation(const float4* __restrict__ dir, const int32_t* __restrict__ row_of,
+                                                   int V, const float* __restrict__ grid, float* __restrict__ part) {
+  __shared__ float lb[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  float m = 0.0f;
+  if (i < V && row_of[i] >= 0) {
+    const float4 d = dir[i];
+    const float dx = d.x - grid[2], dy = d.y - grid[3], dz = d.z - grid[4];
+    m = sqrtf(dx * dx + dy * dy + dz * dz);
+  }
+  lb[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) lb[threadIdx.x] = fmaxf(lb[threadIdx.x], lb[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[5 * blockIdx.x + 1] = lb[0];
+}
+
+// grid[1] = inflation = dhat + 2 max |displacement - mean| (any two vertices' relative displacement is at
+// most that; 0 without a step), grid[0] = cell size = longest edge + inflation; both rounded up slightly
+__global__ void k_grid(const float* __restrict__ part, int groups, int step, float dhat, float* __restrict__ grid) {
+  __shared__ float lb[256];
+  float b = 0.0f;
+  if (step)
+    for (int j = threadIdx.x; j < groups; j += 256) b = fmaxf(b, part[5 * j + 1]);
+  lb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) lb[threadIdx.x] = fmaxf(lb[threadIdx.x], lb[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     const float infl = (dhat + 2.0f * lb[0]) * 1.0001f;
     grid[1] = infl;
-    grid[0] = (la[0] + infl) * 1.0001f;
+    grid[0] = (grid[0] + infl) * 1.0001f;
   }
 }
 
@@ -296,7 +363,8 @@ __global__ void __launch_bounds__(256) k_edge_keys(const int2* __restrict__ edge
 // point-triangle candidates of triangle t: EMIT 0 counts, 1 writes at out + offset[t]
 template <int EMIT>
 __global__ void __launch_bounds__(256) k_pt(const int4* __restrict__ tri, int T, const float4* __restrict__ pos,
-                                            int V, const float* __restrict__ grid, double limit,
+                                            int V, const float* __restrict__ grid, double limit, double dhat,
+                                            const float4* __restrict__ dir, const int32_t* __restrict__ row_of,
                                             const uint32_t* __restrict__ vkey, const uint32_t* __restrict__ vid,
                                             uint32_t* __restrict__ count, const uint32_t* __restrict__ offset,
                                             int4* __restrict__ out) {
@@ -306,6 +374,7 @@ __global__ void __launch_bounds__(256) k_pt(const int4* __restrict__ tri, int T,
   const int4 tv = tri[t];
   const P3 a = p3(pos[tv.x]), b = p3(pos[tv.y]), c = p3(pos[tv.z]);
   const P3 cen = {(a.x + b.x + c.x) / 3.0, (a.y + b.y + c.y) / 3.0, (a.z + b.z + c.z) / 3.0};
+  const double reach = fmax(dist3(a, cen), fmax(dist3(b, cen), dist3(c, cen)));
   const int3 cc = cell_of(cen, h);
   uint32_t n = 0, at = EMIT ? offset[t] : 0;
   for (int dz = -1; dz <= 1; ++dz)
@@ -318,8 +387,12 @@ __global__ void __launch_bounds__(256) k_pt(const int4* __restrict__ tri, int T,
           if (v == tv.x || v == tv.y || v == tv.z) continue;
           const P3 x = p3(pos[v]);
           if (!same_cell(cell_of(x, h), q)) continue;
-          if (pair_geometry(true, x, a, b, c).d >= infl) continue;
-          if (EMIT) out[at++] = make_int4(v, tv.x, tv.y, tv.z);
+          if (dist3(x, cen) - reach >= infl) continue;  // no closer than the bound on any pair
+          const int4 quad = make_int4(v, tv.x, tv.y, tv.z);
+          // along a step: the pairs that may come within dhat
+          const double lim = dir ? (dhat + approach(dir, row_of, true, quad)) * (1.0 + 1e-9) : infl;
+          if (pair_geometry(true, x, a, b, c).d >= lim) continue;
+          if (EMIT) out[at++] = quad;
           ++n;
         }
       }
@@ -329,7 +402,8 @@ __global__ void __launch_bounds__(256) k_pt(const int4* __restrict__ tri, int T,
 // edge-edge candidates (e, f) with f > e and no shared vertex
 template <int EMIT>
 __global__ void __launch_bounds__(256) k_ee(const int2* __restrict__ edges, int E, const float4* __restrict__ pos,
-                                            const float* __restrict__ grid, double limit,
+                                            const float* __restrict__ grid, double limit, double dhat,
+                                            const float4* __restrict__ dir, const int32_t* __restrict__ row_of,
                                             const uint32_t* __restrict__ ekey,
                                             const uint32_t* __restrict__ eid, uint32_t* __restrict__ count,
                                             const uint32_t* __restrict__ offset, int4* __restrict__ out) {
@@ -338,7 +412,9 @@ __global__ void __launch_bounds__(256) k_ee(const int2* __restrict__ edges, int 
   const double h = grid[0], infl = limit > 0.0 ? limit : grid[1];
   const int2 ev = edges[e];
   const P3 p1 = p3(pos[ev.x]), q1 = p3(pos[ev.y]);
-  const int3 cc = cell_of(midpoint(pos, ev), h);
+  const P3 m1 = midpoint(pos, ev);
+  const double half1 = 0.5 * dist3(p1, q1);
+  const int3 cc = cell_of(m1, h);
   uint32_t n = 0, at = EMIT ? offset[e] : 0;
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
@@ -350,9 +426,14 @@ __global__ void __launch_bounds__(256) k_ee(const int2* __restrict__ edges, int 
           if (f <= e) continue;
           const int2 fv = edges[f];
           if (fv.x == ev.x || fv.x == ev.y || fv.y == ev.x || fv.y == ev.y) continue;
-          if (!same_cell(cell_of(midpoint(pos, fv), h), q)) continue;
-          if (pair_geometry(false, p1, q1, p3(pos[fv.x]), p3(pos[fv.y])).d >= infl) continue;
-          if (EMIT) out[at++] = make_int4(ev.x, ev.y, fv.x, fv.y);
+          const P3 m2 = midpoint(pos, fv);
+          if (!same_cell(cell_of(m2, h), q)) continue;
+          const P3 p2 = p3(pos[fv.x]), q2 = p3(pos[fv.y]);
+          if (dist3(m1, m2) - half1 - 0.5 * dist3(p2, q2) >= infl) continue;
+          const int4 quad = make_int4(ev.x, ev.y, fv.x, fv.y);
+          const double lim = dir ? (dhat + approach(dir, row_of, false, quad)) * (1.0 + 1e-9) : infl;
+          if (pair_geometry(false, p1, q1, p2, q2).d >= lim) continue;
+          if (EMIT) out[at++] = quad;
           ++n;
         }
       }
@@ -499,14 +580,12 @@ __global__ void __launch_bounds__(256) k_contact_ccd(const int4* __restrict__ ca
     const bool pt = c < npt;
     const int ids[4] = {v.x, v.y, v.z, v.w};
     P3 x[4], d[4];
-    double m[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       x[k] = p3(pos[ids[k]]);
-      d[k] = row_of[ids[k]] >= 0 ? p3(dir[ids[k]]) : P3{0.0, 0.0, 0.0};
-      m[k] = sqrt(dot(d[k], d[k]));
+      d[k] = disp(dir, row_of, ids[k]);
     }
-    const double lp = pt ? m[0] + fmax(m[1], fmax(m[2], m[3])) : fmax(m[0], m[1]) + fmax(m[2], m[3]);
+    const double lp = approach(dir, row_of, pt, v);
     const double d0 = pair_geometry(pt, x[0], x[1], x[2], x[3]).d;
     if (lp > 0.0 && d0 > 0.0) {
       const double target = 0.1 * d0;
@@ -641,8 +720,8 @@ int setup(Context& c) {
   TRY_RC(grow(&sc->d_toff, T + 1));
   TRY_RC(grow(&sc->d_ecount, E + 1));
   TRY_RC(grow(&sc->d_eoff, E + 1));
-  TRY_RC(grow(&sc->d_part, 2 * (size_t)blocks_for(std::max(V, E))));
-  TRY_RC(grow(&sc->d_grid, 2));
+  TRY_RC(grow(&sc->d_part, 5 * (size_t)blocks_for(std::max(V, E))));
+  TRY_RC(grow(&sc->d_grid, 5));
   TRY_RC(grow(&sc->d_iptr, (size_t)c.R + 1));
   TRY_RC(grow(&sc->d_eout, 2));
   HIP_TRY(hipHostMalloc((void**)&sc->h_totals, 4 * sizeof(uint32_t)));
@@ -698,15 +777,17 @@ int broad_phase(Context& c, const float4* dir, SelfContact::Set& set) {
   const int V = sc->V, T = sc->T, E = sc->E;
   const int groups = blocks_for(std::max(V, E));
   k_extent<<<groups, 256, 0, s>>>(sc->d_edges, E, c.pos, dir, sc->d_row_of, V, sc->d_part);
-  k_grid<<<1, 256, 0, s>>>(sc->d_part, groups, (float)sc->dhat, sc->d_grid);
+  k_mean<<<1, 256, 0, s>>>(sc->d_part, groups, sc->d_grid);
+  if (dir) k_deviation<<<blocks_for(V), 256, 0, s>>>(dir, sc->d_row_of, V, sc->d_grid, sc->d_part);
+  k_grid<<<1, 256, 0, s>>>(sc->d_part, dir ? blocks_for(V) : 0, dir ? 1 : 0, (float)sc->dhat, sc->d_grid);
   k_vertex_keys<<<blocks_for(V), 256, 0, s>>>(c.pos, V, sc->d_grid, sc->d_vkey, sc->d_vid);
   k_edge_keys<<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, sc->d_ekey, sc->d_eid);
   HIP_TRY(hipGetLastError());
   TRY_RC(sort_pairs(sc, sc->d_vkey, sc->d_vkey_s, sc->d_vid, sc->d_vid_s, V, s));
   TRY_RC(sort_pairs(sc, sc->d_ekey, sc->d_ekey_s, sc->d_eid, sc->d_eid_s, E, s));
-  k_pt<0><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->d_vkey_s, sc->d_vid_s, sc->d_tcount,
+  k_pt<0><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_vkey_s, sc->d_vid_s, sc->d_tcount,
                                         nullptr, nullptr);
-  k_ee<0><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->d_ekey_s, sc->d_eid_s, sc->d_ecount,
+  k_ee<0><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_ekey_s, sc->d_eid_s, sc->d_ecount,
                                         nullptr, nullptr);
   HIP_TRY(hipGetLastError());
   // totals: the scans' last entries (the counts' last slots are zero)
@@ -727,10 +808,10 @@ int broad_phase(Context& c, const float4* dir, SelfContact::Set& set) {
   set.npt = npt;
   set.nee = nee;
   if (npt > 0)
-    k_pt<1><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->d_vkey_s, sc->d_vid_s, nullptr,
+    k_pt<1><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_vkey_s, sc->d_vid_s, nullptr,
                                           sc->d_toff, set.d_cand);
   if (nee > 0)
-    k_ee<1><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->d_ekey_s, sc->d_eid_s, nullptr,
+    k_ee<1><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_ekey_s, sc->d_eid_s, nullptr,
                                           sc->d_eoff, set.d_cand + npt);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
