@@ -187,3 +187,22 @@ def test_fold_comes_to_rest_without_crossing():
         ctx.close()
     assert gaps[True] > 0.05 * DHAT
     assert gaps[False] < 0.01 * DHAT
+
+
+def test_fold_equilibrium_by_ptc():
+    """The folded sheet brought to static equilibrium by PTC (solveEquilibrium with self-contact): it stops by
+    tolerance or stagnation with the upper half resting on the lower one and no edge crossing a triangle."""
+    arrays, x, top = folded(3.0 * DHAT)
+    ctx = context(arrays, True)
+    out = ctx.solve_equilibrium(start=x.astype(np.float32).ravel(), cap=200)
+    xe = out["positions"].astype(np.float64).reshape(-1, 3)
+    idx = np.arange(arrays.V)
+    under = (N - 1 - idx // N) * N + idx % N
+    gap = (xe[top, 2] - xe[under[top], 2]).min()
+    crossing = S.crossings(xe, tri_of(arrays))
+    res = out["result"]
+    print(f"[gpu] fold PTC: stop {out['stop']}, {res.accepted_steps} steps, {res.newton_iterations} Newton; min "
+          f"height of the upper half {gap / DHAT:.3f} dhat, {crossing} crossings", flush=True)
+    assert out["stop"] in ("tolerance", "stagnation")
+    assert crossing == 0 and gap > 0.05 * DHAT
+    ctx.close()
