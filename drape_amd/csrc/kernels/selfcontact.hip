@@ -6,9 +6,10 @@
 // 73856093 x ^ 19349663 y ^ 83492791 z) and sorted by key (hipCUB radix sort, stable). A vertex within the
 // inflation distance of a triangle lies within h of its centroid, and two edges within it have midpoints
 // within h, so each triangle queries the vertices, and each edge the edges, of the 27 cells around its
-// centroid (midpoint) cell; the true cell of every visited primitive is checked, so hash collisions neither
-// add nor duplicate pairs. Counting pass, scan, then the emitting pass in the same order: the candidate list
-// is deterministic (point-triangle pairs by triangle, then edge-edge pairs by first edge).
+// centroid (midpoint) cell, one thread per (primitive, cell); the true cell of every visited primitive is
+// checked, so hash collisions neither add nor duplicate pairs. Counting pass, scan, then the emitting pass in
+// the same order: the candidate list is deterministic (point-triangle pairs by triangle and cell, then
+// edge-edge pairs by first edge and cell).
 //
 // Narrow phase: closest points by Voronoi-region classification (point-triangle) and clamped parameters
 // (segment-segment), following Ericson, Real-Time Collision Detection, 5.1.5 and 5.1.9. The distance
@@ -42,8 +43,8 @@ struct SelfContact {
   // broad phase
   uint32_t *d_vkey = nullptr, *d_vid = nullptr, *d_vkey_s = nullptr, *d_vid_s = nullptr;  // [V]
   uint32_t *d_ekey = nullptr, *d_eid = nullptr, *d_ekey_s = nullptr, *d_eid_s = nullptr;  // [E]
-  uint32_t *d_tcount = nullptr, *d_toff = nullptr;  // [T + 1]
-  uint32_t *d_ecount = nullptr, *d_eoff = nullptr;  // [E + 1]
+  uint32_t *d_tcount = nullptr, *d_toff = nullptr;  // [27 T + 1] per (triangle, neighbour cell)
+  uint32_t *d_ecount = nullptr, *d_eoff = nullptr;  // [27 E + 1] per (edge, neighbour cell)
   float* d_part = nullptr;       // [5 * groups] block edge maxima and displacement sums, then deviations
   float* d_grid = nullptr;       // [5] cell size, inflation, mean displacement
   void* d_tmp = nullptr;         // hipCUB scratch
@@ -368,18 +369,18 @@ __global__ void __launch_bounds__(256) k_pt(const int4* __restrict__ tri, int T,
                                             const uint32_t* __restrict__ vkey, const uint32_t* __restrict__ vid,
                                             uint32_t* __restrict__ count, const uint32_t* __restrict__ offset,
                                             int4* __restrict__ out) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= T) return;
+  const int slot = blockIdx.x * 256 + threadIdx.x;  // triangle t, neighbour cell k
+  if (slot >= 27 * T) return;
+  const int t = slot / 27, k = slot % 27;
+  const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
   const double h = grid[0], infl = limit > 0.0 ? limit : grid[1];
   const int4 tv = tri[t];
   const P3 a = p3(pos[tv.x]), b = p3(pos[tv.y]), c = p3(pos[tv.z]);
   const P3 cen = {(a.x + b.x + c.x) / 3.0, (a.y + b.y + c.y) / 3.0, (a.z + b.z + c.z) / 3.0};
   const double reach = fmax(dist3(a, cen), fmax(dist3(b, cen), dist3(c, cen)));
   const int3 cc = cell_of(cen, h);
-  uint32_t n = 0, at = EMIT ? offset[t] : 0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
+  uint32_t n = 0, at = EMIT ? offset[slot] : 0;
+  {
         const int3 q = make_int3(cc.x + dx, cc.y + dy, cc.z + dz);
         const uint32_t key = cell_key(q);
         for (uint32_t i = lower_bound_u32(vkey, V, key); i < (uint32_t)V && vkey[i] == key; ++i) {
@@ -391,12 +392,13 @@ __global__ void __launch_bounds__(256) k_pt(const int4* __restrict__ tri, int T,
           const int4 quad = make_int4(v, tv.x, tv.y, tv.z);
           // along a step: the pairs that may come within dhat
           const double lim = dir ? (dhat + approach(dir, row_of, true, quad)) * (1.0 + 1e-9) : infl;
+          if (dist3(x, cen) - reach >= lim) continue;
           if (pair_geometry(true, x, a, b, c).d >= lim) continue;
           if (EMIT) out[at++] = quad;
           ++n;
         }
       }
-  if (!EMIT) count[t] = n;
+  if (!EMIT) count[slot] = n;
 }
 
 // edge-edge candidates (e, f) with f > e and no shared vertex
@@ -407,18 +409,18 @@ __global__ void __launch_bounds__(256) k_ee(const int2* __restrict__ edges, int 
                                             const uint32_t* __restrict__ ekey,
                                             const uint32_t* __restrict__ eid, uint32_t* __restrict__ count,
                                             const uint32_t* __restrict__ offset, int4* __restrict__ out) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= E) return;
+  const int slot = blockIdx.x * 256 + threadIdx.x;  // edge e, neighbour cell k
+  if (slot >= 27 * E) return;
+  const int e = slot / 27, k = slot % 27;
+  const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
   const double h = grid[0], infl = limit > 0.0 ? limit : grid[1];
   const int2 ev = edges[e];
   const P3 p1 = p3(pos[ev.x]), q1 = p3(pos[ev.y]);
   const P3 m1 = midpoint(pos, ev);
   const double half1 = 0.5 * dist3(p1, q1);
   const int3 cc = cell_of(m1, h);
-  uint32_t n = 0, at = EMIT ? offset[e] : 0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
+  uint32_t n = 0, at = EMIT ? offset[slot] : 0;
+  {
         const int3 q = make_int3(cc.x + dx, cc.y + dy, cc.z + dz);
         const uint32_t key = cell_key(q);
         for (uint32_t i = lower_bound_u32(ekey, E, key); i < (uint32_t)E && ekey[i] == key; ++i) {
@@ -432,12 +434,13 @@ __global__ void __launch_bounds__(256) k_ee(const int2* __restrict__ edges, int 
           if (dist3(m1, m2) - half1 - 0.5 * dist3(p2, q2) >= infl) continue;
           const int4 quad = make_int4(ev.x, ev.y, fv.x, fv.y);
           const double lim = dir ? (dhat + approach(dir, row_of, false, quad)) * (1.0 + 1e-9) : infl;
+          if (dist3(m1, m2) - half1 - 0.5 * dist3(p2, q2) >= lim) continue;
           if (pair_geometry(false, p1, q1, p2, q2).d >= lim) continue;
           if (EMIT) out[at++] = quad;
           ++n;
         }
       }
-  if (!EMIT) count[e] = n;
+  if (!EMIT) count[slot] = n;
 }
 
 // ---- pair terms ----------------------------------------------------------------------------------
@@ -716,10 +719,10 @@ int setup(Context& c) {
   TRY_RC(grow(&sc->d_row_of, V));
   for (uint32_t** p : {&sc->d_vkey, &sc->d_vid, &sc->d_vkey_s, &sc->d_vid_s}) TRY_RC(grow(p, V));
   for (uint32_t** p : {&sc->d_ekey, &sc->d_eid, &sc->d_ekey_s, &sc->d_eid_s}) TRY_RC(grow(p, E));
-  TRY_RC(grow(&sc->d_tcount, T + 1));
-  TRY_RC(grow(&sc->d_toff, T + 1));
-  TRY_RC(grow(&sc->d_ecount, E + 1));
-  TRY_RC(grow(&sc->d_eoff, E + 1));
+  TRY_RC(grow(&sc->d_tcount, 27 * (size_t)T + 1));
+  TRY_RC(grow(&sc->d_toff, 27 * (size_t)T + 1));
+  TRY_RC(grow(&sc->d_ecount, 27 * (size_t)E + 1));
+  TRY_RC(grow(&sc->d_eoff, 27 * (size_t)E + 1));
   TRY_RC(grow(&sc->d_part, 5 * (size_t)blocks_for(std::max(V, E))));
   TRY_RC(grow(&sc->d_grid, 5));
   TRY_RC(grow(&sc->d_iptr, (size_t)c.R + 1));
@@ -727,7 +730,7 @@ int setup(Context& c) {
   HIP_TRY(hipHostMalloc((void**)&sc->h_totals, 4 * sizeof(uint32_t)));
   // scratch for the largest sort and scans
   size_t need = 0, b = 0;
-  const int most = std::max(std::max(V, E), T + 1);
+  const int most = std::max(std::max(V, 27 * E + 1), 27 * T + 1);
   HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                              (uint32_t*)nullptr, most));
   need = std::max(need, b);
@@ -785,18 +788,19 @@ int broad_phase(Context& c, const float4* dir, SelfContact::Set& set) {
   HIP_TRY(hipGetLastError());
   TRY_RC(sort_pairs(sc, sc->d_vkey, sc->d_vkey_s, sc->d_vid, sc->d_vid_s, V, s));
   TRY_RC(sort_pairs(sc, sc->d_ekey, sc->d_ekey_s, sc->d_eid, sc->d_eid_s, E, s));
-  k_pt<0><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_vkey_s, sc->d_vid_s, sc->d_tcount,
+  const int ST = 27 * T, SE = 27 * E;  // (primitive, neighbour cell) slots
+  k_pt<0><<<blocks_for(ST), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_vkey_s, sc->d_vid_s, sc->d_tcount,
                                         nullptr, nullptr);
-  k_ee<0><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_ekey_s, sc->d_eid_s, sc->d_ecount,
+  k_ee<0><<<blocks_for(SE), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_ekey_s, sc->d_eid_s, sc->d_ecount,
                                         nullptr, nullptr);
   HIP_TRY(hipGetLastError());
   // totals: the scans' last entries (the counts' last slots are zero)
-  HIP_TRY(hipMemsetAsync(sc->d_tcount + T, 0, sizeof(uint32_t), s));
-  HIP_TRY(hipMemsetAsync(sc->d_ecount + E, 0, sizeof(uint32_t), s));
-  TRY_RC(scan(sc, sc->d_tcount, sc->d_toff, T + 1, s));
-  TRY_RC(scan(sc, sc->d_ecount, sc->d_eoff, E + 1, s));
-  HIP_TRY(hipMemcpyAsync(sc->h_totals, sc->d_toff + T, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(sc->h_totals + 1, sc->d_eoff + E, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemsetAsync(sc->d_tcount + ST, 0, sizeof(uint32_t), s));
+  HIP_TRY(hipMemsetAsync(sc->d_ecount + SE, 0, sizeof(uint32_t), s));
+  TRY_RC(scan(sc, sc->d_tcount, sc->d_toff, ST + 1, s));
+  TRY_RC(scan(sc, sc->d_ecount, sc->d_eoff, SE + 1, s));
+  HIP_TRY(hipMemcpyAsync(sc->h_totals, sc->d_toff + ST, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(sc->h_totals + 1, sc->d_eoff + SE, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   c.read_bytes += 2 * sizeof(uint32_t);
   const int npt = (int)sc->h_totals[0], nee = (int)sc->h_totals[1], n = npt + nee;
@@ -808,10 +812,10 @@ int broad_phase(Context& c, const float4* dir, SelfContact::Set& set) {
   set.npt = npt;
   set.nee = nee;
   if (npt > 0)
-    k_pt<1><<<blocks_for(T), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_vkey_s, sc->d_vid_s, nullptr,
+    k_pt<1><<<blocks_for(ST), 256, 0, s>>>(c.tri, T, c.pos, V, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_vkey_s, sc->d_vid_s, nullptr,
                                           sc->d_toff, set.d_cand);
   if (nee > 0)
-    k_ee<1><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_ekey_s, sc->d_eid_s, nullptr,
+    k_ee<1><<<blocks_for(SE), 256, 0, s>>>(sc->d_edges, E, c.pos, sc->d_grid, limit, sc->dhat, dir, sc->d_row_of, sc->d_ekey_s, sc->d_eid_s, nullptr,
                                           sc->d_eoff, set.d_cand + npt);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
