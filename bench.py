@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_pcg_spmv, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by the same
 # counters of a known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part); keyed by grid columns.
 # (At n129 the bytes are L2 misses served mostly by the Infinity Cache.)
-SPMV_TRAFFIC = {129: 8_574_432, 1025: 534_700_676}
+SPMV_TRAFFIC = {129: 8_574_432, 1025: 523_831_470}
 
 
 def spmv_bytes(rows: int, blocks: int) -> int:

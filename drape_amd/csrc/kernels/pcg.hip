@@ -433,12 +433,18 @@ __global__ void __launch_bounds__(256) k_vc_post(SellView A, const float4* __res
 // The system's view, with the self-contact coupling when self-contact terms are active. The V-cycle's
 // smoothing residuals use the view without it (the mesh operator with the contact diagonal blocks): a valid
 // symmetric preconditioner of the full operator, which PCG's own products apply.
+static bool streamed(const Context& c) { return c.sell_slots * 64 * 40 > ((size_t)128 << 20); }
 static SellView view_of(Context& c) {
   SellView v{c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm};
+  v.stream = streamed(c);
   self_contact_view(c, &v.cptr, &v.cinc, &v.crow, &v.cdat, &v.ccoef);
   return v;
 }
-static SellView mesh_view_of(Context& c) { return {c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm}; }
+static SellView mesh_view_of(Context& c) {
+  SellView v{c.slice_ptr, c.sell_col, c.sell_val, c.row_cnt, c.perm};
+  v.stream = streamed(c);
+  return v;
+}
 
 // z = M^-1 r by the V-cycle for the current r (after the caller's z1 = w D^-1 r): guard as above.
 static int vcycle_rest(Context& c, float* partials, int guard) {
