@@ -71,6 +71,18 @@ struct TwoLevel {
   _Float16* d_Bh = nullptr;         // [N3 * N3p] FP16 B~ = S^-1 B, S = diag(B) (zero above the diagonal)
   _Float16* d_BTh = nullptr;        // [N3 * N3p] its transpose (the same values)
   bool half_valid = false;          // the FP16 copies belong to the current factorisation
+  // coarse_reuse 4: the factorisation of the step size used before the current one, kept for when the
+  // step size returns to it (PTC alternates between neighbouring step sizes)
+  struct Saved {
+    bool valid = false, half_valid = false;
+    uint64_t factored = 0, factored_shift = 0;
+    double factored_h = 0.0, base_rate = 0.0;
+    int solves = 0;
+    double* linv = nullptr;
+    float *B = nullptr, *BT = nullptr;
+    _Float16 *Bh = nullptr, *BTh = nullptr;
+  } saved;
+  int64_t restores = 0;             // factorisations avoided by returning to the saved one
   float* d_t = nullptr;             // [N3p] B r_c
   float* d_rc = nullptr;            // [N3p]
   float* d_yc = nullptr;            // [N3p]
@@ -85,6 +97,7 @@ void two_level_destroy(TwoLevel* t) {
   for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_wbil, (void*)t->d_fxy, (void*)t->d_theta,
                   (void*)t->d_gnn, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
                   (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT, (void*)t->d_Bh, (void*)t->d_BTh,
+                  (void*)t->saved.linv, (void*)t->saved.B, (void*)t->saved.BT, (void*)t->saved.Bh, (void*)t->saved.BTh,
                   (void*)t->d_t, (void*)t->d_rc, (void*)t->d_yc, (void*)t->d_info})
     if (p) (void)hipFree(p);
   if (t->h_info) (void)hipHostFree(t->h_info);
@@ -1091,6 +1104,56 @@ static int factorise(Context& c, bool model, bool* ok);
 
 int two_level_factor_weights(Context& c, bool* ok) { return factorise(c, false, ok); }
 
+// Exchanges the current factorisation (its inverse factor copies, diagonal-tile inverses and bookkeeping)
+// with the saved one; `keep` marks the outgoing current one as valid. Allocates the saved buffers on first
+// use.
+static int swap_saved(Context& c, bool keep) {
+  TwoLevel* t = c.two_level;
+  TwoLevel::Saved& v = t->saved;
+  if (!v.B) {
+    const size_t nb = (size_t)t->N3 * t->N3p;
+    HIP_TRY(hipMalloc((void**)&v.linv, (size_t)t->nt * TILE * TILE * sizeof(double)));
+    HIP_TRY(hipMalloc((void**)&v.B, nb * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&v.BT, nb * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&v.Bh, nb * sizeof(_Float16)));
+    HIP_TRY(hipMalloc((void**)&v.BTh, nb * sizeof(_Float16)));
+    // the copies are written below/above the diagonal only: the rest stays zero
+    HIP_TRY(hipMemsetAsync(v.B, 0, nb * sizeof(float), c.stream));
+    HIP_TRY(hipMemsetAsync(v.BT, 0, nb * sizeof(float), c.stream));
+    HIP_TRY(hipMemsetAsync(v.Bh, 0, nb * sizeof(_Float16), c.stream));
+    HIP_TRY(hipMemsetAsync(v.BTh, 0, nb * sizeof(_Float16), c.stream));
+  }
+  std::swap(v.linv, t->d_linv);
+  std::swap(v.B, t->d_B);
+  std::swap(v.BT, t->d_BT);
+  std::swap(v.Bh, t->d_Bh);
+  std::swap(v.BTh, t->d_BTh);
+  std::swap(v.half_valid, t->half_valid);
+  const bool was_valid = v.valid;
+  const uint64_t f = v.factored, fs = v.factored_shift;
+  const double fh = v.factored_h, br = v.base_rate;
+  const int so = v.solves;
+  v.valid = keep;
+  v.factored = t->factored;
+  v.factored_shift = t->factored_shift;
+  v.factored_h = t->factored_h;
+  v.base_rate = t->base_rate;
+  v.solves = t->solves;
+  if (was_valid) {
+    t->factored = f;
+    t->factored_shift = fs;
+    t->factored_h = fh;
+    t->base_rate = br;
+    t->solves = so;
+    t->ok = true;
+  } else {
+    t->factored = ~0ull;
+    t->ok = false;
+  }
+  t->refresh = false;
+  return DRAPE_OK;
+}
+
 float* two_level_weights_device(Context& c) { return c.two_level->d_w; }
 
 // A_c = P^T A P of the assembled system, its Cholesky factor and the factor's inverse, unless the current
@@ -1099,12 +1162,25 @@ float* two_level_weights_device(Context& c) { return c.two_level->d_w; }
 int two_level_prepare(Context& c, bool* ok) {
   TwoLevel* t = c.two_level;
   const bool fresh = t->factored == c.system_version;
-  const bool reusable = c.rules.coarse_reuse && t->factored != ~0ull && t->ok && !t->refresh &&
-                        (c.rules.coarse_reuse == 2 || t->factored_shift == c.shift_version ||
-                         (c.rules.coarse_reuse == 3 && t->factored_h == c.shift_h));
+  const int rule = c.rules.coarse_reuse;
+  const bool by_h = rule == 3 || rule == 4;
+  const bool reusable = rule && t->factored != ~0ull && t->ok && !t->refresh &&
+                        (rule == 2 || t->factored_shift == c.shift_version || (by_h && t->factored_h == c.shift_h));
   if (fresh || reusable) {
     *ok = t->ok;
     return DRAPE_OK;
+  }
+  if (rule == 4) {
+    // the step size moved: keep the current factorisation (unless PCG slowed down on it) and return to the
+    // saved one if it belongs to the new step size
+    const bool keep = t->factored != ~0ull && t->ok && !t->refresh && t->factored_h != c.shift_h;
+    const bool back = t->saved.valid && t->saved.factored_h == c.shift_h;
+    if (keep || back) TRY_RC(swap_saved(c, keep));
+    if (back) {
+      t->restores++;
+      *ok = t->ok;
+      return DRAPE_OK;
+    }
   }
   return factorise(c, true, ok);
 }

@@ -79,7 +79,9 @@ typedef struct {
                              and the step size are unchanged and PCG needs at most twice the iterations per
                              decade of the first solve after it; 2 keeps it across mu and step-size changes too
                              (refactoring only on such a slowdown or after a fallback); 3 keeps it across mu
-                             changes but refactors when the step size changes */
+                             changes but refactors when the step size changes; 4 is 3 that also keeps the
+                             factorisation of the previous step size and returns to it when the step size
+                             does (PTC alternates between two step sizes) */
   int32_t fallback;       /* two-level breakdown (p^T A p <= 0): 1 re-solves with block-Jacobi, 0 reports the
                              breakdown to the Newton loop, which raises mu; 2 returns the last iterate before
                              the negative curvature (status 4) when there is one, else reports the breakdown.

@@ -182,3 +182,27 @@ def test_ptc_n33_two_level_reaches_the_oracle_equilibrium():
         assert out["stop"] in ("tolerance", "stagnation")
         assert r_over_l <= 2.0 * floor
     assert per["two_level"] * 2 < per["block_jacobi"]
+
+
+def test_saved_factorisation_for_alternating_step_sizes():
+    """coarse_reuse = 4 keeps the factorisation of the previous step size and returns to it when PTC does:
+    on 12 PTC steps of the n65 drape (after 3 warm-up steps), which alternate between two step sizes, it needs
+    fewer factorisations than rule 3 and no more PCG iterations, and stops with a residual no worse than 2x
+    rule 3's."""
+    scene = D.drape_config_scene(65)
+    arrays = D.SceneArrays(scene)
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=1))
+    w = ctx.solve_equilibrium(start=arrays.array("rest_positions"), h_start=ctx.rules.h0, cap=3)
+    start, h = w["positions"], w["result"].final_h
+    ctx.close()
+    out = {}
+    for rule in (3, 4):
+        c = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_reuse=rule))
+        r = c.solve_equilibrium(start=start, h_start=h, cap=12)["result"]
+        out[rule] = (c.two_level_layout()[3][0], r.pcg_iterations, r.final_residual / r.gravity_load)
+        c.close()
+    print(f"[gpu] factorisation reuse n65: rule 3 {out[3]}, rule 4 {out[4]} (factorisations, PCG iterations, "
+          f"r/L)", flush=True)
+    assert out[4][0] < out[3][0]
+    assert out[4][1] <= out[3][1]
+    assert out[4][2] <= 2.0 * out[3][2]
