@@ -52,7 +52,8 @@ class Rules(C.Structure):
                 ("noise_aware", C.c_int32), ("exact_tangent", C.c_int32), ("refine_fp64", C.c_int32),
                 ("preconditioner", C.c_int32), ("coarse_reuse", C.c_int32),
                 ("fallback", C.c_int32), ("coarse_shift", C.c_double), ("smoother", C.c_double),
-                ("coarse_half", C.c_int32), ("self_contact", C.c_int32)]
+                ("coarse_half", C.c_int32), ("refresh_rate", C.c_double), ("refresh_on_breakdown", C.c_int32),
+                ("self_contact", C.c_int32)]
 
 
 class PcgStats(C.Structure):

@@ -321,6 +321,8 @@ void drape_default_rules(drape_rules* r) {
   r->coarse_shift = 0.0;
   r->smoother = 0.0;
   r->coarse_half = 2;
+  r->refresh_rate = 2.0;
+  r->refresh_on_breakdown = 1;
   r->self_contact = 0;
 }
 

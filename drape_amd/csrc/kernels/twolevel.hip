@@ -1260,13 +1260,13 @@ int two_level_inverse_factor(Context& c, float* out) {
 void two_level_observe(Context& c, int iterations, double relative_residual) {
   TwoLevel* t = c.two_level;
   if (iterations < 0) {
-    t->refresh = true;
+    if (c.rules.refresh_on_breakdown) t->refresh = true;
     return;
   }
   const double decades = relative_residual > 0 && relative_residual < 1 ? std::max(0.5, -std::log10(relative_residual)) : 0.5;
   const double rate = iterations / decades;
   if (t->solves++ == 0) t->base_rate = rate;
-  else if (rate > 2.0 * t->base_rate + 4.0) t->refresh = true;
+  else if (c.rules.refresh_rate > 0.0 && rate > c.rules.refresh_rate * t->base_rate + 4.0) t->refresh = true;
 }
 
 // y_c = B^T B r_c: from the FP16 scaled copies when they are selected and were formed with the current

@@ -99,6 +99,10 @@ typedef struct {
                              the FP32 copy and still exactly a Gram form, so symmetric positive definite;
                              0 applies the FP32 B = L^-1; 2 (default) chooses FP16 when the FP32 B and B^T
                              together exceed 128 MiB (3 Nc above about 4,000: n257 and larger) */
+  double refresh_rate;    /* two-level, coarse_reuse >= 1: a kept factorisation is refactored at the next system
+                             when a solve needs more than refresh_rate x (+4) the iterations per decade of the
+                             first solve after the factorisation (default 2; 0 never) */
+  int32_t refresh_on_breakdown; /* two-level: 1 (default) also refactors after a breakdown */
   int32_t self_contact;   /* 1 adds the sheet's self-contact: the IPC barrier b(d) of every point-triangle and
                              edge-edge pair closer than dhat (the colliders' dhat and kappa), found by a
                              spatial-hash broad phase, with its Gauss-Newton Hessian b'' grad d grad d^T; the
