@@ -110,8 +110,8 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
-                                   "not implemented", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (domain decomposition with one "
+                                   "halo exchange per PCG iteration) are not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.spmv_only:

@@ -123,7 +123,7 @@ std::vector<double> Mesh::vertex_areas() const {
   return areas;
 }
 
-// ---- material: isotropic plane stress (SURVEY derivation; W of material.rs:39-45) ------------
+// ---- material: isotropic plane stress mapped onto W of material.rs:39-45 ----------------------
 Material Material::isotropic(double E, double nu, double t, double bend, double density) {
   double membrane = E * t / (1.0 - nu * nu);
   return {membrane, membrane, nu * membrane, E * t / (2.0 * (1.0 + nu)), bend, bend, density};
