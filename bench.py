@@ -110,7 +110,7 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (domain decomposition with one "
+        print(json.dumps({"error": "one GPU only: multi-GPU runs (domain decomposition with one "
                                    "halo exchange per PCG iteration) are not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
