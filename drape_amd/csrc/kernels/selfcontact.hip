@@ -45,11 +45,20 @@ struct SelfContact {
   uint32_t *d_ekey = nullptr, *d_eid = nullptr, *d_ekey_s = nullptr, *d_eid_s = nullptr;  // [E]
   uint32_t *d_tcount = nullptr, *d_toff = nullptr;  // [27 T + 1] per (triangle, neighbour cell)
   uint32_t *d_ecount = nullptr, *d_eoff = nullptr;  // [27 E + 1] per (edge, neighbour cell)
+  // the step's swept boxes: (cell key, id) of every cell a vertex's / an edge's box over the step overlaps
+  int vcap = 0, ecap = 0;        // entries allocated
+  uint32_t *d_bvkey = nullptr, *d_bvid = nullptr, *d_bvkey_s = nullptr, *d_bvid_s = nullptr;
+  uint32_t *d_bekey = nullptr, *d_beid = nullptr, *d_bekey_s = nullptr, *d_beid_s = nullptr;
+  uint32_t* d_over = nullptr;    // [1] set when a box spans more cells than its limit
+  // per query slot (triangle or edge, queried cell) of the step: candidate counts and offsets
+  int ptcap = 0, eecap = 0;
+  uint32_t *d_ptcount = nullptr, *d_ptoff = nullptr, *d_eecount = nullptr, *d_eeoff = nullptr;
+  uint32_t *d_tq = nullptr, *d_tqoff = nullptr;  // [T + 1] queried cells per triangle, their offsets
   float* d_part = nullptr;       // [5 * groups] block edge maxima and displacement sums, then deviations
   float* d_grid = nullptr;       // [5] cell size, inflation, mean displacement
   void* d_tmp = nullptr;         // hipCUB scratch
   size_t tmp_bytes = 0;
-  uint32_t* h_totals = nullptr;  // pinned [4]
+  uint32_t* h_totals = nullptr;  // pinned [8]
   // candidates of the evaluated state (their pair terms below) and of the current step (the line-search set)
   struct Set {
     int cap = 0, npt = 0, nee = 0;
@@ -82,7 +91,10 @@ void self_contact_destroy(SelfContact* sc) {
                   (void*)sc->d_grid, sc->d_tmp, (void*)sc->eval.d_cand, (void*)sc->step.d_cand, (void*)sc->d_crow, (void*)sc->d_cdat,
                   (void*)sc->d_ccoef, (void*)sc->d_cfirst, (void*)sc->d_dist, (void*)sc->d_ikey, (void*)sc->d_ival,
                   (void*)sc->d_ikey_s, (void*)sc->d_ival_s, (void*)sc->d_iptr, (void*)sc->d_epart,
-                  (void*)sc->d_eout, (void*)sc->d_apart, (void*)sc->d_aidx})
+                  (void*)sc->d_eout, (void*)sc->d_apart, (void*)sc->d_aidx, (void*)sc->d_bvkey, (void*)sc->d_bvid,
+                  (void*)sc->d_bvkey_s, (void*)sc->d_bvid_s, (void*)sc->d_bekey, (void*)sc->d_beid,
+                  (void*)sc->d_bekey_s, (void*)sc->d_beid_s, (void*)sc->d_over, (void*)sc->d_ptcount,
+                  (void*)sc->d_ptoff, (void*)sc->d_eecount, (void*)sc->d_eeoff, (void*)sc->d_tq, (void*)sc->d_tqoff})
     if (p) (void)hipFree(p);
   if (sc->h_totals) (void)hipHostFree(sc->h_totals);
   delete sc;
@@ -443,6 +455,214 @@ __global__ void __launch_bounds__(256) k_ee(const int2* __restrict__ edges, int 
   if (!EMIT) count[slot] = n;
 }
 
+// ---- swept boxes of the step ------------------------------------------------------------------------
+// A pair that comes within dhat somewhere along x + t dir, t in [0, 1], has its point (edge) inside its own
+// box over the step and its triangle (other edge) inside the other's, within dhat: the boxes of the
+// vertices and edges are hashed into every cell they overlap, and each triangle (edge) queries the cells of
+// its own box grown by dhat, one thread per (primitive, queried cell). A pair is reported from the lowest
+// cell common to both boxes' cell ranges only, and only when the queried cell lies in the stored box's range
+// (a hash collision otherwise).
+
+// Most cells a stored box may span, and a queried one; past them (a step far longer than the edges) the
+// step's set is found on the inflated grid instead.
+constexpr int BOX_CELLS = 512;
+constexpr int QUERY_CELLS = 4096;
+
+__device__ inline void grow_box(P3 x, P3& lo, P3& hi) {
+  lo = {fmin(lo.x, x.x), fmin(lo.y, x.y), fmin(lo.z, x.z)};
+  hi = {fmax(hi.x, x.x), fmax(hi.y, x.y), fmax(hi.z, x.z)};
+}
+// the vertices of primitive i: KIND 0 vertex, 1 edge, 2 triangle
+template <int KIND>
+__device__ inline int prim_ids(int i, const int2* edges, const int4* tri, int ids[3]) {
+  if (KIND == 0) {
+    ids[0] = i;
+    return 1;
+  }
+  if (KIND == 1) {
+    const int2 e = edges[i];
+    ids[0] = e.x;
+    ids[1] = e.y;
+    return 2;
+  }
+  const int4 t = tri[i];
+  ids[0] = t.x;
+  ids[1] = t.y;
+  ids[2] = t.z;
+  return 3;
+}
+// Cell range c0..c1 of the box over the step of vertices ids[0..n), grown by pad; false when it is not
+// finite or lies past 1e9 cells from the origin.
+__device__ inline bool step_range(const float4* pos, const float4* dir, const int32_t* row_of, const int* ids, int n,
+                                  double h, double pad, int3& c0, int3& c1) {
+  P3 lo = {INFINITY, INFINITY, INFINITY}, hi = {-INFINITY, -INFINITY, -INFINITY};
+  for (int k = 0; k < n; ++k) {
+    const P3 x = p3(pos[ids[k]]);
+    grow_box(x, lo, hi);
+    grow_box(axpy(x, 1.0, disp(dir, row_of, ids[k])), lo, hi);
+  }
+  const double f0x = floor((lo.x - pad) / h), f0y = floor((lo.y - pad) / h), f0z = floor((lo.z - pad) / h);
+  const double f1x = floor((hi.x + pad) / h), f1y = floor((hi.y + pad) / h), f1z = floor((hi.z + pad) / h);
+  const double m = fmax(fmax(fmax(fabs(f0x), fabs(f0y)), fmax(fabs(f0z), fabs(f1x))), fmax(fabs(f1y), fabs(f1z)));
+  if (!(m < 1e9)) return false;
+  c0 = make_int3((int)f0x, (int)f0y, (int)f0z);
+  c1 = make_int3((int)f1x, (int)f1y, (int)f1z);
+  return true;
+}
+__device__ inline long long range_cells(int3 c0, int3 c1) {
+  return (long long)(c1.x - c0.x + 1) * (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
+}
+__device__ inline bool in_range(int3 c, int3 a0, int3 a1) {
+  return c.x >= a0.x && c.x <= a1.x && c.y >= a0.y && c.y <= a1.y && c.z >= a0.z && c.z <= a1.z;
+}
+
+// count[i] = cells of primitive i's box over the step grown by pad; a box past `most` cells, or not finite,
+// sets *over
+template <int KIND>
+__global__ void __launch_bounds__(256) k_span(const int2* __restrict__ edges, const int4* __restrict__ tri, int n,
+                                              const float4* __restrict__ pos, const float4* __restrict__ dir,
+                                              const int32_t* __restrict__ row_of, const float* __restrict__ grid,
+                                              double pad, int most, uint32_t* __restrict__ count,
+                                              uint32_t* __restrict__ over) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int ids[3];
+  const int k = prim_ids<KIND>(i, edges, tri, ids);
+  int3 c0, c1;
+  const bool ok = step_range(pos, dir, row_of, ids, k, grid[0], pad, c0, c1);
+  const long long cells = ok ? range_cells(c0, c1) : 0;
+  const bool fits = ok && cells <= most;
+  if (!fits) *over = 1u;
+  count[i] = fits ? (uint32_t)cells : 0u;
+}
+
+// (cell key, i) for every cell of primitive i's box over the step, at key + offset[i] (KIND 0 vertex, 1 edge)
+template <int KIND>
+__global__ void __launch_bounds__(256) k_box_cells(const int2* __restrict__ edges, int n,
+                                                   const float4* __restrict__ pos, const float4* __restrict__ dir,
+                                                   const int32_t* __restrict__ row_of, const float* __restrict__ grid,
+                                                   const uint32_t* __restrict__ offset, uint32_t* __restrict__ key,
+                                                   uint32_t* __restrict__ id) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int ids[3];
+  const int k = prim_ids<KIND>(i, edges, nullptr, ids);
+  int3 c0, c1;
+  step_range(pos, dir, row_of, ids, k, grid[0], 0.0, c0, c1);  // in range: its span was counted
+  uint32_t at = offset[i];
+  for (int z = c0.z; z <= c1.z; ++z)
+    for (int y = c0.y; y <= c1.y; ++y)
+      for (int x = c0.x; x <= c1.x; ++x) {
+        key[at] = cell_key(make_int3(x, y, z));
+        id[at] = (uint32_t)i;
+        ++at;
+      }
+}
+
+// The primitive of query slot `slot` (the last p with qoff[p] <= slot) and its queried cell.
+__device__ inline int slot_owner(const uint32_t* qoff, int n, uint32_t slot) {
+  uint32_t lo = 0, hi = (uint32_t)n + 1;  // first index with qoff > slot
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (qoff[mid] <= slot) lo = mid + 1; else hi = mid;
+  }
+  return (int)lo - 1;
+}
+__device__ inline int3 slot_cell(int3 q0, int3 q1, uint32_t local) {
+  const uint32_t nx = (uint32_t)(q1.x - q0.x + 1), ny = (uint32_t)(q1.y - q0.y + 1);
+  return make_int3(q0.x + (int)(local % nx), q0.y + (int)((local / nx) % ny), q0.z + (int)(local / (nx * ny)));
+}
+
+// point-triangle candidates of query slot (triangle t, queried cell q) along the step: EMIT 0 counts,
+// 1 writes at out + offset[slot]
+template <int EMIT>
+__global__ void __launch_bounds__(256) k_swept_pt(const int4* __restrict__ tri, int T, const uint32_t* __restrict__ qoff,
+                                                  uint32_t slots, const float4* __restrict__ pos,
+                                                  const float4* __restrict__ dir, const int32_t* __restrict__ row_of,
+                                                  const float* __restrict__ grid, double dhat,
+                                                  const uint32_t* __restrict__ key, const uint32_t* __restrict__ vid,
+                                                  uint32_t nkeys, uint32_t* __restrict__ count,
+                                                  const uint32_t* __restrict__ offset, int4* __restrict__ out) {
+  const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+  if (slot >= slots) return;
+  const int t = slot_owner(qoff, T, slot);
+  const double h = grid[0];
+  const int4 tv = tri[t];
+  const int tids[3] = {tv.x, tv.y, tv.z};
+  int3 q0, q1;
+  step_range(pos, dir, row_of, tids, 3, h, dhat, q0, q1);
+  const int3 q = slot_cell(q0, q1, slot - qoff[t]);
+  const P3 a = p3(pos[tv.x]), b = p3(pos[tv.y]), c = p3(pos[tv.z]);
+  const P3 cen = {(a.x + b.x + c.x) / 3.0, (a.y + b.y + c.y) / 3.0, (a.z + b.z + c.z) / 3.0};
+  const double reach = fmax(dist3(a, cen), fmax(dist3(b, cen), dist3(c, cen)));
+  const uint32_t k = cell_key(q);
+  uint32_t n = 0, at = EMIT ? offset[slot] : 0;
+  for (uint32_t i = lower_bound_u32(key, nkeys, k); i < nkeys && key[i] == k; ++i) {
+    const int v = (int)vid[i];
+    if (v == tv.x || v == tv.y || v == tv.z) continue;
+    if (i > 0 && key[i - 1] == k && (int)vid[i - 1] == v) continue;  // two of its cells share the key
+    int3 v0, v1;
+    step_range(pos, dir, row_of, &v, 1, h, 0.0, v0, v1);
+    if (!in_range(q, v0, v1)) continue;
+    if (q.x != max(v0.x, q0.x) || q.y != max(v0.y, q0.y) || q.z != max(v0.z, q0.z)) continue;
+    const int4 quad = make_int4(v, tv.x, tv.y, tv.z);
+    const double lim = (dhat + approach(dir, row_of, true, quad)) * (1.0 + 1e-9);
+    const P3 xv = p3(pos[v]);
+    if (dist3(xv, cen) - reach >= lim) continue;
+    if (pair_geometry(true, xv, a, b, c).d >= lim) continue;
+    if (EMIT) out[at++] = quad;
+    ++n;
+  }
+  if (!EMIT) count[slot] = n;
+}
+
+// edge-edge candidates (e, f), f > e, without a shared vertex, of query slot (edge e, queried cell q)
+template <int EMIT>
+__global__ void __launch_bounds__(256) k_swept_ee(const int2* __restrict__ edges, int E,
+                                                  const uint32_t* __restrict__ qoff, uint32_t slots,
+                                                  const float4* __restrict__ pos, const float4* __restrict__ dir,
+                                                  const int32_t* __restrict__ row_of, const float* __restrict__ grid,
+                                                  double dhat, const uint32_t* __restrict__ key,
+                                                  const uint32_t* __restrict__ eid, uint32_t nkeys,
+                                                  uint32_t* __restrict__ count, const uint32_t* __restrict__ offset,
+                                                  int4* __restrict__ out) {
+  const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+  if (slot >= slots) return;
+  const int e = slot_owner(qoff, E, slot);
+  const double h = grid[0];
+  const int2 ev = edges[e];
+  const int eids[2] = {ev.x, ev.y};
+  int3 q0, q1;
+  step_range(pos, dir, row_of, eids, 2, h, dhat, q0, q1);
+  const int3 q = slot_cell(q0, q1, slot - qoff[e]);
+  const P3 p1 = p3(pos[ev.x]), q1p = p3(pos[ev.y]);
+  const P3 m1 = midpoint(pos, ev);
+  const double half1 = 0.5 * dist3(p1, q1p);
+  const uint32_t k = cell_key(q);
+  uint32_t n = 0, at = EMIT ? offset[slot] : 0;
+  for (uint32_t i = lower_bound_u32(key, nkeys, k); i < nkeys && key[i] == k; ++i) {
+    const int f = (int)eid[i];
+    if (f <= e) continue;
+    if (i > 0 && key[i - 1] == k && (int)eid[i - 1] == f) continue;  // two of its cells share the key
+    const int2 fv = edges[f];
+    if (fv.x == ev.x || fv.x == ev.y || fv.y == ev.x || fv.y == ev.y) continue;
+    const int fids[2] = {fv.x, fv.y};
+    int3 f0, f1;
+    step_range(pos, dir, row_of, fids, 2, h, 0.0, f0, f1);
+    if (!in_range(q, f0, f1)) continue;
+    if (q.x != max(f0.x, q0.x) || q.y != max(f0.y, q0.y) || q.z != max(f0.z, q0.z)) continue;
+    const int4 quad = make_int4(ev.x, ev.y, fv.x, fv.y);
+    const double lim = (dhat + approach(dir, row_of, false, quad)) * (1.0 + 1e-9);
+    const P3 p2 = p3(pos[fv.x]), q2 = p3(pos[fv.y]);
+    const P3 m2 = midpoint(pos, fv);
+    if (dist3(m1, m2) - half1 - 0.5 * dist3(p2, q2) >= lim) continue;
+    if (pair_geometry(false, p1, q1p, p2, q2).d >= lim) continue;
+    if (EMIT) out[at++] = quad;
+    ++n;
+  }
+  if (!EMIT) count[slot] = n;
+}
+
 // ---- pair terms ----------------------------------------------------------------------------------
 
 // distance, direction, coefficients and barrier derivatives of each candidate; the incidence entries
@@ -719,18 +939,22 @@ int setup(Context& c) {
   TRY_RC(grow(&sc->d_row_of, V));
   for (uint32_t** p : {&sc->d_vkey, &sc->d_vid, &sc->d_vkey_s, &sc->d_vid_s}) TRY_RC(grow(p, V));
   for (uint32_t** p : {&sc->d_ekey, &sc->d_eid, &sc->d_ekey_s, &sc->d_eid_s}) TRY_RC(grow(p, E));
-  TRY_RC(grow(&sc->d_tcount, 27 * (size_t)T + 1));
-  TRY_RC(grow(&sc->d_toff, 27 * (size_t)T + 1));
+  // per (triangle, neighbour cell), and per vertex for the step's boxes
+  TRY_RC(grow(&sc->d_tcount, std::max(27 * (size_t)T, (size_t)V) + 1));
+  TRY_RC(grow(&sc->d_toff, std::max(27 * (size_t)T, (size_t)V) + 1));
   TRY_RC(grow(&sc->d_ecount, 27 * (size_t)E + 1));
   TRY_RC(grow(&sc->d_eoff, 27 * (size_t)E + 1));
   TRY_RC(grow(&sc->d_part, 5 * (size_t)blocks_for(std::max(V, E))));
   TRY_RC(grow(&sc->d_grid, 5));
   TRY_RC(grow(&sc->d_iptr, (size_t)c.R + 1));
   TRY_RC(grow(&sc->d_eout, 2));
-  HIP_TRY(hipHostMalloc((void**)&sc->h_totals, 4 * sizeof(uint32_t)));
+  TRY_RC(grow(&sc->d_over, 1));
+  TRY_RC(grow(&sc->d_tq, (size_t)T + 1));
+  TRY_RC(grow(&sc->d_tqoff, (size_t)T + 1));
+  HIP_TRY(hipHostMalloc((void**)&sc->h_totals, 8 * sizeof(uint32_t)));
   // scratch for the largest sort and scans
   size_t need = 0, b = 0;
-  const int most = std::max(std::max(V, 27 * E + 1), 27 * T + 1);
+  const int most = std::max(std::max(V + 1, 27 * E + 1), 27 * T + 1);
   HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                              (uint32_t*)nullptr, most));
   need = std::max(need, b);
@@ -821,6 +1045,121 @@ int broad_phase(Context& c, const float4* dir, SelfContact::Set& set) {
   return DRAPE_OK;
 }
 
+// The step's line-search set from the swept boxes (cell size: longest edge + dhat). Returns with *done false,
+// and nothing written to `set`, when a box spans more cells than its limit, or the boxes more than 32 (stored)
+// or 64 (queried) cells per primitive on average.
+int swept_phase(Context& c, SelfContact::Set& set, bool* done) {
+  SelfContact* sc = c.self_contact;
+  hipStream_t s = c.stream;
+  const int V = sc->V, T = sc->T, E = sc->E;
+  const float4* dir = c.dir;
+  const double dhat = sc->dhat;
+  *done = false;
+  const int groups = blocks_for(std::max(V, E));
+  k_extent<<<groups, 256, 0, s>>>(sc->d_edges, E, c.pos, nullptr, sc->d_row_of, V, sc->d_part);
+  k_mean<<<1, 256, 0, s>>>(sc->d_part, groups, sc->d_grid);
+  k_grid<<<1, 256, 0, s>>>(sc->d_part, 0, 0, (float)dhat, sc->d_grid);
+  HIP_TRY(hipMemsetAsync(sc->d_over, 0, sizeof(uint32_t), s));
+  // stored boxes: vertices (d_tcount, [V + 1]) and edges (d_ecount); queried: triangles (d_tq) and edges
+  // (d_ecount past the edges' stored counts)
+  uint32_t* eq = sc->d_ecount + (E + 1);
+  uint32_t* eqoff = sc->d_eoff + (E + 1);
+  k_span<0><<<blocks_for(V), 256, 0, s>>>(nullptr, nullptr, V, c.pos, dir, sc->d_row_of, sc->d_grid, 0.0, BOX_CELLS,
+                                          sc->d_tcount, sc->d_over);
+  k_span<1><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, nullptr, E, c.pos, dir, sc->d_row_of, sc->d_grid, 0.0,
+                                          BOX_CELLS, sc->d_ecount, sc->d_over);
+  k_span<2><<<blocks_for(T), 256, 0, s>>>(nullptr, c.tri, T, c.pos, dir, sc->d_row_of, sc->d_grid, dhat, QUERY_CELLS,
+                                          sc->d_tq, sc->d_over);
+  k_span<1><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, nullptr, E, c.pos, dir, sc->d_row_of, sc->d_grid, dhat,
+                                          QUERY_CELLS, eq, sc->d_over);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(sc->d_tcount + V, 0, sizeof(uint32_t), s));
+  HIP_TRY(hipMemsetAsync(sc->d_ecount + E, 0, sizeof(uint32_t), s));
+  HIP_TRY(hipMemsetAsync(sc->d_tq + T, 0, sizeof(uint32_t), s));
+  HIP_TRY(hipMemsetAsync(eq + E, 0, sizeof(uint32_t), s));
+  TRY_RC(scan(sc, sc->d_tcount, sc->d_toff, V + 1, s));
+  TRY_RC(scan(sc, sc->d_ecount, sc->d_eoff, E + 1, s));
+  TRY_RC(scan(sc, sc->d_tq, sc->d_tqoff, T + 1, s));
+  TRY_RC(scan(sc, eq, eqoff, E + 1, s));
+  uint32_t* h = sc->h_totals;
+  HIP_TRY(hipMemcpyAsync(h, sc->d_toff + V, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h + 1, sc->d_eoff + E, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h + 2, sc->d_tqoff + T, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h + 3, eqoff + E, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h + 4, sc->d_over, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c.read_bytes += 5 * sizeof(uint32_t);
+  // the per-primitive limits bound every total below 2^32 for up to 2^22 primitives of a kind
+  if (h[4] || h[0] > 32u * (uint32_t)V + 1024u || h[1] > 32u * (uint32_t)E + 1024u ||
+      h[2] > 64u * (uint32_t)T + 1024u || h[3] > 64u * (uint32_t)E + 1024u)
+    return DRAPE_OK;
+  const int nbv = (int)h[0], nbe = (int)h[1];
+  const uint32_t spt = h[2], see = h[3];
+  if (nbv > sc->vcap) {
+    const int cap = std::max(nbv, 2 * sc->vcap);
+    for (uint32_t** p : {&sc->d_bvkey, &sc->d_bvid, &sc->d_bvkey_s, &sc->d_bvid_s}) TRY_RC(grow(p, cap));
+    sc->vcap = cap;
+  }
+  if (nbe > sc->ecap) {
+    const int cap = std::max(nbe, 2 * sc->ecap);
+    for (uint32_t** p : {&sc->d_bekey, &sc->d_beid, &sc->d_bekey_s, &sc->d_beid_s}) TRY_RC(grow(p, cap));
+    sc->ecap = cap;
+  }
+  if ((int)spt + 1 > sc->ptcap) {
+    const int cap = std::max((int)spt + 1, 2 * sc->ptcap);
+    TRY_RC(grow(&sc->d_ptcount, cap));
+    TRY_RC(grow(&sc->d_ptoff, cap));
+    sc->ptcap = cap;
+  }
+  if ((int)see + 1 > sc->eecap) {
+    const int cap = std::max((int)see + 1, 2 * sc->eecap);
+    TRY_RC(grow(&sc->d_eecount, cap));
+    TRY_RC(grow(&sc->d_eeoff, cap));
+    sc->eecap = cap;
+  }
+  k_box_cells<0><<<blocks_for(V), 256, 0, s>>>(nullptr, V, c.pos, dir, sc->d_row_of, sc->d_grid, sc->d_toff,
+                                               sc->d_bvkey, sc->d_bvid);
+  k_box_cells<1><<<blocks_for(E), 256, 0, s>>>(sc->d_edges, E, c.pos, dir, sc->d_row_of, sc->d_grid, sc->d_eoff,
+                                               sc->d_bekey, sc->d_beid);
+  HIP_TRY(hipGetLastError());
+  TRY_RC(sort_pairs(sc, sc->d_bvkey, sc->d_bvkey_s, sc->d_bvid, sc->d_bvid_s, nbv, s));
+  TRY_RC(sort_pairs(sc, sc->d_bekey, sc->d_bekey_s, sc->d_beid, sc->d_beid_s, nbe, s));
+  const int bpt = blocks_for((int)spt), bee = blocks_for((int)see);
+  if (spt > 0)
+    k_swept_pt<0><<<bpt, 256, 0, s>>>(c.tri, T, sc->d_tqoff, spt, c.pos, dir, sc->d_row_of, sc->d_grid, dhat,
+                                      sc->d_bvkey_s, sc->d_bvid_s, (uint32_t)nbv, sc->d_ptcount, nullptr, nullptr);
+  if (see > 0)
+    k_swept_ee<0><<<bee, 256, 0, s>>>(sc->d_edges, E, eqoff, see, c.pos, dir, sc->d_row_of, sc->d_grid, dhat,
+                                      sc->d_bekey_s, sc->d_beid_s, (uint32_t)nbe, sc->d_eecount, nullptr, nullptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(sc->d_ptcount + spt, 0, sizeof(uint32_t), s));
+  HIP_TRY(hipMemsetAsync(sc->d_eecount + see, 0, sizeof(uint32_t), s));
+  TRY_RC(scan(sc, sc->d_ptcount, sc->d_ptoff, (int)spt + 1, s));
+  TRY_RC(scan(sc, sc->d_eecount, sc->d_eeoff, (int)see + 1, s));
+  HIP_TRY(hipMemcpyAsync(h, sc->d_ptoff + spt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h + 1, sc->d_eeoff + see, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c.read_bytes += 2 * sizeof(uint32_t);
+  const int npt = (int)h[0], nee = (int)h[1], n = npt + nee;
+  if (n > set.cap) {
+    const int cap = std::max(n, 2 * set.cap);
+    TRY_RC(grow(&set.d_cand, cap));
+    set.cap = cap;
+  }
+  set.npt = npt;
+  set.nee = nee;
+  if (npt > 0)
+    k_swept_pt<1><<<bpt, 256, 0, s>>>(c.tri, T, sc->d_tqoff, spt, c.pos, dir, sc->d_row_of, sc->d_grid, dhat,
+                                      sc->d_bvkey_s, sc->d_bvid_s, (uint32_t)nbv, nullptr, sc->d_ptoff, set.d_cand);
+  if (nee > 0)
+    k_swept_ee<1><<<bee, 256, 0, s>>>(sc->d_edges, E, eqoff, see, c.pos, dir, sc->d_row_of, sc->d_grid, dhat,
+                                      sc->d_bekey_s, sc->d_beid_s, (uint32_t)nbe, nullptr, sc->d_eeoff,
+                                      set.d_cand + npt);
+  HIP_TRY(hipGetLastError());
+  *done = true;
+  return DRAPE_OK;
+}
+
 // Term arrays for n pairs and partial buffers for n candidates.
 int ensure_terms(SelfContact* sc, int n) {
   if (n > sc->tcap) {
@@ -902,7 +1241,9 @@ void self_contact_moved(Context& c) {
 static int step_set(Context& c) {
   SelfContact* sc = c.self_contact;
   if (!sc->step_ready) {
-    TRY_RC(broad_phase(c, c.dir, sc->step));
+    bool done = false;
+    TRY_RC(swept_phase(c, sc->step, &done));
+    if (!done) TRY_RC(broad_phase(c, c.dir, sc->step));
     TRY_RC(ensure_parts(sc, sc->step.npt + sc->step.nee));
     sc->step_ready = true;
   }

@@ -6,8 +6,9 @@ folded n17 sheet whose upper half lies over its lower half:
   - the assembled right-hand side changes by the oracle's barrier gradient (FP32: 2e-5 of its largest entry);
   - PCG solves the full operator, the contact coupling outside the block pattern included: the true residual
     of the oracle's matrix (assembled blocks + Gauss-Newton couplings) is at the tolerance;
-  - the step bound keeps every pair that can come within dhat above 0.1 of its distance, and the line
-    search's energy change includes the pairs' barrier change (1e-6 relative);
+  - the step bound keeps every pair that can come within dhat above 0.1 of its distance, from the swept-box
+    hash and, for a step too long for it, from the inflated grid; the line search's energy change includes
+    the pairs' barrier change (1e-6 relative);
   - dynamics: under gravity the upper half comes to rest on the lower half with self-contact and never
     crosses it, while without self-contact it falls through to the floor.
 With rules.self_contact = 0 (the default) every other test runs the unchanged path."""
@@ -162,6 +163,41 @@ def test_step_bound_and_energy_change():
     assert abs((s_on - s_off) - want) <= 1e-6 * abs(want)
     ctx.close()
     off.close()
+
+
+def test_step_bound_of_a_far_step():
+    """A step that moves one vertex of the upper half 52 m (its box over the step spans more cells than the
+    swept-box hash takes) is bounded on the inflated grid instead: every pair within the reach of the step
+    keeps at least 0.1 of its distance."""
+    arrays, x, top = folded(2.0 * DHAT)
+    ctx = assembled(arrays, x, True)
+    rows = arrays.array("row_vertex")
+    d = np.zeros((arrays.V, 3))
+    d[top, 2] = -4.0 * DHAT
+    w = int(np.flatnonzero(top)[len(np.flatnonzero(top)) // 2])
+    d[w] += 30.0
+    ctx.set_direction(d[rows].astype(np.float32).ravel())
+    alpha, row, _ = ctx.max_step()
+    xs = x + alpha * d.astype(np.float32).astype(np.float64)
+    tri = tri_of(arrays)
+    # the pairs the ordinary part of the step reaches, and every pair of the far vertex
+    quads, _, kinds = S.candidates(x, tri, DHAT + 8.0 * DHAT * 1.01 + 1e-3)
+    extra = [((w, *map(int, t)), 0) for t in tri if w not in t]
+    extra += [((int(u), *map(int, t)), 0) for t in tri if w in t for u in range(arrays.V) if u not in t]
+    edges = S.unique_edges(tri)
+    for p, q in edges:
+        for r, t in edges:
+            if w in (p, q) and len({int(p), int(q), int(r), int(t)}) == 4:
+                extra.append(((int(p), int(q), int(r), int(t)), 1))
+    pairs = [(tuple(map(int, q)), int(k)) for q, k in zip(quads, kinds)] + extra
+    d0 = np.array([S.pair_geometry(x, np.array(q), k)[0] for q, k in pairs])
+    d1 = np.array([S.pair_geometry(xs, np.array(q), k)[0] for q, k in pairs])
+    ratio = (d1 / d0).min()
+    print(f"[gpu] self-contact far step: alpha {alpha:.3e}, row {row}, {len(pairs)} pairs, min d(x + alpha p)/d(x) "
+          f"{ratio:.3f}", flush=True)
+    assert 0.0 < alpha < 0.01  # the far vertex bounds the step (0.45 without it)
+    assert ratio >= 0.1 * (1 - 1e-6)
+    ctx.close()
 
 
 def test_fold_comes_to_rest_without_crossing():
