@@ -1,12 +1,13 @@
 """Benchmark of the implicit Newton step at n129: ms per Newton iteration and PCG iterations per Newton
 iteration of the quasi-static sphere drape solved by pseudo-transient continuation (129 x 129 grid,
 16641 vertices, 32768 triangles; isotropic E = 1e6 Pa, nu = 0.3, t = 0.5 mm, 0.2 kg/m^2, hinge 1e-5 N m,
-sphere r = 0.15 m, dhat = 1 mm), two-level preconditioner against block-Jacobi FP32 PCG.
+sphere r = 0.15 m, dhat = 1 mm), the learned two-level preconditioner against block-Jacobi FP32 PCG.
 
 A bench step is one accepted PTC step (one inexact Newton solve of the step's energy, several Newton
 iterations). `--warmup W` steps run untimed from the flat start; the next `--steps K` are timed, once with
-the selected preconditioner (the reported value) and once with block-Jacobi from the same state (the
-comparison in `config`). Prints one JSON line.
+the selected preconditioner (the reported value; by default the two-level preconditioner with the learned
+prolongation shipped with the package), then from the same state with the bilinear two-level preconditioner
+and with block-Jacobi (the comparisons in `config`). Prints one JSON line.
 
 The roofline object is for the SpMV of PCG, the dominant kernel of the block-Jacobi solve (92% of the
 reference's n129 run) and the largest of the two-level iteration: algorithmic bytes 40 B per block + 28 B
@@ -101,8 +102,9 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--columns", type=int, default=129)
-    ap.add_argument("--preconditioner", choices=["block_jacobi", "two_level"], default="two_level")
-    ap.add_argument("--no-compare", action="store_true", help="skip the block-Jacobi comparison run")
+    ap.add_argument("--preconditioner", choices=["learned", "two_level", "block_jacobi"], default="learned",
+                    help="learned: two-level with the learned prolongation; two_level: with bilinear prolongation")
+    ap.add_argument("--no-compare", action="store_true", help="skip the comparison runs")
     ap.add_argument("--roofline-columns", type=int, default=1025,
                     help="grid for the HBM-bound SpMV roofline (0 skips it)")
     ap.add_argument("--spmv-only", action="store_true",
@@ -122,7 +124,15 @@ def main() -> None:
     scene = D.drape_config_scene(args.columns)
     arrays = D.SceneArrays(scene)
     kind = 0 if args.preconditioner == "block_jacobi" else 1
-    ctx = D.Drape(arrays, D.default_rules(preconditioner=kind))
+
+    def context(pre):
+        c = D.Drape(arrays, D.default_rules(preconditioner=0 if pre == "block_jacobi" else 1))
+        if pre == "learned":
+            from drape_amd.learned import load_model
+            c.set_prolongation_model(load_model())
+        return c
+
+    ctx = context(args.preconditioner)
     print(f"[bench] n{args.columns}: V {arrays.V} T {arrays.T} H {arrays.H} rows {arrays.R} blocks {arrays.B}",
           file=sys.stderr, flush=True)
 
@@ -147,26 +157,19 @@ def main() -> None:
     out, res, newton, wall = timed(ctx, args.preconditioner)
     ms_per_newton = 1000.0 * res.seconds / newton
     pcg_per_newton = res.pcg_iterations / newton
-    comparison = None
-    learned = None
-    if kind != 0 and not args.no_compare:
-        # the same steps with the learned prolongation (trained parameters shipped with the package)
-        from drape_amd.learned import load_model
-        lctx = D.Drape(arrays, D.default_rules(preconditioner=1))
-        lctx.set_prolongation_model(load_model())
-        _, lres, lnewton, _ = timed(lctx, "two_level_learned")
-        learned = {"ms_per_newton": round(1000.0 * lres.seconds / lnewton, 4),
-                   "pcg_per_newton": round(lres.pcg_iterations / lnewton, 2),
-                   "newton_iterations": lres.newton_iterations, "pcg_iterations": lres.pcg_iterations,
-                   "fallbacks": lres.fallbacks, "accepted_steps": lres.accepted_steps}
-        lctx.close()
-        bj = D.Drape(arrays, D.default_rules(preconditioner=0))
-        _, bres, bnewton, _ = timed(bj, "block_jacobi")
-        comparison = {"ms_per_newton": round(1000.0 * bres.seconds / bnewton, 4),
-                      "pcg_per_newton": round(bres.pcg_iterations / bnewton, 2),
-                      "newton_iterations": bres.newton_iterations, "pcg_iterations": bres.pcg_iterations,
-                      "accepted_steps": bres.accepted_steps}
-        bj.close()
+    comparisons = {}
+    if not args.no_compare:
+        # the same steps from the same state with the other preconditioners
+        for pre in ("learned", "two_level", "block_jacobi"):
+            if pre == args.preconditioner:
+                continue
+            other = context(pre)
+            _, ores, onewton, _ = timed(other, pre)
+            comparisons[pre] = {"ms_per_newton": round(1000.0 * ores.seconds / onewton, 4),
+                                "pcg_per_newton": round(ores.pcg_iterations / onewton, 2),
+                                "newton_iterations": ores.newton_iterations, "pcg_iterations": ores.pcg_iterations,
+                                "fallbacks": ores.fallbacks, "accepted_steps": ores.accepted_steps}
+            other.close()
     layout = ctx.two_level_layout() if kind != 0 else None
 
     # dominant kernel: SpMV of the PCG on the last assembled system, timed with HIP events
@@ -200,8 +203,9 @@ def main() -> None:
                    "coarse_rows": layout[0] if layout else None,
                    "factorisations": layout[3][0] if layout else None,
                    "factorisation_ms": round(layout[3][3] / 1000.0, 2) if layout else None,
-                   "learned_prolongation": learned,
-                   "block_jacobi": comparison},
+                   "two_level_learned": comparisons.get("learned"),
+                   "two_level_bilinear": comparisons.get("two_level"),
+                   "block_jacobi": comparisons.get("block_jacobi")},
         "roofline": roofline,
     }
     if args.roofline_columns > 0:
