@@ -191,6 +191,29 @@ def test_pcg_on_assembled_system_bit_identical(bent_n33):
     assert slope == pytest.approx(-float(np.dot(rhs.astype(np.float64), x.astype(np.float64))), rel=1e-12)
 
 
+def test_streamed_pcg_bit_identical_at_n1025():
+    """At n1025 (1,050,625 rows, 11.5 M blocks, 415 MB of matrix values) the context streams the matrix past
+    the caches (non-temporal loads above 128 MiB): 20 PCG iterations on the assembled system of the flat
+    sheet equal the oracle's f32 emulation on the same f32 matrix, bit for bit."""
+    arrays = D.SceneArrays(D.drape_config_scene(1025))
+    ctx = D.Drape(arrays)
+    rest = arrays.array("rest_positions")
+    x_hat = rest.copy()
+    x_hat[2::3] -= np.float32(9.81 / 3600.0)
+    ctx.set_state(rest, x_hat, 1.0 / 60.0, 0.0)
+    ctx.residual()
+    rhs, vals = ctx.system()
+    st, x = ctx.solve_once(1e-9, 20)
+    xo, so, _ = O.pcg(arrays.array("row_ptr").astype(np.int32), arrays.array("col_idx").astype(np.int32),
+                      vals.astype(np.float64), rhs.astype(np.float64), 1e-9, 20, 50, 1e-2, single=True,
+                      breakdown_check=True)
+    print(f"[gpu] streamed PCG n1025: {st.iterations} iterations (oracle {so.iterations}), status "
+          f"{D.PCG_STATUS[st.status]}", flush=True)
+    assert st.iterations == so.iterations == 20
+    assert np.array_equal(x.view(np.uint32), xo.astype(np.float32).view(np.uint32))
+    ctx.close()
+
+
 def test_energy_change_matches_f64(bent_n33):
     """W(x + alpha d) - W(x) summed in FP64 (membrane S:dE + W(dE), hinge k(t'-t)(t'+t), inertia) against the
     oracle's f64 energy change at the same f32 state and direction."""
