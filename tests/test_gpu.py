@@ -64,6 +64,58 @@ def test_e1_pcg_bit_identical_to_oracle(case):
     sys.close()
 
 
+def _tension_n33():
+    c = O.Case("tension", 33)
+    return c.row_ptr, c.col_idx, c.vals, c.rhs
+
+
+@pytest.mark.parametrize("edge", ["zero_rhs", "one_row", "ragged_rows"])
+def test_pcg_edge_cases_match_oracle(edge):
+    """Edge cases of the FP32 PCG (pcg.rs:60-80 semantics), each against the oracle's f32 emulation: a zero
+    right-hand side stops converged before the first iteration with x = 0 (rr <= 0, pcg.rs:65); a one-row system (one 3x3 block) converges in one
+    iteration, since block-Jacobi inverts it exactly; and a three-row system whose rows hold 2, 2 and 1
+    blocks (ragged rows of one SELL slice padded to 64 rows)."""
+    rng = np.random.default_rng(11)
+    cap = 100
+    if edge == "zero_rhs":
+        row_ptr, col_idx, vals, rhs = _tension_n33()
+        rhs = np.zeros_like(rhs)
+    else:
+        def spd():
+            a = rng.standard_normal((3, 3))
+            return a @ a.T + 3.0 * np.eye(3)
+        if edge == "one_row":
+            row_ptr, col_idx = np.array([0, 1]), np.array([0])
+            vals = spd().ravel()
+        else:
+            # rows 0 and 1 coupled by c, row 2 alone: 2, 2 and 1 blocks per row
+            c = 0.3 * rng.standard_normal((3, 3))
+            row_ptr, col_idx = np.array([0, 2, 4, 5]), np.array([0, 1, 0, 1, 2])
+            vals = np.concatenate([spd().ravel(), c.ravel(), c.T.ravel(), spd().ravel(), spd().ravel()])
+        vals = vals.astype(np.float32).astype(np.float64)
+        rhs = rng.standard_normal(3 * (len(row_ptr) - 1)).astype(np.float32).astype(np.float64)
+    rules = D.default_rules(replace_every=50, replace_above=1e-2)
+    sys = D.LinearSystem(np.asarray(row_ptr, np.uint32), np.asarray(col_idx, np.uint32),
+                         np.asarray(vals, np.float32), np.asarray(rhs, np.float32), rules)
+    st, x = sys.solve_once(1e-6, cap)
+    xo, so, _ = O.pcg(np.asarray(row_ptr, np.int32), np.asarray(col_idx, np.int32), vals, rhs, 1e-6, cap, 50, 1e-2,
+                      single=True, breakdown_check=True)
+    print(f"[gpu] PCG edge case {edge}: {st.iterations} iterations, status {D.PCG_STATUS[st.status]} (oracle "
+          f"{so.iterations}, {D.PCG_STATUS[so.status]})", flush=True)
+    assert st.iterations == so.iterations and st.status == so.status
+    assert np.array_equal(x.view(np.uint32), xo.astype(np.float32).view(np.uint32))
+    want = {"zero_rhs": ("converged", 0), "one_row": ("converged", 1)}
+    if edge in want:
+        assert (D.PCG_STATUS[st.status], st.iterations) == want[edge]
+    if edge == "zero_rhs":
+        assert not np.any(x)
+    # arguments outside the solver's domain are errors, not statuses
+    for tol, it in ((1e-6, 0), (0.0, 10), (float("nan"), 10)):
+        with pytest.raises(D.DrapeError, match="invalid PCG arguments"):
+            sys.solve_once(tol, it)
+    sys.close()
+
+
 # ---- element kernels: the reference's kernel gate -----------------------------------------------
 def scene_pair(n, sphere):
     ps = D.reference_scene(n, sphere_on_floor=sphere, placed=True)
