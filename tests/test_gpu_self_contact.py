@@ -6,6 +6,7 @@ folded n17 sheet whose upper half lies over its lower half:
   - the assembled right-hand side changes by the oracle's barrier gradient (FP32: 2e-5 of its largest entry);
   - PCG solves the full operator, the contact coupling outside the block pattern included: the true residual
     of the oracle's matrix (assembled blocks + Gauss-Newton couplings) is at the tolerance;
+  - with no pair within dhat the path is bit-identical to the one without self-contact;
   - the step bound keeps every pair that can come within dhat above 0.1 of its distance, from the swept-box
     hash and, for a step too long for it, from the inflated grid; the line search's energy change includes
     the pairs' barrier change (1e-6 relative);
@@ -57,6 +58,38 @@ def kappa_of(arrays):
 
 def tri_of(arrays):
     return arrays.array("elem_vertices").reshape(-1, 3)
+
+
+def test_no_pairs_leaves_the_step_unchanged():
+    """A flat sheet (no two primitives within dhat, empty pair sets in both broad phases): with self-contact on,
+    the assembled system, the PCG solution, the step bound and the energy change are bit-identical to the
+    path without it."""
+    arrays, _, _ = folded(0.5 * DHAT)
+    flat = arrays.array("rest_positions").astype(np.float64).reshape(-1, 3).copy()
+    flat[:, 2] = 0.5 * DHAT
+    on, off = assembled(arrays, flat, True), assembled(arrays, flat, False)
+    _, verts, _ = on.self_contact_pairs()
+    assert len(verts) == 0
+    rows = arrays.array("row_vertex")
+    d = np.zeros((arrays.V, 3))
+    d[:, 2] = -0.2 * DHAT
+    d[:, 0] = 1e-3 * np.sin(np.arange(arrays.V))
+    out = {}
+    for name, c in (("on", on), ("off", off)):
+        rhs, vals = c.system()
+        st, sol = c.solve_once(1e-6, 2000)
+        c.set_direction(d[rows].astype(np.float32).ravel())
+        alpha, row, _ = c.max_step()
+        e = c.energy_change(0.5 * alpha)
+        out[name] = (rhs, vals, sol, st.iterations, alpha, row, e)
+    a, b = out["on"], out["off"]
+    print(f"[gpu] self-contact without pairs: PCG {a[3]} / {b[3]} iterations, alpha {a[4]:.4f} / {b[4]:.4f}",
+          flush=True)
+    for u, v in zip(a[:3], b[:3]):
+        assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
+    assert a[3:] == b[3:]
+    on.close()
+    off.close()
 
 
 def test_pairs_match_brute_force():
