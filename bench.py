@@ -110,6 +110,8 @@ def main() -> None:
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the large-grid SpMV and a copy of known size (for PMC byte counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--self-contact", action="store_true",
+                    help="turn on the sheet's self-contact (point-triangle / edge-edge barriers) in every run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: multi-GPU runs (domain decomposition with one "
@@ -126,7 +128,8 @@ def main() -> None:
     kind = 0 if args.preconditioner == "block_jacobi" else 1
 
     def context(pre):
-        c = D.Drape(arrays, D.default_rules(preconditioner=0 if pre == "block_jacobi" else 1))
+        c = D.Drape(arrays, D.default_rules(preconditioner=0 if pre == "block_jacobi" else 1,
+                                            self_contact=1 if args.self_contact else 0))
         if pre == "learned":
             from drape_amd.learned import load_model
             c.set_prolongation_model(load_model())
@@ -196,7 +199,8 @@ def main() -> None:
         "config": {"workload": f"quasi-static sphere drape by PTC, n{args.columns} "
                                f"({arrays.V} vertices, {arrays.T} triangles), isotropic E=1e6 nu=0.3 t=0.5mm, "
                                "sphere r=0.15 m, dhat=1 mm",
-                   "preconditioner": args.preconditioner, "pcg_per_newton": round(pcg_per_newton, 2),
+                   "preconditioner": args.preconditioner, "self_contact": args.self_contact,
+                   "pcg_per_newton": round(pcg_per_newton, 2),
                    "newton_iterations": res.newton_iterations, "pcg_iterations": res.pcg_iterations,
                    "accepted_steps": res.accepted_steps, "phase_ms": {k: round(v, 2) for k, v in zip(D.PHASES, res.phase_ms)},
                    "fallbacks": res.fallbacks, "wall_s": round(wall, 3),
