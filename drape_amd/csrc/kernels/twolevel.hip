@@ -129,29 +129,45 @@ __device__ inline void tile_to_lds(const double* __restrict__ src, double* dst) 
   }
 }
 
-// Entry q of the coarse block pair (I, J) into the band where it lies on or below the diagonal: one thread
-// per (pair, entry), the pair's terms in their fixed order.
+// The coarse block pair (I, J) into the band, entries on or below the diagonal: one wavefront per pair,
+// lane l summing the pair's terms l, l + 64, ... for all nine entries, then a fixed-order wavefront tree, so
+// the sum is reproducible. (One thread per entry walking a pair's few hundred terms in sequence was
+// latency-bound: 248 us per product at n129.)
 __global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restrict__ pair_ptr,
                                                        const uint32_t* __restrict__ pair_key,
                                                        const uint4* __restrict__ terms, const float* __restrict__ w,
                                                        const float* __restrict__ sell_val,
                                                        const uint32_t* __restrict__ block_dst, int npairs, int Nc,
                                                        int kb, double* __restrict__ band) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= 9 * npairs) return;
-  const int p = g / 9, q = g % 9;
+  const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= npairs) return;  // uniform per wavefront
   const uint32_t key = pair_key[p];
   const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
-  const int R = 3 * I + q / 3, C = 3 * J + q % 3;
-  if (R < C) return;
-  double acc = 0.0;
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
   const uint32_t t1 = pair_ptr[p + 1];
-  for (uint32_t t = pair_ptr[p]; t < t1; ++t) {
+  for (uint32_t t = pair_ptr[p] + lane; t < t1; t += 64) {
     const uint4 e = terms[t];
-    acc += (double)w[e.y] * (double)w[e.z] * (double)sell_val[block_dst[e.x] + 64 * q];
+    const double ww = (double)w[e.y] * (double)w[e.z];
+    const float* a = sell_val + block_dst[e.x];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] += ww * (double)a[64 * q];
   }
-  const int k = C >> 6, d = (R >> 6) - k;
-  band_tile(band, kb, k, d)[((R & 63) << 6) + (C & 63)] = acc;
+#pragma unroll
+  for (int q = 0; q < 9; ++q)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_down(acc[q], o, 64);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const int R = 3 * I + q / 3, C = 3 * J + q % 3;
+      if (R < C) continue;
+      const int k = C >> 6, d = (R >> 6) - k;
+      band_tile(band, kb, k, d)[((R & 63) << 6) + (C & 63)] = acc[q];
+    }
+  }
 }
 
 // Diagonal of A_c scaled by `factor`; the padding rows past N3 get a unit diagonal.
@@ -1206,7 +1222,7 @@ static int factorise(Context& c, bool model, bool* ok) {
   if (model && t->d_theta) TRY_RC(run_model(c));
   HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * TILE * TILE * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
-  k_galerkin_band<<<blocks_for(9 * t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
+  k_galerkin_band<<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
                                                         c.sell_val, c.block_dst, t->npairs, t->Nc, t->kb, t->d_band);
   k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(t->d_band, t->kb, t->nt, t->N3, 1.0 + c.rules.coarse_shift);
   const int updates = t->kb * (t->kb + 1) / 2;
