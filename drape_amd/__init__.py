@@ -53,7 +53,7 @@ class Rules(C.Structure):
                 ("preconditioner", C.c_int32), ("coarse_reuse", C.c_int32),
                 ("fallback", C.c_int32), ("coarse_shift", C.c_double), ("smoother", C.c_double),
                 ("coarse_half", C.c_int32), ("refresh_rate", C.c_double), ("refresh_on_breakdown", C.c_int32),
-                ("self_contact", C.c_int32)]
+                ("self_contact", C.c_int32), ("coarse_async", C.c_int32)]
 
 
 class PcgStats(C.Structure):

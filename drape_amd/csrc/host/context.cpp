@@ -324,6 +324,7 @@ void drape_default_rules(drape_rules* r) {
   r->refresh_rate = 2.0;
   r->refresh_on_breakdown = 1;
   r->self_contact = 0;
+  r->coarse_async = 0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }

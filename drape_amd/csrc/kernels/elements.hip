@@ -678,6 +678,7 @@ int launch_evaluate(Context& c, bool exact) {
 int launch_assemble(Context& c) {
   hipStream_t s = c.stream;
   ++c.system_version;
+  two_level_before_assembly(c);
   k_assemble_blocks<<<blocks_for(c.B), 256, 0, s>>>(c.bc_ptr, c.bc_idx, c.lhess, c.block_dst, c.B, c.sell_val);
   k_assemble_rhs<<<blocks_for(c.R), 256, 0, s>>>(c.fc_ptr, c.fc_idx, c.lgrad, c.R, c.b);
   HIP_TRY(hipGetLastError());

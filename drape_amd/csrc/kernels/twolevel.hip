@@ -83,6 +83,20 @@ struct TwoLevel {
     _Float16 *Bh = nullptr, *BTh = nullptr;
   } saved;
   int64_t restores = 0;             // factorisations avoided by returning to the saved one
+  // coarse_async: a factorisation running on `side` into these buffers while the solves keep the current one
+  struct Pending {
+    bool active = false, half_valid = false;
+    bool galerkin = false;          // the next assembly waits for `read`: the product still reads the matrix
+    uint64_t factored = 0, factored_shift = 0;
+    double factored_h = 0.0;
+    float* w = nullptr;
+    double* linv = nullptr;
+    float *B = nullptr, *BT = nullptr;
+    _Float16 *Bh = nullptr, *BTh = nullptr;
+    int* h_info = nullptr;          // pinned
+    hipStream_t side = nullptr;
+    hipEvent_t ready = nullptr, read = nullptr, done = nullptr;
+  } pending;
   float* d_t = nullptr;             // [N3p] B r_c
   float* d_rc = nullptr;            // [N3p]
   float* d_yc = nullptr;            // [N3p]
@@ -94,6 +108,14 @@ struct TwoLevel {
 
 void two_level_destroy(TwoLevel* t) {
   if (!t) return;
+  TwoLevel::Pending& q = t->pending;
+  if (q.side) (void)hipStreamSynchronize(q.side);
+  for (void* p : {(void*)q.w, (void*)q.linv, (void*)q.B, (void*)q.BT, (void*)q.Bh, (void*)q.BTh})
+    if (p) (void)hipFree(p);
+  if (q.h_info) (void)hipHostFree(q.h_info);
+  for (hipEvent_t e : {q.ready, q.read, q.done})
+    if (e) (void)hipEventDestroy(e);
+  if (q.side) (void)hipStreamDestroy(q.side);
   for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_wbil, (void*)t->d_fxy, (void*)t->d_theta,
                   (void*)t->d_gnn, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
                   (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT, (void*)t->d_Bh, (void*)t->d_BTh,
@@ -1035,7 +1057,10 @@ int two_level_create(Context& c, int columns) {
   return DRAPE_OK;
 }
 
+static int adopt_pending(Context& c);
+
 int two_level_set_weights(Context& c, const float* w) {
+  TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
   if (!t) {
     set_error("no two-level layout on this context");
@@ -1056,6 +1081,7 @@ int two_level_set_weights(Context& c, const float* w) {
 // The learned prolongation: from the next factorisation on, the weights come from the network evaluated on
 // each assembled system (null theta: back to the bilinear weights).
 int two_level_set_model(Context& c, const float* theta, int count) {
+  TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
   if (!t) {
     set_error("no two-level layout on this context");
@@ -1078,6 +1104,7 @@ int two_level_set_model(Context& c, const float* theta, int count) {
 }
 
 int two_level_weights(Context& c, float* out) {
+  TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
   HIP_TRY(hipMemcpyAsync(out, t->d_w, 4 * (size_t)c.R * sizeof(float), hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1118,7 +1145,10 @@ static bool use_half(const Context& c, const TwoLevel* t) {
 
 static int factorise(Context& c, bool model, bool* ok);
 
-int two_level_factor_weights(Context& c, bool* ok) { return factorise(c, false, ok); }
+int two_level_factor_weights(Context& c, bool* ok) {
+  TRY_RC(adopt_pending(c));
+  return factorise(c, false, ok);
+}
 
 // Exchanges the current factorisation (its inverse factor copies, diagonal-tile inverses and bookkeeping)
 // with the saved one; `keep` marks the outgoing current one as valid. Allocates the saved buffers on first
@@ -1172,11 +1202,99 @@ static int swap_saved(Context& c, bool keep) {
 
 float* two_level_weights_device(Context& c) { return c.two_level->d_w; }
 
+// Waits for a factorisation running beside the solves and makes it the current one (its buffers and the
+// replaced one's swap).
+static int adopt_pending(Context& c) {
+  TwoLevel* t = c.two_level;
+  TwoLevel::Pending& q = t->pending;
+  if (!q.active) return DRAPE_OK;
+  HIP_TRY(hipEventSynchronize(q.done));
+  c.read_bytes += sizeof(int);
+  std::swap(q.w, t->d_w);
+  std::swap(q.linv, t->d_linv);
+  std::swap(q.B, t->d_B);
+  std::swap(q.BT, t->d_BT);
+  std::swap(q.Bh, t->d_Bh);
+  std::swap(q.BTh, t->d_BTh);
+  std::swap(q.half_valid, t->half_valid);
+  t->factored = q.factored;
+  t->factored_shift = q.factored_shift;
+  t->factored_h = q.factored_h;
+  t->refresh = false;
+  t->solves = 0;
+  t->factorisations++;
+  t->ok = *q.h_info == 0;
+  if (!t->ok) t->failures++;
+  q.active = false;
+  q.galerkin = false;
+  return DRAPE_OK;
+}
+
+void two_level_before_assembly(Context& c) {
+  TwoLevel* t = c.two_level;
+  if (t && t->pending.galerkin) {
+    (void)hipStreamWaitEvent(c.stream, t->pending.read, 0);
+    t->pending.galerkin = false;
+  }
+}
+
+static int factorise_chain(Context& c, hipStream_t s, const float* w, double* linv, float* B, float* BT,
+                           _Float16* Bh, _Float16* BTh, bool half, hipEvent_t after_product);
+
+// The factorisation of the assembled system on the side stream, into the pending buffers; the current one
+// keeps serving until the next system adopts it.
+static int factorise_async(Context& c) {
+  TwoLevel* t = c.two_level;
+  TwoLevel::Pending& q = t->pending;
+  if (!q.side) {
+    const size_t nb = (size_t)t->N3 * t->N3p;
+    HIP_TRY(hipStreamCreateWithFlags(&q.side, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&q.ready, &q.read, &q.done}) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    HIP_TRY(hipMalloc((void**)&q.w, 4 * (size_t)c.R * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&q.linv, (size_t)t->nt * TILE * TILE * sizeof(double)));
+    HIP_TRY(hipMalloc((void**)&q.B, nb * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&q.BT, nb * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&q.Bh, nb * sizeof(_Float16)));
+    HIP_TRY(hipMalloc((void**)&q.BTh, nb * sizeof(_Float16)));
+    HIP_TRY(hipHostMalloc((void**)&q.h_info, sizeof(int)));
+    // the copies are written below/above the diagonal only: the rest stays zero
+    HIP_TRY(hipMemsetAsync(q.B, 0, nb * sizeof(float), c.stream));
+    HIP_TRY(hipMemsetAsync(q.BT, 0, nb * sizeof(float), c.stream));
+    HIP_TRY(hipMemsetAsync(q.Bh, 0, nb * sizeof(_Float16), c.stream));
+    HIP_TRY(hipMemsetAsync(q.BTh, 0, nb * sizeof(_Float16), c.stream));
+  }
+  const auto started = std::chrono::steady_clock::now();
+  // the side stream starts once the assembled system is complete
+  HIP_TRY(hipEventRecord(q.ready, c.stream));
+  HIP_TRY(hipStreamWaitEvent(q.side, q.ready, 0));
+  if (t->d_theta) {
+    GnnArgs a = two_level_gnn_args(c);
+    a.stream = q.side;
+    a.w = q.w;
+    TRY_RC(gnn_infer(a));
+  } else {
+    HIP_TRY(hipMemcpyAsync(q.w, t->d_w, 4 * (size_t)c.R * sizeof(float), hipMemcpyDeviceToDevice, q.side));
+  }
+  q.half_valid = use_half(c, t);
+  TRY_RC(factorise_chain(c, q.side, q.w, q.linv, q.B, q.BT, q.Bh, q.BTh, q.half_valid, q.read));
+  HIP_TRY(hipMemcpyAsync(q.h_info, t->d_info, sizeof(int), hipMemcpyDeviceToHost, q.side));
+  HIP_TRY(hipEventRecord(q.done, q.side));
+  q.active = true;
+  q.galerkin = true;
+  q.factored = c.system_version;
+  q.factored_shift = c.shift_version;
+  q.factored_h = c.shift_h;
+  t->refresh = false;
+  t->factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - started).count();
+  return DRAPE_OK;
+}
+
 // A_c = P^T A P of the assembled system, its Cholesky factor and the factor's inverse, unless the current
 // factorisation may serve this system (same system, or coarse_reuse with unchanged mu and step size and no
 // slowdown). *ok is false when A_c is not positive definite.
 int two_level_prepare(Context& c, bool* ok) {
   TwoLevel* t = c.two_level;
+  TRY_RC(adopt_pending(c));
   const bool fresh = t->factored == c.system_version;
   const int rule = c.rules.coarse_reuse;
   const bool by_h = rule == 3 || rule == 4;
@@ -1197,6 +1315,12 @@ int two_level_prepare(Context& c, bool* ok) {
       *ok = t->ok;
       return DRAPE_OK;
     }
+  }
+  if (c.rules.coarse_async && rule >= 1 && rule <= 3 && t->factored != ~0ull && t->ok) {
+    // refactor beside this solve; the kept factorisation serves it
+    TRY_RC(factorise_async(c));
+    *ok = t->ok;
+    return DRAPE_OK;
   }
   return factorise(c, true, ok);
 }
@@ -1220,24 +1344,8 @@ static int factorise(Context& c, bool model, bool* ok) {
     ~Clock() { t->factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
   } clock{t, started};
   if (model && t->d_theta) TRY_RC(run_model(c));
-  HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * TILE * TILE * sizeof(double), s));
-  HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
-  k_galerkin_band<<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
-                                                        c.sell_val, c.block_dst, t->npairs, t->Nc, t->kb, t->d_band);
-  k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(t->d_band, t->kb, t->nt, t->N3, 1.0 + c.rules.coarse_shift);
-  const int updates = t->kb * (t->kb + 1) / 2;
-  for (int k = 0; k < t->nt; ++k) {
-    k_chol_panel<<<1, 256, 0, s>>>(t->d_band, t->d_linv, k, t->kb, t->d_info);
-    if (k + 1 < t->nt && updates > 0) {
-      k_chol_trsm<<<4 * t->kb, 256, 0, s>>>(t->d_band, t->d_lpanel, t->d_linv, k, t->nt, t->kb, t->d_info);
-      k_chol_update<<<4 * updates, 256, 0, s>>>(t->d_band, t->d_lpanel, k, t->nt, t->kb, t->d_info);
-    }
-  }
   t->half_valid = use_half(c, t);
-  k_band_inverse<<<t->nt * (TILE / INV_COLS), 256, 0, s>>>(t->d_lpanel, t->d_linv, t->nt, t->kb, t->N3, t->N3p, t->d_B,
-                                                           t->d_BT, t->half_valid ? t->d_Bh : nullptr,
-                                                           t->half_valid ? t->d_BTh : nullptr, t->d_info);
-  HIP_TRY(hipGetLastError());
+  TRY_RC(factorise_chain(c, s, t->d_w, t->d_linv, t->d_B, t->d_BT, t->d_Bh, t->d_BTh, t->half_valid, nullptr));
   TRY_RC(read_back(c, t->h_info, t->d_info, sizeof(int)));
   t->factorisations++;
   t->ok = *t->h_info == 0;
@@ -1246,9 +1354,36 @@ static int factorise(Context& c, bool model, bool* ok) {
   return DRAPE_OK;
 }
 
+// Galerkin product with weights w, banded Cholesky and the inverse factor into (linv, B, B^T and, when
+// `half`, the FP16 copies), on stream s; `after_product` (if not null) is recorded once the product has read
+// the assembled matrix.
+static int factorise_chain(Context& c, hipStream_t s, const float* w, double* linv, float* B, float* BT,
+                           _Float16* Bh, _Float16* BTh, bool half, hipEvent_t after_product) {
+  TwoLevel* t = c.two_level;
+  HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * TILE * TILE * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
+  k_galerkin_band<<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, w,
+                                                        c.sell_val, c.block_dst, t->npairs, t->Nc, t->kb, t->d_band);
+  if (after_product) HIP_TRY(hipEventRecord(after_product, s));
+  k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(t->d_band, t->kb, t->nt, t->N3, 1.0 + c.rules.coarse_shift);
+  const int updates = t->kb * (t->kb + 1) / 2;
+  for (int k = 0; k < t->nt; ++k) {
+    k_chol_panel<<<1, 256, 0, s>>>(t->d_band, linv, k, t->kb, t->d_info);
+    if (k + 1 < t->nt && updates > 0) {
+      k_chol_trsm<<<4 * t->kb, 256, 0, s>>>(t->d_band, t->d_lpanel, linv, k, t->nt, t->kb, t->d_info);
+      k_chol_update<<<4 * updates, 256, 0, s>>>(t->d_band, t->d_lpanel, k, t->nt, t->kb, t->d_info);
+    }
+  }
+  k_band_inverse<<<t->nt * (TILE / INV_COLS), 256, 0, s>>>(t->d_lpanel, linv, t->nt, t->kb, t->N3, t->N3p, B, BT,
+                                                           half ? Bh : nullptr, half ? BTh : nullptr, t->d_info);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
 // The Galerkin operator of the assembled system, row-major [3Nc x 3Nc], into host memory (a dense scratch
 // copy allocated on first use). The current factorisation is unaffected.
 int two_level_galerkin(Context& c, double* out) {
+  TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
   const size_t n2 = (size_t)t->N3 * t->N3;
@@ -1264,6 +1399,7 @@ int two_level_galerkin(Context& c, double* out) {
 
 // B = L^-1 of the current factorisation (FP32, row-major [3Nc x 3Nc]) into host memory.
 int two_level_inverse_factor(Context& c, float* out) {
+  TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
   HIP_TRY(hipMemcpy2DAsync(out, (size_t)t->N3 * sizeof(float), t->d_B, (size_t)t->N3p * sizeof(float),
                            (size_t)t->N3 * sizeof(float), t->N3, hipMemcpyDeviceToHost, c.stream));

@@ -108,6 +108,10 @@ typedef struct {
                              spatial-hash broad phase, with its Gauss-Newton Hessian b'' grad d grad d^T; the
                              step bound then also keeps every such pair above 0.1 of its distance. No reference
                              counterpart (the reference's contact is vertex-collider only). Default 0 */
+  int32_t coarse_async;   /* two-level, coarse_reuse 1-3: 1 runs a refactorisation that a kept factorisation
+                             would otherwise wait for on a second stream beside the solve, which keeps the
+                             kept one; the new factorisation serves from the next system on. The first
+                             factorisation and any after a failure stay synchronous. Default 0 */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0

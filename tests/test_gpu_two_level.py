@@ -184,6 +184,64 @@ def test_ptc_n33_two_level_reaches_the_oracle_equilibrium():
     assert per["two_level"] * 2 < per["block_jacobi"]
 
 
+def test_factorisation_beside_the_solve():
+    """coarse_async = 1 runs a refactorisation on a second stream while the kept factorisation serves the
+    solve, and the next system adopts it: over 12 PTC steps of the n65 drape (after 3 warm-up steps) the
+    solve stops as the synchronous one does, with a residual within 2x of it, no failed factorisation, and
+    at most as many factorisations. The factorisations' SPD flags and inverse factors are those of the
+    synchronous kernels (the same launches on another stream), so the check that matters is the schedule."""
+    scene = D.drape_config_scene(65)
+    arrays = D.SceneArrays(scene)
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=1))
+    w = ctx.solve_equilibrium(start=arrays.array("rest_positions"), h_start=ctx.rules.h0, cap=3)
+    start, h = w["positions"], w["result"].final_h
+    ctx.close()
+    out = {}
+    for mode in (0, 1):
+        c = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_async=mode))
+        o = c.solve_equilibrium(start=start, h_start=h, cap=12)
+        r = o["result"]
+        counters = c.two_level_layout()[3]
+        out[mode] = (o["stop"], counters[0], counters[1], r.pcg_iterations, r.final_residual / r.gravity_load)
+        # the adopted factorisation is the one the inverse-factor entry point reports
+        B = c.two_level_inverse_factor()
+        assert np.all(np.isfinite(B))
+        c.close()
+    print(f"[gpu] factorisation beside the solve n65: synchronous {out[0]}, asynchronous {out[1]} (stop, "
+          f"factorisations, failures, PCG iterations, r/L)", flush=True)
+    assert out[1][0] == out[0][0]
+    assert out[1][2] == 0
+    assert out[1][1] <= out[0][1]
+    assert out[1][4] <= 2.0 * out[0][4]
+
+
+def test_factorisation_beside_the_solve_equals_the_synchronous_one():
+    """With coarse_async = 1, a second system at a new step size is solved with the first system's
+    factorisation while its own runs beside the solve; solving it again adopts that factorisation, whose
+    inverse factor is bit-identical to the synchronous context's, and so is the PCG."""
+    arrays = D.SceneArrays(D.drape_config_scene(65))
+    rest = arrays.array("rest_positions")
+    x2 = rest.copy()
+    x2[2::3] -= np.float32(1e-3) * np.sin(np.arange(rest.size // 3, dtype=np.float32))
+    B, its = {}, {}
+    for mode in (0, 1):
+        c = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_async=mode))
+        c.set_state(rest, rest, 1.0 / 60.0, 0.0)
+        c.residual()
+        st1, _ = c.solve_once(1e-3, 2000)
+        c.set_state(x2, x2, 1.0 / 120.0, 0.0)
+        c.residual()
+        st2, _ = c.solve_once(1e-3, 2000)
+        st3, x3 = c.solve_once(1e-3, 2000)
+        B[mode], its[mode] = c.two_level_inverse_factor(), (st1.iterations, st2.iterations, st3.iterations, x3)
+        c.close()
+    print(f"[gpu] factorisation beside the solve: PCG iterations synchronous {its[0][:3]}, asynchronous "
+          f"{its[1][:3]}", flush=True)
+    assert np.array_equal(B[0].view(np.uint32), B[1].view(np.uint32))
+    assert its[1][0] == its[0][0] and its[1][2] == its[0][2]
+    assert np.array_equal(its[0][3].view(np.uint32), its[1][3].view(np.uint32))
+
+
 def test_saved_factorisation_for_alternating_step_sizes():
     """coarse_reuse = 4 keeps the factorisation of the previous step size and returns to it when PTC does:
     on 12 PTC steps of the n65 drape (after 3 warm-up steps), which alternate between two step sizes, it needs
