@@ -221,6 +221,7 @@ int two_level_create(Context& c, int columns);           // coarse layout of a g
 int two_level_set_weights(Context& c, const float* w);  // [4R] parent-slot weights (NULL: bilinear)
 int two_level_prepare(Context& c, bool* ok);            // Galerkin + factorisation when the system changed
 void two_level_before_assembly(Context& c);             // an assembly waits until a side factorisation has read A
+int two_level_launch_deferred(Context& c);              // queue a decided side factorisation (after the solve's first launches)
 int two_level_apply(Context& c, float* partials, int guard);  // z = M^-1 r and partial (r.z, r.r)
 // One two-level PCG iteration after its SpMV: alpha from the p.q partials, x += alpha p, r -= alpha q and
 // z = M^-1 r with partial (r.z, r.r), in the restriction, the coarse solve and the prolongation launches.

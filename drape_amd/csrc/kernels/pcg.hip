@@ -536,6 +536,7 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
       if (npending < 2 && it < max_launch) continue;
     }
     if (npending == 0) break;
+    if (coarse) TRY_RC(two_level_launch_deferred(c));  // a refactorisation beside this solve, now the solve is queued
     const int oldest = pending[0];
     HIP_TRY(hipEventSynchronize(c.ev_poll[oldest]));
     c.read_bytes += (int64_t)sizeof(int32_t);

@@ -86,6 +86,7 @@ struct TwoLevel {
   // coarse_async: a factorisation running on `side` into these buffers while the solves keep the current one
   struct Pending {
     bool active = false, half_valid = false;
+    bool deferred = false;          // decided, waiting for the solve's first launches to be queued
     bool galerkin = false;          // the next assembly waits for `read`: the product still reads the matrix
     uint64_t factored = 0, factored_shift = 0;
     double factored_h = 0.0;
@@ -1208,6 +1209,7 @@ static int adopt_pending(Context& c) {
   TwoLevel* t = c.two_level;
   TwoLevel::Pending& q = t->pending;
   if (!q.active) return DRAPE_OK;
+  TRY_RC(two_level_launch_deferred(c));
   HIP_TRY(hipEventSynchronize(q.done));
   c.read_bytes += sizeof(int);
   std::swap(q.w, t->d_w);
@@ -1232,6 +1234,7 @@ static int adopt_pending(Context& c) {
 
 void two_level_before_assembly(Context& c) {
   TwoLevel* t = c.two_level;
+  if (t && t->pending.deferred) (void)two_level_launch_deferred(c);
   if (t && t->pending.galerkin) {
     (void)hipStreamWaitEvent(c.stream, t->pending.read, 0);
     t->pending.galerkin = false;
@@ -1263,9 +1266,24 @@ static int factorise_async(Context& c) {
     HIP_TRY(hipMemsetAsync(q.Bh, 0, nb * sizeof(_Float16), c.stream));
     HIP_TRY(hipMemsetAsync(q.BTh, 0, nb * sizeof(_Float16), c.stream));
   }
-  const auto started = std::chrono::steady_clock::now();
-  // the side stream starts once the assembled system is complete
+  // the side stream starts once the assembled system is complete; its launches are queued after the
+  // solve's first ones (two_level_launch_deferred), so that the device is not idle while the host queues them
   HIP_TRY(hipEventRecord(q.ready, c.stream));
+  q.active = true;
+  q.deferred = true;
+  q.factored = c.system_version;
+  q.factored_shift = c.shift_version;
+  q.factored_h = c.shift_h;
+  t->refresh = false;
+  return DRAPE_OK;
+}
+
+int two_level_launch_deferred(Context& c) {
+  TwoLevel* t = c.two_level;
+  if (!t || !t->pending.deferred) return DRAPE_OK;
+  TwoLevel::Pending& q = t->pending;
+  q.deferred = false;
+  const auto started = std::chrono::steady_clock::now();
   HIP_TRY(hipStreamWaitEvent(q.side, q.ready, 0));
   if (t->d_theta) {
     GnnArgs a = two_level_gnn_args(c);
@@ -1279,12 +1297,7 @@ static int factorise_async(Context& c) {
   TRY_RC(factorise_chain(c, q.side, q.w, q.linv, q.B, q.BT, q.Bh, q.BTh, q.half_valid, q.read));
   HIP_TRY(hipMemcpyAsync(q.h_info, t->d_info, sizeof(int), hipMemcpyDeviceToHost, q.side));
   HIP_TRY(hipEventRecord(q.done, q.side));
-  q.active = true;
   q.galerkin = true;
-  q.factored = c.system_version;
-  q.factored_shift = c.shift_version;
-  q.factored_h = c.shift_h;
-  t->refresh = false;
   t->factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - started).count();
   return DRAPE_OK;
 }
