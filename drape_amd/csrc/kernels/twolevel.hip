@@ -1329,7 +1329,10 @@ int two_level_prepare(Context& c, bool* ok) {
       return DRAPE_OK;
     }
   }
-  if (c.rules.coarse_async && rule >= 1 && rule <= 3 && t->factored != ~0ull && t->ok) {
+  // 2: only a refresh after a slowdown or breakdown at an unchanged step size goes beside the solve
+  const bool same_h = t->factored_h == c.shift_h;
+  const bool async = c.rules.coarse_async == 1 || (c.rules.coarse_async == 2 && t->refresh && same_h);
+  if (async && rule >= 1 && rule <= 3 && t->factored != ~0ull && t->ok) {
     // refactor beside this solve; the kept factorisation serves it
     TRY_RC(factorise_async(c));
     *ok = t->ok;

@@ -110,8 +110,11 @@ typedef struct {
                              counterpart (the reference's contact is vertex-collider only). Default 0 */
   int32_t coarse_async;   /* two-level, coarse_reuse 1-3: 1 runs a refactorisation that a kept factorisation
                              would otherwise wait for on a second stream beside the solve, which keeps the
-                             kept one; the new factorisation serves from the next system on. The first
-                             factorisation and any after a failure stay synchronous. Default 0 */
+                             kept one; the new factorisation serves from the next system on. 2 does so only
+                             for a refresh after a slowdown or breakdown at an unchanged step size (a new
+                             step size is factored before its first solve). 0 keeps every refactorisation
+                             synchronous. The first factorisation and any after a failure are always
+                             synchronous. Default 0 */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0

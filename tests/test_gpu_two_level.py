@@ -192,7 +192,7 @@ def test_factorisation_beside_the_solve():
     synchronous kernels (the same launches on another stream), so the check that matters is the schedule."""
     scene = D.drape_config_scene(65)
     arrays = D.SceneArrays(scene)
-    ctx = D.Drape(arrays, D.default_rules(preconditioner=1))
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_async=0))
     w = ctx.solve_equilibrium(start=arrays.array("rest_positions"), h_start=ctx.rules.h0, cap=3)
     start, h = w["positions"], w["result"].final_h
     ctx.close()
@@ -250,13 +250,13 @@ def test_saved_factorisation_for_alternating_step_sizes():
     than 2x rule 3's."""
     scene = D.drape_config_scene(65)
     arrays = D.SceneArrays(scene)
-    ctx = D.Drape(arrays, D.default_rules(preconditioner=1))
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_async=0))
     w = ctx.solve_equilibrium(start=arrays.array("rest_positions"), h_start=ctx.rules.h0, cap=3)
     start, h = w["positions"], w["result"].final_h
     ctx.close()
     out = {}
     for rule in (3, 4):
-        c = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_reuse=rule))
+        c = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_reuse=rule, coarse_async=0))
         r = c.solve_equilibrium(start=start, h_start=h, cap=12)["result"]
         out[rule] = (c.two_level_layout()[3][0], r.pcg_iterations, r.final_residual / r.gravity_load)
         c.close()
