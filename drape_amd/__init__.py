@@ -53,7 +53,8 @@ class Rules(C.Structure):
                 ("preconditioner", C.c_int32), ("coarse_reuse", C.c_int32),
                 ("fallback", C.c_int32), ("coarse_shift", C.c_double), ("smoother", C.c_double),
                 ("coarse_half", C.c_int32), ("refresh_rate", C.c_double), ("refresh_on_breakdown", C.c_int32),
-                ("self_contact", C.c_int32), ("coarse_async", C.c_int32)]
+                ("self_contact", C.c_int32), ("coarse_async", C.c_int32), ("block_jacobi_truncate", C.c_int32),
+                ("adaptive_kappa", C.c_int32), ("kappa_distance", C.c_double), ("kappa_max_factor", C.c_double)]
 
 
 class PcgStats(C.Structure):
@@ -87,7 +88,7 @@ class PtcResult(C.Structure):
                 ("pcg_iterations", C.c_int64), ("records", C.c_int32), ("attempts", C.c_int32),
                 ("final_h", C.c_double), ("final_residual", C.c_double), ("gravity_load", C.c_double),
                 ("seconds", C.c_double), ("phase_ms", C.c_double * 7), ("fallbacks", C.c_int64),
-                ("read_bytes", C.c_int64)]
+                ("read_bytes", C.c_int64), ("truncations", C.c_int64), ("breakdowns", C.c_int64)]
 
 
 PHASES = ["assembly", "pcg", "line_search", "ccd", "reduce", "io", "other"]
@@ -161,6 +162,11 @@ def lib():
         L.drape_simulate.argtypes = [vp, C.c_int32, f32p, f32p, f32p, f32p, i32p, f32p, C.POINTER(Record), C.c_int32,
                                      i32p]
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
+        L.drape_ptc_probe.argtypes = [vp, f32p, C.c_double, C.c_int32, C.POINTER(Record), C.c_int32, i32p, f32p,
+                                      f32p]
+        L.drape_set_kappa.argtypes = [vp, C.c_double]
+        L.drape_kappa.argtypes = [vp, f64p]
+        L.drape_kappa_history.argtypes = [vp, f64p, C.c_int32, i32p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_build_device.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_contributions.argtypes = [vp, C.POINTER(C.c_int64), u32p, u32p, u32p, u32p, u32p]
@@ -549,6 +555,35 @@ class Drape:
                                     _ptr(up, i32p), _ptr(rel, f32p), recs, keep, C.byref(n)))
         return {"positions": x, "velocities": v, "updates": up[:steps], "relative": rel[:steps],
                 "records": [recs[i] for i in range(min(n.value, keep))]}
+
+    def ptc_probe(self, start, h, step=1, cap=64, iterates=False):
+        """One PTC attempt from `start` at step size h (runPtcProbes, drape-e7.ts:183-196): the last iterate,
+        the records and, with `iterates`, the positions at each record [n, 3V]."""
+        recs = (Record * cap)()
+        n = C.c_int32()
+        final = np.zeros(3 * self.V, np.float32)
+        its = np.zeros((cap, 3 * self.V), np.float32) if iterates else None
+        _check(lib().drape_ptc_probe(self.h, _ptr(_f32(start), f32p), h, step, recs, cap, C.byref(n),
+                                     None if its is None else _ptr(its, f32p), _ptr(final, f32p)))
+        k = min(n.value, cap)
+        return {"positions": final, "records": [recs[i] for i in range(k)],
+                "iterates": None if its is None else its[:k].copy()}
+
+    @property
+    def kappa(self):
+        out = C.c_double()
+        _check(lib().drape_kappa(self.h, C.byref(out)))
+        return out.value
+
+    def set_kappa(self, kappa):
+        _check(lib().drape_set_kappa(self.h, kappa))
+
+    def kappa_history(self):
+        n = C.c_int32()
+        _check(lib().drape_kappa_history(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        _check(lib().drape_kappa_history(self.h, _ptr(out, f64p), n.value, C.byref(n)))
+        return out
 
     def solve_equilibrium(self, start=None, h_start=None, cap=400, budget_s=0.0, keep=20000, keep_states=False):
         """solveEquilibrium (drape-e7.ts:156-181). "states" holds the start and every accepted state when

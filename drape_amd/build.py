@@ -33,18 +33,29 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compiles every source (in parallel, `jobs` processes; 0 = one per source up to the CPU count) and
+    links the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(HERE, "_build", os.path.basename(src) + ".o")
-        cmd = [hipcc, *[f for f in FLAGS if f != "-shared"], "-x", "hip", "-c", path, "-o", obj]
+        cmds.append([hipcc, *[f for f in FLAGS if f != "-shared"], "-x", "hip", "-c", path, "-o", obj])
+        objs.append(obj)
+    workers = jobs if jobs > 0 else max(1, min(len(cmds), os.cpu_count() or 1, 16))
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)

@@ -34,8 +34,19 @@ template <class T> static int upload(T* dst, const T* src, size_t n, hipStream_t
     if (_rc != DRAPE_OK) return _rc; \
   } while (0)
 
+int set_kappa(Context& c, float kappa) {
+  c.colliders_host[2] = kappa;
+  self_contact_set_kappa(c, kappa);
+  HIP_TRY(hipMemcpyAsync(c.colliders + 2, &c.colliders_host[2], sizeof(float), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  return DRAPE_OK;
+}
+
 static void free_context(Context* c) {
   if (!c) return;
+  // first: a factorisation still running on the two-level side stream reads the matrix and the GNN inputs
+  if (c->two_level) two_level_destroy(c->two_level);
+  c->two_level = nullptr;
   void* ptrs[] = {c->pos, c->xhat, c->dir, c->tri, c->geom0, c->geom1, c->hin, c->hstiff, c->rvert, c->rmass,
                   c->colliders, c->mu, c->unit_mass_dev, c->org, c->material, c->lgrad, c->lhess, c->bc_ptr,
                   c->bc_idx, c->fc_ptr, c->fc_idx, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->perm, c->diag_slot,
@@ -46,7 +57,6 @@ static void free_context(Context* c) {
   if (c->state_host) (void)hipHostFree(c->state_host);
   if (c->poll_host) (void)hipHostFree(c->poll_host);
   if (c->dout_host) (void)hipHostFree(c->dout_host);
-  if (c->two_level) two_level_destroy(c->two_level);
   if (c->gnn_train) gnn_train_destroy(c->gnn_train);
   if (c->self_contact) self_contact_destroy(c->self_contact);
   for (hipEvent_t e : {c->ev0, c->ev1, c->ev_spmv0, c->ev_spmv1, c->ev_poll[0], c->ev_poll[1]})
@@ -220,6 +230,7 @@ static int create_mesh_context(const drape_problem* pr, const drape_rules* rules
     }
     c.colliders_host.assign(pr->colliders, pr->colliders + 4 + 8 * c.K);
     c.colliders_host[3] = 0.0f;
+    c.kappa0 = c.colliders_host[2];
   }
   // element records
   std::vector<int4> tri(c.T), hin(c.H);
@@ -325,6 +336,10 @@ void drape_default_rules(drape_rules* r) {
   r->refresh_on_breakdown = 1;
   r->self_contact = 0;
   r->coarse_async = 0;
+  r->block_jacobi_truncate = 0;
+  r->adaptive_kappa = 0;
+  r->kappa_distance = 0.1;
+  r->kappa_max_factor = 100.0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }
@@ -473,6 +488,30 @@ int drape_set_colliders(drape_ctx* p, int32_t enabled) {
   c->colliders_host[0] = enabled ? (float)c->K : 0.0f;
   HIP_TRY(hipMemcpyAsync(c->colliders, &c->colliders_host[0], sizeof(float), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return DRAPE_OK;
+}
+
+int drape_set_kappa(drape_ctx* p, double kappa) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear || !(kappa > 0) || !std::isfinite(kappa)) { set_error("invalid argument"); return DRAPE_E_INVALID; }
+  if (c->K == 0) return DRAPE_OK;
+  return set_kappa(*c, (float)kappa);
+}
+
+int drape_kappa(drape_ctx* p, double* kappa) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear || !kappa) { set_error("invalid argument"); return DRAPE_E_INVALID; }
+  *kappa = c->K > 0 ? (double)c->colliders_host[2] : 0.0;
+  return DRAPE_OK;
+}
+
+int drape_kappa_history(drape_ctx* p, double* kappa, int32_t cap, int32_t* count) {
+  Context* c = ctx_of(p);
+  if (!c || c->linear || cap < 0) { set_error("invalid argument"); return DRAPE_E_INVALID; }
+  const int32_t n = (int32_t)c->kappa_history.size();
+  if (count) *count = n;
+  if (kappa)
+    for (int32_t i = 0; i < n && i < cap; ++i) kappa[i] = c->kappa_history[i];
   return DRAPE_OK;
 }
 

@@ -39,6 +39,24 @@ struct Timer {
   ~Timer() { *slot += std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
 };
 
+// newton.ts:108-111 record: appends the record and, when iterates are kept, the positions at that moment.
+int record(Run& run, const drape_record& e) {
+  run.records.push_back(e);
+  Context& c = run.c;
+  if (!c.keep_iterates) return DRAPE_OK;
+  Timer t(run, IO);
+  std::vector<float4> v(c.V);
+  TRY_RC(read_back(c, v.data(), c.pos, v.size() * sizeof(float4)));
+  const size_t at = c.iterates.size();
+  c.iterates.resize(at + 3 * (size_t)c.V);
+  for (int i = 0; i < c.V; ++i) {
+    c.iterates[at + 3 * i] = v[i].x;
+    c.iterates[at + 3 * i + 1] = v[i].y;
+    c.iterates[at + 3 * i + 2] = v[i].z;
+  }
+  return DRAPE_OK;
+}
+
 struct StepState {
   double mu = 0.0, eta = 0.0, previous_norm = NAN;
 };
@@ -177,10 +195,7 @@ int update(Run& run, StepState& state, int step, int iteration, float relative, 
   bool capped = false;
   int rc = solve_direction(run, state, &solved, &capped);
   if (rc != DRAPE_OK) return rc;
-  if (capped) {
-    run.records.push_back(terminal(step, iteration, relative, REGULARIZATION_CAP, state.mu));
-    return DRAPE_OK;
-  }
+  if (capped) return record(run, terminal(step, iteration, relative, REGULARIZATION_CAP, state.mu));
   float alpha_max = 1.0f, min_dist = INFINITY;
   int32_t limit_row = -1, min_row = -1;
   rc = launch_scatter_direction(c, c.x);
@@ -233,7 +248,8 @@ int update(Run& run, StepState& state, int step, int iteration, float relative, 
   e.slope_exact = NAN;
   e.norm_squared = res.norm_squared;
   e.norm_squared_exact = NAN;
-  run.records.push_back(e);
+  rc = record(run, e);
+  if (rc != DRAPE_OK) return rc;
   if (search.found && run.rules.regularize) state.mu = search.halvings == 0 ? relax_mu(run.rules, state.mu) : state.mu;
   *extra_out = e;
   *cont = search.found;
@@ -251,11 +267,9 @@ int solve_step_robust(Run& run, float scale, int step) {
     rc = residual(run, &res);
     if (rc != DRAPE_OK) return rc;
     const float relative = f32((double)res.norm / (double)scale);
-    if (relative <= run.rules.newton_tolerance || iteration == run.rules.newton_max_iterations) {
-      run.records.push_back(terminal(step, iteration, relative,
-                                     relative <= run.rules.newton_tolerance ? CONVERGED : ITERATION_LIMIT, state.mu));
-      return DRAPE_OK;
-    }
+    if (relative <= run.rules.newton_tolerance || iteration == run.rules.newton_max_iterations)
+      return record(run, terminal(step, iteration, relative,
+                                  relative <= run.rules.newton_tolerance ? CONVERGED : ITERATION_LIMIT, state.mu));
     if (run.rules.forcing)
       state.eta = iteration == 0 ? run.rules.eta_start
                                  : next_forcing(run.rules, state.eta, f32((double)res.norm / state.previous_norm));
@@ -274,8 +288,7 @@ int solve_step_robust(Run& run, float scale, int step) {
       if (rc != DRAPE_OK) return rc;
       rc = residual(run, &res);
       if (rc != DRAPE_OK) return rc;
-      run.records.push_back(terminal(step, iteration + 1, f32((double)res.norm / (double)scale), UPDATE_RULE, state.mu));
-      return DRAPE_OK;
+      return record(run, terminal(step, iteration + 1, f32((double)res.norm / (double)scale), UPDATE_RULE, state.mu));
     }
   }
 }
@@ -367,7 +380,7 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
               std::vector<drape_ptc_step>& steps, std::vector<drape_ptc_attempt>& attempts,
               std::vector<drape_record>& records_out) {
   Run run(c);
-  const int64_t fallbacks0 = c.fallbacks, read0 = c.read_bytes;
+  const int64_t fallbacks0 = c.fallbacks, read0 = c.read_bytes, trunc0 = c.truncations, breaks0 = c.breakdowns;
   c.kept_states.clear();
   TRY_RC(keep_state(run, c.org));
   run.rules.newton_tolerance = c.rules.inner_tolerance;
@@ -477,18 +490,39 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
   }
   result->phase_ms[OTHER] = 1000.0 * seconds - tracked;
   result->fallbacks = c.fallbacks - fallbacks0;
+  result->truncations = c.truncations - trunc0;
+  result->breakdowns = c.breakdowns - breaks0;
   result->read_bytes = c.read_bytes - read0;
   records_out = std::move(run.records);
   return DRAPE_OK;
 }
 
+// The barrier-stiffness update of Incremental Potential Contact (Li et al. 2020, section 4.3), applied after
+// each time step: when the smallest vertex-collider distance is below kappa_distance * dhat and smaller than
+// after the previous step, kappa doubles (exactly, in FP32), up to kappa_max_factor times its start value.
+int adapt_kappa(Context& c, float* previous_min) {
+  float d = INFINITY;
+  TRY_RC(launch_min_distance(c, &d));
+  const float dhat = c.colliders_host[1];
+  const float kappa = c.colliders_host[2];
+  const float cap = (float)(c.rules.kappa_max_factor * c.kappa0);
+  if (d < (float)(c.rules.kappa_distance * dhat) && d < *previous_min && kappa < cap)
+    TRY_RC(set_kappa(c, std::fmin(2.0f * kappa, cap)));
+  *previous_min = d;
+  return DRAPE_OK;
+}
+
 // drape.rs:426-439 simulate / time_step: per step x_hat = x + h v + h^2 g, the residual scale |g(x_n)|, the
-// start (x_hat, cut by CCD when colliders exist), one robust Newton solve, v = (x_{n+1} - x_n) / h.
+// start (x_hat, cut by CCD when colliders exist), one robust Newton solve, v = (x_{n+1} - x_n) / h; then,
+// under rules.adaptive_kappa, the stiffness update.
 int simulate(Context& c, int steps, std::vector<int32_t>& updates, std::vector<float>& relative,
              std::vector<drape_record>& records_out) {
   Run run(c);
   const double h = c.problem.time_step;
+  c.kappa_history.clear();
+  float previous_min = INFINITY;
   for (int n = 0; n < steps; ++n) {
+    c.kappa_history.push_back(c.K > 0 ? (double)c.colliders_host[2] : 0.0);
     HIP_TRY(hipMemcpyAsync(c.org, c.pos, (size_t)c.V * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     int rc = launch_dyn_predict(c, c.vel, h);
     if (rc != DRAPE_OK) return rc;
@@ -511,7 +545,25 @@ int simulate(Context& c, int steps, std::vector<int32_t>& updates, std::vector<f
     relative.push_back(run.records.empty() ? NAN : (float)run.records.back().relative_residual);
     rc = launch_dyn_velocity(c, c.vel, h);
     if (rc != DRAPE_OK) return rc;
+    if (c.rules.adaptive_kappa && c.K > 0) TRY_RC(adapt_kappa(c, &previous_min));
   }
+  records_out = std::move(run.records);
+  return DRAPE_OK;
+}
+
+// drape-e7.ts:183-196 runPtcProbes, one probe: prepare x_hat and M/h^2 at `start` (c.org), the residual scale
+// there, the Newton start and one robust Newton solve with the PTC inner settings.
+int ptc_probe(Context& c, double h, int step, std::vector<drape_record>& records_out) {
+  Run run(c);
+  run.rules.newton_tolerance = c.rules.inner_tolerance;
+  run.rules.newton_max_iterations = c.rules.inner_max_iterations;
+  HIP_TRY(hipMemcpyAsync(c.pos, c.org, (size_t)c.V * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+  TRY_RC(launch_ptc_prepare(c, c.org, h));
+  TRY_RC(write_mu(run, 0.0));
+  Residual res;
+  TRY_RC(residual(run, &res));
+  TRY_RC(newton_start(run));
+  TRY_RC(solve_step_robust(run, res.norm, step));
   records_out = std::move(run.records);
   return DRAPE_OK;
 }
@@ -630,6 +682,41 @@ int drape_ptc_solve(drape_ctx* p, const float* start, double h_start, int32_t ca
     for (int i = 0; i < (int)at.size() && i < attempts_cap; ++i) attempts[i] = at[i];
   if (records)
     for (int i = 0; i < (int)recs.size() && i < records_cap; ++i) records[i] = recs[i];
+  if (final_positions) {
+    HIP_TRY(hipMemcpyAsync(v.data(), c->pos, v.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->V; ++i) {
+      final_positions[3 * i] = v[i].x; final_positions[3 * i + 1] = v[i].y; final_positions[3 * i + 2] = v[i].z;
+    }
+  }
+  return DRAPE_OK;
+}
+
+int drape_ptc_probe(drape_ctx* p, const float* start, double h, int32_t step, drape_record* records, int32_t cap,
+                    int32_t* count, float* iterates, float* final_positions) {
+  Context* c = reinterpret_cast<Context*>(p);
+  if (!c || c->linear || !start || !(h > 0) || cap < 0) {
+    set_error("invalid probe arguments");
+    return DRAPE_E_INVALID;
+  }
+  std::vector<float4> v(c->V);
+  for (int i = 0; i < c->V; ++i) v[i] = make_float4(start[3 * i], start[3 * i + 1], start[3 * i + 2], 0.0f);
+  HIP_TRY(hipMemcpyAsync(c->org, v.data(), v.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  c->keep_iterates = iterates != nullptr;
+  c->iterates.clear();
+  std::vector<drape_record> recs;
+  int rc = ptc_probe(*c, h, step, recs);
+  c->keep_iterates = false;
+  if (rc != DRAPE_OK) return rc;
+  const int n = (int)recs.size();
+  if (records)
+    for (int i = 0; i < n && i < cap; ++i) records[i] = recs[i];
+  if (count) *count = n;
+  const size_t per = 3 * (size_t)c->V;
+  if (iterates)
+    for (int i = 0; i < n && i < cap; ++i)
+      std::copy(c->iterates.begin() + i * per, c->iterates.begin() + (i + 1) * per, iterates + i * per);
+  c->iterates.clear();
   if (final_positions) {
     HIP_TRY(hipMemcpyAsync(v.data(), c->pos, v.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -117,10 +117,21 @@ struct Context {
   uint64_t shift_version = 0;     // bumped when mu or the step size of M/h^2 changes
   float shift_mu = 0.0f;
   double shift_h = 0.0;
-  int64_t fallbacks = 0;          // two-level solves redone with block-Jacobi
+  int64_t fallbacks = 0;          // two-level solves redone with block-Jacobi (non-SPD A_c, or fallback = 1)
+  int64_t truncations = 0;        // solves returned truncated at negative curvature (status 4)
+  int64_t breakdowns = 0;         // solves that reported a breakdown (status 3) to their caller
   int64_t read_bytes = 0;         // device-to-host bytes read back by the solve loops
   int64_t block_contribs = 0;     // entries of bc_idx and fc_idx
   int64_t force_contribs = 0;
+
+  // barrier stiffness: the start value (colliders[2] at creation) and the value of each step of the last
+  // drape_simulate
+  double kappa0 = 0.0;
+  std::vector<double> kappa_history;
+
+  // Newton iterates kept by a probe (positions after each accepted update, [count][3V]) when keep_iterates
+  bool keep_iterates = false;
+  std::vector<float> iterates;
 
   // accepted PTC states kept for the caller: every state when keep_states, else the start and the last
   bool keep_states = false;
@@ -185,6 +196,7 @@ int launch_scatter_direction(Context& c, const float4* rows_src);  // dir[vertex
 int launch_advance(Context& c, float alpha);        // pos[v] += alpha * dir[v] on rows
 int launch_energy_change(Context& c, float alpha, double* sum, double* abs_sum);
 int launch_max_step(Context& c, float* alpha, int32_t* row, float* min_distance, int32_t* min_row);
+int launch_min_distance(Context& c, float* min_distance);  // smallest vertex-collider distance at pos
 // out = {max|solution row|, max|pos - org| on rows, max|pos| on rows, non-finite count}
 int launch_row_maxes(Context& c, const float4* org, float out[4]);
 int launch_towards(Context& c);                     // dir = x_hat - pos (all vertices)
@@ -211,6 +223,7 @@ void self_contact_moved(Context& c);
 int self_contact_step_bound(Context& c, float* alpha, int32_t* row);
 int self_contact_energy_change(Context& c, float alpha, double* sum, double* abs_sum);
 int self_contact_pairs(Context& c, int32_t* n_pt, int32_t* n_ee, int32_t* vertices, double* distances);
+void self_contact_set_kappa(Context& c, double kappa);  // follows the colliders' kappa
 
 // ---- PCG (pcg.hip) --------------------------------------------------------------------------
 int pcg_solve(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats);
@@ -220,7 +233,7 @@ void two_level_destroy(TwoLevel* t);
 int two_level_create(Context& c, int columns);           // coarse layout of a grid sheet, bilinear weights
 int two_level_set_weights(Context& c, const float* w);  // [4R] parent-slot weights (NULL: bilinear)
 int two_level_prepare(Context& c, bool* ok);            // Galerkin + factorisation when the system changed
-void two_level_before_assembly(Context& c);             // an assembly waits until a side factorisation has read A
+int two_level_before_assembly(Context& c);              // an assembly waits until a side factorisation has read A
 int two_level_launch_deferred(Context& c);              // queue a decided side factorisation (after the solve's first launches)
 int two_level_apply(Context& c, float* partials, int guard);  // z = M^-1 r and partial (r.z, r.r)
 // One two-level PCG iteration after its SpMV: alpha from the p.q partials, x += alpha p, r -= alpha q and
@@ -243,7 +256,7 @@ int two_level_set_model(Context& c, const float* theta, int count);  // learned 
 int two_level_factor_weights(Context& c, bool* ok);
 float* two_level_weights_device(Context& c);  // [4R] current weights on the device
 int two_level_weights(Context& c, float* out);         // [4R] weights of the current factorisation
-int two_level_info(const Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
+int two_level_info(Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
                    int64_t* failures, double* factor_ms);
 
 // ---- learned prolongation (gnn.hip) ---------------------------------------------------------
@@ -275,6 +288,9 @@ void gnn_train_destroy(GnnTrain* g);
 int gnn_train_set_vectors(Context& c, const float* vectors, int count);
 int gnn_train_make_vectors(Context& c, int count, int sweeps, float omega, uint64_t seed, float* out);
 int gnn_train_gradient(Context& c, const float* params, int count, double* loss, float* grad);
+
+// ---- host state (context.cpp) -----------------------------------------------------------------
+int set_kappa(Context& c, float kappa);  // the colliders' barrier stiffness, host copy and device
 
 // ---- helpers --------------------------------------------------------------------------------
 inline int blocks_for(int n, int per = WG) { return (n + per - 1) / per; }

@@ -678,7 +678,7 @@ int launch_evaluate(Context& c, bool exact) {
 int launch_assemble(Context& c) {
   hipStream_t s = c.stream;
   ++c.system_version;
-  two_level_before_assembly(c);
+  TRY_RC(two_level_before_assembly(c));
   k_assemble_blocks<<<blocks_for(c.B), 256, 0, s>>>(c.bc_ptr, c.bc_idx, c.lhess, c.block_dst, c.B, c.sell_val);
   k_assemble_rhs<<<blocks_for(c.R), 256, 0, s>>>(c.fc_ptr, c.fc_idx, c.lgrad, c.R, c.b);
   HIP_TRY(hipGetLastError());
@@ -761,6 +761,18 @@ int launch_max_step(Context& c, float* alpha, int32_t* row, float* min_distance,
       *row = r;
     }
   }
+  return DRAPE_OK;
+}
+
+int launch_min_distance(Context& c, float* min_distance) {
+  *min_distance = INFINITY;
+  if (c.K == 0) return DRAPE_OK;
+  hipStream_t s = c.stream;
+  k_ccd<<<c.G, 256, 0, s>>>(c.pos, c.dir, c.rvert, c.colliders, c.R, c.fpart, c.ipart);
+  k_ccd_final<<<1, 256, 0, s>>>(c.fpart, c.ipart, c.G, c.dout);
+  HIP_TRY(hipGetLastError());
+  TRY_RC(read_back(c, c.dout_host, c.dout, 4 * sizeof(double)));
+  *min_distance = (float)c.dout_host[2];
   return DRAPE_OK;
 }
 

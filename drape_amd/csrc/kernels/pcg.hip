@@ -569,9 +569,21 @@ static int pcg_run(Context& c, double tolerance, int max_iterations, drape_pcg_s
   return DRAPE_OK;
 }
 
-// The preconditioner the rules select. A two-level solve falls back to block-Jacobi when the coarse
-// operator is not positive definite, and on a breakdown either re-solves with block-Jacobi or reports
-// the breakdown (rules.fallback); every such event is counted.
+// Truncated CG: a solve that met p^T A p <= 0 returns its last iterate before the negative curvature
+// (status 4) when that iterate reduced the residual; the caller then steps along a descent direction.
+static bool truncate_breakdown(Context& c, drape_pcg_stats* stats) {
+  if (stats->iterations > 0 && stats->recursive_relative_residual < 1.0) {
+    stats->status = 4;
+    c.truncations++;
+    return true;
+  }
+  return false;
+}
+
+// The preconditioner the rules select. A two-level solve is redone with block-Jacobi when the coarse operator
+// is not positive definite; on a breakdown it is truncated (rules.fallback = 2), redone with block-Jacobi (1)
+// or reported (0). A block-Jacobi breakdown is truncated under rules.block_jacobi_truncate, else reported.
+// Truncations, reported breakdowns and block-Jacobi re-solves are counted apart.
 static int pcg_solve_once(Context& c, double tolerance, int max_iterations, drape_pcg_stats* stats) {
   if (c.rules.preconditioner == DRAPE_PRECOND_TWO_LEVEL && c.two_level) {
     bool ok = false;
@@ -583,18 +595,20 @@ static int pcg_solve_once(Context& c, double tolerance, int max_iterations, drap
         return DRAPE_OK;
       }
       two_level_observe(c, -1, 0.0);
-      c.fallbacks++;
-      // truncation returns the last iterate before the negative curvature, if it improved on x = 0
-      if (c.rules.fallback == 2 && stats->iterations > 0 && stats->recursive_relative_residual < 1.0) {
-        stats->status = 4;
+      if (c.rules.fallback == 2 && truncate_breakdown(c, stats)) return DRAPE_OK;
+      if (c.rules.fallback != 1) {
+        c.breakdowns++;
         return DRAPE_OK;
       }
-      if (c.rules.fallback != 1) return DRAPE_OK;
-    } else {
-      c.fallbacks++;
     }
+    c.fallbacks++;
   }
-  return pcg_run(c, tolerance, max_iterations, stats, false);
+  TRY_RC(pcg_run(c, tolerance, max_iterations, stats, false));
+  if (stats->status == 3) {
+    if (c.rules.block_jacobi_truncate && truncate_breakdown(c, stats)) return DRAPE_OK;
+    c.breakdowns++;
+  }
+  return DRAPE_OK;
 }
 
 // ---- FP64 iterative refinement around the FP32 PCG ------------------------------------------------

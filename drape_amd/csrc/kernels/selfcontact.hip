@@ -1188,6 +1188,10 @@ int ensure_parts(SelfContact* sc, int n) {
 
 }  // namespace
 
+void self_contact_set_kappa(Context& c, double kappa) {
+  if (c.self_contact) c.self_contact->kappa = kappa;
+}
+
 int self_contact_evaluate(Context& c) {
   if (!c.rules.self_contact) return DRAPE_OK;
   TRY_RC(setup(c));

@@ -97,6 +97,7 @@ struct TwoLevel {
     int* h_info = nullptr;          // pinned
     hipStream_t side = nullptr;
     hipEvent_t ready = nullptr, read = nullptr, done = nullptr;
+    hipEvent_t began = nullptr, ended = nullptr;  // timing events around the chain on the side stream
   } pending;
   float* d_t = nullptr;             // [N3p] B r_c
   float* d_rc = nullptr;            // [N3p]
@@ -104,7 +105,8 @@ struct TwoLevel {
   int* d_info = nullptr;
   int* h_info = nullptr;
   int64_t factorisations = 0, failures = 0;
-  double factor_ms = 0.0;     // wall time of all factorisations (Galerkin product, Cholesky, inverse)
+  double factor_ms = 0.0;     // time of all factorisations (Galerkin product, Cholesky, inverse): wall time of the
+                              // synchronous ones, device time (events on the side stream) of the side-stream ones
 };
 
 void two_level_destroy(TwoLevel* t) {
@@ -114,7 +116,7 @@ void two_level_destroy(TwoLevel* t) {
   for (void* p : {(void*)q.w, (void*)q.linv, (void*)q.B, (void*)q.BT, (void*)q.Bh, (void*)q.BTh})
     if (p) (void)hipFree(p);
   if (q.h_info) (void)hipHostFree(q.h_info);
-  for (hipEvent_t e : {q.ready, q.read, q.done})
+  for (hipEvent_t e : {q.ready, q.read, q.done, q.began, q.ended})
     if (e) (void)hipEventDestroy(e);
   if (q.side) (void)hipStreamDestroy(q.side);
   for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_wbil, (void*)t->d_fxy, (void*)t->d_theta,
@@ -1212,6 +1214,9 @@ static int adopt_pending(Context& c) {
   TRY_RC(two_level_launch_deferred(c));
   HIP_TRY(hipEventSynchronize(q.done));
   c.read_bytes += sizeof(int);
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, q.began, q.ended));
+  t->factor_ms += ms;
   std::swap(q.w, t->d_w);
   std::swap(q.linv, t->d_linv);
   std::swap(q.B, t->d_B);
@@ -1232,13 +1237,31 @@ static int adopt_pending(Context& c) {
   return DRAPE_OK;
 }
 
-void two_level_before_assembly(Context& c) {
+// Drops a side-stream factorisation whose chain could not be queued completely: the side stream is drained
+// and nothing is adopted (the kept factorisation stays current).
+static void abandon_pending(TwoLevel* t) {
+  TwoLevel::Pending& q = t->pending;
+  if (q.side) (void)hipStreamSynchronize(q.side);
+  q.active = false;
+  q.deferred = false;
+  q.galerkin = false;
+}
+
+int two_level_before_assembly(Context& c) {
   TwoLevel* t = c.two_level;
-  if (t && t->pending.deferred) (void)two_level_launch_deferred(c);
-  if (t && t->pending.galerkin) {
-    (void)hipStreamWaitEvent(c.stream, t->pending.read, 0);
+  if (!t) return DRAPE_OK;
+  if (t->pending.deferred) {
+    const int rc = two_level_launch_deferred(c);
+    if (rc != DRAPE_OK) {
+      abandon_pending(t);
+      return rc;
+    }
+  }
+  if (t->pending.galerkin) {
+    HIP_TRY(hipStreamWaitEvent(c.stream, t->pending.read, 0));
     t->pending.galerkin = false;
   }
+  return DRAPE_OK;
 }
 
 static int factorise_chain(Context& c, hipStream_t s, const float* w, double* linv, float* B, float* BT,
@@ -1253,6 +1276,7 @@ static int factorise_async(Context& c) {
     const size_t nb = (size_t)t->N3 * t->N3p;
     HIP_TRY(hipStreamCreateWithFlags(&q.side, hipStreamNonBlocking));
     for (hipEvent_t* e : {&q.ready, &q.read, &q.done}) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    for (hipEvent_t* e : {&q.began, &q.ended}) HIP_TRY(hipEventCreate(e));
     HIP_TRY(hipMalloc((void**)&q.w, 4 * (size_t)c.R * sizeof(float)));
     HIP_TRY(hipMalloc((void**)&q.linv, (size_t)t->nt * TILE * TILE * sizeof(double)));
     HIP_TRY(hipMalloc((void**)&q.B, nb * sizeof(float)));
@@ -1278,13 +1302,24 @@ static int factorise_async(Context& c) {
   return DRAPE_OK;
 }
 
+static int launch_chain(Context& c);
+
+// Queues a decided side-stream factorisation. A failure part-way leaves nothing to adopt: the side stream is
+// drained and the pending state cleared, so the kept factorisation stays current.
 int two_level_launch_deferred(Context& c) {
   TwoLevel* t = c.two_level;
   if (!t || !t->pending.deferred) return DRAPE_OK;
+  t->pending.deferred = false;
+  const int rc = launch_chain(c);
+  if (rc != DRAPE_OK) abandon_pending(t);
+  return rc;
+}
+
+static int launch_chain(Context& c) {
+  TwoLevel* t = c.two_level;
   TwoLevel::Pending& q = t->pending;
-  q.deferred = false;
-  const auto started = std::chrono::steady_clock::now();
   HIP_TRY(hipStreamWaitEvent(q.side, q.ready, 0));
+  HIP_TRY(hipEventRecord(q.began, q.side));
   if (t->d_theta) {
     GnnArgs a = two_level_gnn_args(c);
     a.stream = q.side;
@@ -1295,10 +1330,10 @@ int two_level_launch_deferred(Context& c) {
   }
   q.half_valid = use_half(c, t);
   TRY_RC(factorise_chain(c, q.side, q.w, q.linv, q.B, q.BT, q.Bh, q.BTh, q.half_valid, q.read));
+  HIP_TRY(hipEventRecord(q.ended, q.side));
   HIP_TRY(hipMemcpyAsync(q.h_info, t->d_info, sizeof(int), hipMemcpyDeviceToHost, q.side));
   HIP_TRY(hipEventRecord(q.done, q.side));
   q.galerkin = true;
-  t->factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - started).count();
   return DRAPE_OK;
 }
 
@@ -1488,13 +1523,14 @@ int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32) 
   return DRAPE_OK;
 }
 
-int two_level_info(const Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
+int two_level_info(Context& c, int32_t* coarse_rows, int32_t* parent, float* weights, int64_t* factorisations,
                    int64_t* failures, double* factor_ms) {
   const TwoLevel* t = c.two_level;
   if (!t) {
     set_error("no two-level layout on this context");
     return DRAPE_E_INVALID;
   }
+  TRY_RC(adopt_pending(c));  // a factorisation still running beside the solves is waited for and counted
   if (coarse_rows) *coarse_rows = t->Nc;
   if (parent) std::copy(t->parent.begin(), t->parent.end(), parent);
   if (weights) std::copy(t->bilinear.begin(), t->bilinear.end(), weights);

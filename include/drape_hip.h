@@ -115,6 +115,19 @@ typedef struct {
                              step size is factored before its first solve). 0 keeps every refactorisation
                              synchronous. The first factorisation and any after a failure are always
                              synchronous. Default 0 */
+  int32_t block_jacobi_truncate; /* 1: a block-Jacobi solve that meets p^T A p <= 0 returns the last iterate
+                             before it (status 4) when that reduced the residual, the rule rules.fallback = 2
+                             applies to two-level solves, so both preconditioners can run under one
+                             negative-curvature policy; 0 (default) reports the breakdown to the Newton loop,
+                             which raises mu (the reference's rule, robust-newton.ts:121-133) */
+  int32_t adaptive_kappa; /* 1: the barrier stiffness kappa adapts once per time step of drape_simulate by the
+                             update of Incremental Potential Contact (Li et al. 2020, section 4.3): after a step
+                             whose smallest vertex-collider distance is below kappa_distance * dhat and smaller
+                             than the previous step's, kappa doubles, up to kappa_max_factor times its start
+                             value (the reference's resting_kappa, contact.rs:103-107); kappa never decreases.
+                             0 (default) keeps the start value (the reference's fixed kappa) */
+  double kappa_distance;   /* default 0.1 */
+  double kappa_max_factor; /* default 100 */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0
@@ -161,8 +174,11 @@ typedef struct {
   double final_h, final_residual, gravity_load;
   double seconds;          /* wall clock of the solve loop */
   double phase_ms[7];      /* assembly, pcg, line_search, ccd, reduce, io, other (timed-backend.ts:3-53) */
-  int64_t fallbacks;       /* two-level solves that fell back to block-Jacobi */
+  int64_t fallbacks;       /* two-level solves redone with block-Jacobi: the coarse operator was not positive
+                              definite, or a breakdown under rules.fallback = 1 */
   int64_t read_bytes;      /* device-to-host bytes read back by the loop (EquilibriumRun.readBytes) */
+  int64_t truncations;     /* solves of either preconditioner truncated at negative curvature (status 4) */
+  int64_t breakdowns;      /* solves that reported a breakdown (status 3) to the Newton loop */
 } drape_ptc_result;
 
 /* A sheet scene: the reference's Scene / SceneParams and the material and step constants its Rust
@@ -257,6 +273,22 @@ int drape_ptc_keep_states(drape_ctx* ctx, int32_t keep);
  * states; NULL to query the count) -- the contents of gpu-states.bin (run-e7.ts:98). */
 int drape_ptc_states(drape_ctx* ctx, float* states, int32_t cap, int32_t* count);
 
+/* One PTC attempt from `start` [3V] at step size h, as the reference repeats each oracle probe step
+ * (drape-e7.ts:183-196 runPtcProbes): x_hat and M/h^2 of `start`, the residual scale at `start`, the
+ * CCD-cut Newton start and one robust Newton solve with the PTC inner settings; no rejection loop. Records
+ * (step index `step`) go to `records` (capacity `cap`, count in *count), the positions after each accepted
+ * update to `iterates` [cap][3V] (NULL: not kept) and the last iterate to `final_positions` [3V]. */
+int drape_ptc_probe(drape_ctx* ctx, const float* start, double h, int32_t step, drape_record* records, int32_t cap,
+                    int32_t* count, float* iterates, float* final_positions);
+
+/* The barrier stiffness kappa the kernels use (the colliders' kappa, problem.colliders[2]): drape_set_kappa
+ * writes it (kappa > 0), drape_kappa reads it. drape_kappa_history gives the kappa of each step of the last
+ * drape_simulate ([count], up to cap; NULL queries the count): the values rules.adaptive_kappa produced.
+ * The reference's kappa is a constant of the problem (drape.rs:81); these serve the adaptive update. */
+int drape_set_kappa(drape_ctx* ctx, double kappa);
+int drape_kappa(drape_ctx* ctx, double* kappa);
+int drape_kappa_history(drape_ctx* ctx, double* kappa, int32_t cap, int32_t* count);
+
 /* Prolongation of a coarse state to the next grid as a PTC start (e7_prolong.rs:9-47, the oracle's
  * e7-prolong): `coarse` [3 coarse_columns^2] f32 -> `fine` [3 (2 coarse_columns - 1)^2] f32, computed in f64.
  * `colliders` [collider_doubles = 4 + 8K] = [K, dhat, kappa, flag, (kind, a, b, c, w, 0, 0, 0) x K] in f64
@@ -300,7 +332,8 @@ int drape_linear_create(int32_t rows, int32_t blocks, const uint32_t* row_ptr, c
 int drape_set_preconditioner(drape_ctx* ctx, int32_t kind, const float* weights);
 /* The coarse layout of the two-level preconditioner: coarse rows, parent [4R] (coarse row of each parent
  * slot, -1 if none) and the bilinear weights [4R]; counters [4] = factorisations, failed factorisations
- * (A_c not positive definite), solves that fell back to block-Jacobi, microseconds spent factorising.
+ * (A_c not positive definite), two-level solves redone with block-Jacobi, microseconds spent factorising
+ * (device time of the factorisation chains, side-stream chains included).
  * Any output may be NULL. */
 int drape_two_level_layout(drape_ctx* ctx, int32_t* coarse_rows, int32_t* parent, float* bilinear,
                            int64_t* counters);

@@ -281,6 +281,53 @@ void or_ptc_solve(or_drape* d, const double* start, double h_start, int32_t max_
   }
 }
 
+int32_t or_ptc_steps(or_drape* d, const double* start, double h_start, int32_t max_steps, double* rows, int32_t cap,
+                     double* final_positions, or_ptc_stats* stats) {
+  Drape& dr = d->drape;
+  size_t nv = dr.mesh.rest.size();
+  auto x = start ? vec3s(start, nv) : dr.rest_positions();
+  PtcRules rules;
+  const double load = gravity_load(dr);
+  PtcRun run = ptc_solve(dr, x, h_start, rules, max_steps, false, nullptr);
+  const int32_t n = (int32_t)run.steps.size();
+  for (int32_t i = 0; i < n && i < cap && rows; ++i) {
+    const PtcStep& s = run.steps[i];
+    double* o = rows + 5 * i;
+    o[0] = s.h;
+    o[1] = s.residual / load;
+    o[2] = s.inner_iterations;
+    o[3] = s.rejected_attempts;
+    o[4] = s.max_mu;
+  }
+  if (final_positions)
+    for (size_t i = 0; i < nv; ++i)
+      for (int k = 0; k < 3; ++k) final_positions[3 * i + k] = run.states.back()[i][k];
+  if (stats) {
+    stats->steps = n;
+    int newton = 0, rejected = 0;
+    double max_mu = 0.0;
+    for (const auto& s : run.steps) {
+      newton += s.inner_iterations;
+      rejected += s.rejected_attempts;
+      max_mu = std::fmax(max_mu, s.max_mu);
+    }
+    stats->newton_iterations = newton;
+    stats->rejected_attempts = rejected;
+    stats->reason = run.reason == "tolerance" ? 0 : run.reason == "floor" ? 1 : run.reason == "cap" ? 2 : 3;
+    stats->final_relative_residual = run.final_residual / load;
+    stats->final_h = run.final_h;
+    stats->static_energy = static_energy(dr, run.states.back());
+    stats->max_mu = max_mu;
+    stats->seconds = 0.0;
+  }
+  return n;
+}
+
+double or_static_floor(or_drape* d, const double* x) {
+  Drape& dr = d->drape;
+  return static_floor(dr, vec3s(x, dr.mesh.rest.size()));
+}
+
 int32_t or_simulate(or_drape* d, int32_t steps, double tolerance, int32_t max_iterations, int32_t regularized,
                     double floor_factor, int32_t* iterations, double* history, int64_t history_cap, int64_t* offsets,
                     double* final_positions) {

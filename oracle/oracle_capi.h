@@ -66,6 +66,14 @@ typedef struct {
 void or_ptc_solve(or_drape* d, const double* start, double h_start, int32_t max_steps, char* log, int64_t log_cap,
                   double* final_positions, or_ptc_stats* stats);
 
+/* PTC from `start` (NULL: rest) at most `max_steps` accepted steps (0: the oracle's own cap): per accepted step
+ * `rows` [cap][5] = (h, r/L at the step's start state, inner Newton iterations, rejected attempts, max mu); the
+ * last state into `final_positions`. Returns the number of accepted steps. */
+int32_t or_ptc_steps(or_drape* d, const double* start, double h_start, int32_t max_steps, double* rows, int32_t cap,
+                     double* final_positions, or_ptc_stats* stats);
+/* The representation floor of the static residual at x (e7_ptc.rs static_floor). */
+double or_static_floor(or_drape* d, const double* x);
+
 /* Dynamic trajectory from rest (drape.rs:426-439): per step newton iterations, final residual and the
  * residual history; `history` holds `history_cap` doubles, `offsets` steps+1 entries. */
 int32_t or_simulate(or_drape* d, int32_t steps, double tolerance, int32_t max_iterations, int32_t regularized,
