@@ -297,6 +297,18 @@ int drape_kappa_history(drape_ctx* ctx, double* kappa, int32_t cap, int32_t* cou
 int drape_prolong_state(int32_t coarse_columns, const float* coarse, const double* colliders,
                         int32_t collider_doubles, float* fine, int32_t* moved, int32_t* infeasible);
 
+/* drape_prolong_state without any allocation, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * `coarse` [3 coarse_columns^2] and `fine` [3 (2 coarse_columns - 1)^2] are DEVICE arrays of the current device
+ * (checked; anything else is DRAPE_E_INVALID); `colliders` is a host array as in drape_prolong_state, with at
+ * most 16 colliders. Work goes on `stream` (a hipStream_t; NULL: the default stream) and the call returns once
+ * moved / infeasible are on the host. The only device memory it uses beyond the caller's arrays is a 9.5 KB
+ * array of the library (collider staging and per-workgroup counts), one per GPU for the life of the process:
+ * calls that overlap on one GPU race. Same positions, bit for bit, and counts as drape_prolong_state. */
+int drape_prolong_state_noalloc(int32_t coarse_columns, const float* coarse, const double* colliders,
+                                int32_t collider_doubles, float* fine, int32_t* moved, int32_t* infeasible,
+                                void* stream);
+
 /* The context's assembly scatter maps, built on the device at drape_create: problem.bin's
  * block_contrib_ptr [B + 1] / block_contrib_idx [sizes[0]] and force_contrib_ptr [R + 1] /
  * force_contrib_idx [sizes[1]] (e3_generate.rs:111-161). Any output but `sizes` may be NULL. */
