@@ -4,17 +4,22 @@ iteration of the quasi-static sphere drape solved by pseudo-transient continuati
 sphere r = 0.15 m, dhat = 1 mm), the learned two-level preconditioner against block-Jacobi FP32 PCG.
 
 A bench step is one accepted PTC step (one inexact Newton solve of the step's energy, several Newton
-iterations). `--warmup W` steps run untimed from the flat start; the next `--steps K` are timed, once with
-the selected preconditioner (the reported value; by default the two-level preconditioner with the learned
-prolongation shipped with the package), then from the same state with the bilinear two-level preconditioner
-and with block-Jacobi (the comparisons in `config`). Prints one JSON line.
+iterations). `--warmup W` steps (learned two-level) run untimed from the flat start; from the state and step
+size they reach, every leg runs the next `--steps K` steps: the selected preconditioner (the reported value;
+by default the two-level preconditioner with the learned prolongation shipped with the package), the bilinear
+two-level preconditioner and block-Jacobi, all under one negative-curvature policy (`--policy`), and then the
+same three under the other policy. Each leg reports its stop reason, its final and least r/L (static residual
+over the gravity load) and its truncation, breakdown and block-Jacobi re-solve counts. With the defaults the
+window is PTC steps 6-25, which end near the FP32 representation floor of the n129 drape (r/L about 0.46).
+Prints one JSON line.
 
-The roofline object is for the SpMV of PCG, the dominant kernel of the block-Jacobi solve (92% of the
-reference's n129 run) and the largest of the two-level iteration: algorithmic bytes 40 B per block + 28 B
-per row + 4 (vals 36 B + column 4 B per block; row pointer 4 B, x gather 12 B and y 12 B per row), divided
-by the kernel's average duration from HIP events on the context's stream. The CPU baseline is the oracle
-(f64 Newton with band Cholesky, as the reference's Rust oracle does) timed for one Newton iteration of the
-same first PTC step on one core.
+`roofline` is the fused direction update + SpMV (`k_pcg_dir_spmv`), the most-launched SpMV of the measured
+run; `rooflines` adds the plain SpMV and the two triangular GEMVs of the coarse solve. Each is timed live with
+HIP events on the context's stream over back-to-back launches on the last assembled system, against its
+algorithmic bytes per launch (SpMV 40 B per block + 28 B per row + 4; the fused kernel 24 B per row more; a
+GEMV the triangle in FP32 plus 8 B per coarse unknown). `roofline_large_grid` is the plain SpMV at n1025,
+whose matrix exceeds the Infinity Cache. The CPU baseline is the oracle (f64 Newton with band Cholesky, as
+the reference's Rust oracle does) timed for one Newton iteration of the same first PTC step on one core.
 """
 from __future__ import annotations
 
@@ -33,10 +38,11 @@ sys.path.insert(0, ROOT)
 import drape_amd as D  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_pcg_spmv, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by the same
-# counters of a known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part); keyed by grid columns.
-# (At n129 the bytes are L2 misses served mostly by the Infinity Cache.)
-SPMV_TRAFFIC = {129: 8_574_432, 1025: 523_831_470}
+# HBM bytes per launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by the same counters of a
+# known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part), keyed by (kernel, grid columns); the
+# summaries are under profiles/. (At n129 the bytes are L2 misses served mostly by the Infinity Cache.)
+TRAFFIC = {("k_pcg_spmv", 1025): 523_831_470, ("k_pcg_spmv", 129): 8_638_625, ("k_pcg_dir_spmv", 129): 9_762_528,
+           ("k_tri_gemv_lower", 129): 21_726_692, ("k_tri_gemv_upper", 129): 22_528_454}
 
 
 def spmv_bytes(rows: int, blocks: int) -> int:
@@ -57,7 +63,7 @@ def large_grid_spmv(columns: int, launches: int = 50) -> dict:
     nbytes = spmv_bytes(arrays.R, arrays.B)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": SPMV_TRAFFIC.get(columns), "kernel": "k_pcg_spmv",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(("k_pcg_spmv", columns)), "kernel": "k_pcg_spmv",
            "avg_kernel_us": round(1000.0 * spmv_ms, 3), "algorithmic_bytes": nbytes,
            "workload": f"n{columns} ({arrays.R} rows, {arrays.B} blocks)"}
     ctx.close()
@@ -96,19 +102,42 @@ def cpu_baseline(columns: int) -> dict:
                       f"PTC step of the n{columns} drape, {seconds:.1f} s on 1 core ({model})"}
 
 
+LEG_NAMES = {"learned": "two-level, learned prolongation", "two_level": "two-level, bilinear prolongation",
+             "block_jacobi": "block-Jacobi"}
+# negative-curvature policies applied to every leg alike: "truncate" returns the last iterate before
+# p^T A p <= 0 when it reduced the residual (two-level fallback = 2, block_jacobi_truncate = 1) and otherwise
+# reports the breakdown; "regularize" always reports it (fallback = 0, block_jacobi_truncate = 0), the
+# reference's rule. Either way the Newton loop raises mu on a reported breakdown.
+POLICIES = {"truncate": {"fallback": 2, "block_jacobi_truncate": 1},
+            "regularize": {"fallback": 0, "block_jacobi_truncate": 0}}
+
+
+def kernel_roofline(ctx, kernel: str, launches: int, note: str) -> dict:
+    ms, nbytes = ctx.kernel_timing(kernel, launches)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get((kernel, ctx.columns)),
+            "kernel": kernel, "avg_kernel_us": round(1000.0 * ms, 3), "algorithmic_bytes": int(nbytes),
+            "launches": launches, "note": note}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=12)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--columns", type=int, default=129)
-    ap.add_argument("--preconditioner", choices=["learned", "two_level", "block_jacobi"], default="learned",
+    ap.add_argument("--preconditioner", choices=list(LEG_NAMES), default="learned",
                     help="learned: two-level with the learned prolongation; two_level: with bilinear prolongation")
-    ap.add_argument("--no-compare", action="store_true", help="skip the comparison runs")
+    ap.add_argument("--policy", choices=list(POLICIES), default="truncate",
+                    help="negative-curvature policy of every leg")
+    ap.add_argument("--no-compare", action="store_true", help="skip the comparison legs")
     ap.add_argument("--roofline-columns", type=int, default=1025,
                     help="grid for the HBM-bound SpMV roofline (0 skips it)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the large-grid SpMV and a copy of known size (for PMC byte counters)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the n129 kernel timings after the warm-up (for PMC byte counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--self-contact", action="store_true",
                     help="turn on the sheet's self-contact (point-triangle / edge-edge barriers) in every run")
@@ -125,72 +154,103 @@ def main() -> None:
 
     scene = D.drape_config_scene(args.columns)
     arrays = D.SceneArrays(scene)
-    kind = 0 if args.preconditioner == "block_jacobi" else 1
 
-    def context(pre):
+    def context(pre, policy):
         c = D.Drape(arrays, D.default_rules(preconditioner=0 if pre == "block_jacobi" else 1,
-                                            self_contact=1 if args.self_contact else 0))
+                                            self_contact=1 if args.self_contact else 0, **POLICIES[policy]))
         if pre == "learned":
             from drape_amd.learned import load_model
             c.set_prolongation_model(load_model())
+        c.columns = args.columns
         return c
 
-    ctx = context(args.preconditioner)
     print(f"[bench] n{args.columns}: V {arrays.V} T {arrays.T} H {arrays.H} rows {arrays.R} blocks {arrays.B}",
           file=sys.stderr, flush=True)
 
+    # the warm-up steps (learned two-level) give every leg the same start state and step size
     start = arrays.array("rest_positions")
-    h = ctx.rules.h0
+    warm = context("learned", args.policy)
+    h = warm.rules.h0
     if args.warmup > 0:
-        w = ctx.solve_equilibrium(start=start, h_start=h, cap=args.warmup)
+        w = warm.solve_equilibrium(start=start, h_start=h, cap=args.warmup)
         start, h = w["positions"], w["result"].final_h
         print(f"[bench] warmup: {w['result'].accepted_steps} steps, stop {w['stop']}", file=sys.stderr, flush=True)
+    warm.close()
 
-    def timed(context, label):
+    def leg(pre, policy):
+        c = context(pre, policy)
         t0 = time.perf_counter()
-        out = context.solve_equilibrium(start=start, h_start=h, cap=args.steps)
+        out = c.solve_equilibrium(start=start, h_start=h, cap=args.steps)
         wall = time.perf_counter() - t0
         res = out["result"]
         newton = max(res.newton_iterations, 1)
-        print(f"[bench] {label}: {res.accepted_steps} steps, {res.newton_iterations} Newton, {res.pcg_iterations} "
-              f"PCG, {res.seconds:.3f} s, fallbacks {res.fallbacks}, stop {out['stop']}, "
-              f"r/L {res.final_residual / res.gravity_load:.3e}", file=sys.stderr, flush=True)
-        return out, res, newton, wall
+        load = res.gravity_load
+        rl = [s.residual / load for s in out["steps"]] + [res.final_residual / load]
+        info = {"preconditioner": LEG_NAMES[pre], "policy": policy,
+                "ms_per_newton": round(1000.0 * res.seconds / newton, 4),
+                "pcg_per_newton": round(res.pcg_iterations / newton, 2),
+                "newton_iterations": res.newton_iterations, "pcg_iterations": res.pcg_iterations,
+                "accepted_steps": res.accepted_steps, "seconds": round(res.seconds, 4), "wall_s": round(wall, 3),
+                "stop": out["stop"], "final_relative_residual": round(rl[-1], 5),
+                "min_relative_residual": round(min(rl), 5),
+                "truncations": res.truncations, "breakdowns": res.breakdowns,
+                "block_jacobi_resolves": res.fallbacks,
+                "phase_ms": {k: round(v, 2) for k, v in zip(D.PHASES, res.phase_ms)}}
+        if pre != "block_jacobi":
+            layout = c.two_level_layout()
+            info["factorisations"] = int(layout[3][0])
+            info["factorisation_ms"] = round(layout[3][3] / 1000.0, 2)
+            info["factorisation_share"] = round(layout[3][3] / 1000.0 / (1000.0 * res.seconds), 4)
+        print(f"[bench] {policy} {pre}: {res.accepted_steps} steps, {res.newton_iterations} Newton, "
+              f"{res.pcg_iterations} PCG, {res.seconds:.3f} s, truncations {res.truncations}, breakdowns "
+              f"{res.breakdowns}, stop {out['stop']}, r/L final {rl[-1]:.3e} min {min(rl):.3e}",
+              file=sys.stderr, flush=True)
+        return c, info
 
-    out, res, newton, wall = timed(ctx, args.preconditioner)
-    ms_per_newton = 1000.0 * res.seconds / newton
-    pcg_per_newton = res.pcg_iterations / newton
-    comparisons = {}
+    if args.kernels_only:
+        ctx, _ = leg(args.preconditioner, args.policy)
+        for k in ("k_pcg_spmv", "k_pcg_dir_spmv", "k_tri_gemv_lower", "k_tri_gemv_upper"):
+            ms, nbytes = ctx.kernel_timing(k, 400)
+            print(json.dumps({"kernel": k, "avg_kernel_us": round(1000 * ms, 3), "algorithmic_bytes": nbytes}),
+                  flush=True)
+        print(json.dumps({"copy": copy_calibration()}), flush=True)
+        return
+
+    ctx, main_leg = leg(args.preconditioner, args.policy)
+    legs = {args.policy: {args.preconditioner: main_leg}}
     if not args.no_compare:
-        # the same steps from the same state with the other preconditioners
-        for pre in ("learned", "two_level", "block_jacobi"):
-            if pre == args.preconditioner:
-                continue
-            other = context(pre)
-            _, ores, onewton, _ = timed(other, pre)
-            comparisons[pre] = {"ms_per_newton": round(1000.0 * ores.seconds / onewton, 4),
-                                "pcg_per_newton": round(ores.pcg_iterations / onewton, 2),
-                                "newton_iterations": ores.newton_iterations, "pcg_iterations": ores.pcg_iterations,
-                                "fallbacks": ores.fallbacks, "accepted_steps": ores.accepted_steps}
-            other.close()
-    layout = ctx.two_level_layout() if kind != 0 else None
+        for policy in [args.policy] + [p for p in POLICIES if p != args.policy]:
+            for pre in LEG_NAMES:
+                if policy == args.policy and pre == args.preconditioner:
+                    continue
+                other, info = leg(pre, policy)
+                other.close()
+                legs.setdefault(policy, {})[pre] = info
+    same = legs[args.policy]
+    ratios = None
+    if "block_jacobi" in same and args.preconditioner != "block_jacobi":
+        bj = same["block_jacobi"]
+        ratios = {"time_per_newton": round(bj["ms_per_newton"] / main_leg["ms_per_newton"], 2),
+                  "pcg_per_newton": round(bj["pcg_per_newton"] / main_leg["pcg_per_newton"], 2)}
 
-    # dominant kernel: SpMV of the PCG on the last assembled system, timed with HIP events
-    spmv_ms, _ = ctx.spmv_timing(400)
-    nbytes = spmv_bytes(arrays.R, arrays.B)
-    achieved = nbytes / (spmv_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": SPMV_TRAFFIC.get(args.columns),
-                "kernel": "k_pcg_spmv", "avg_kernel_us": round(1000.0 * spmv_ms, 3),
-                "algorithmic_bytes": nbytes, "note": "n129 working set (~10 MB) is cache-resident"}
+    # rooflines of the PCG iteration's kernels on the last assembled system of the measured leg, timed with
+    # HIP events on the context's stream
+    cache = "n129 working set (matrix 7.7 MB, coarse inverse 2 x 21.3 MB) is resident in the 256 MB Infinity Cache"
+    kernels = [kernel_roofline(ctx, "k_pcg_dir_spmv", 400, "direction update fused into the SpMV, every "
+                               "iteration after a chunk's first; " + cache),
+               kernel_roofline(ctx, "k_pcg_spmv", 400, "plain SpMV, the first iteration of a chunk; " + cache)]
+    if args.preconditioner != "block_jacobi":
+        kernels += [kernel_roofline(ctx, "k_tri_gemv_lower", 400, "coarse solve t = B r_c; " + cache),
+                    kernel_roofline(ctx, "k_tri_gemv_upper", 400, "coarse solve y_c = B^T t; " + cache)]
+    ctx.close()
     line = {
         "metric": "ms/Newton step + PCG iters at n129: learned 2-level vs block-Jacobi PCG",
-        "value": round(ms_per_newton, 4),
+        "value": main_leg["ms_per_newton"],
         "unit": "ms/Newton iteration",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * res.seconds / max(res.accepted_steps, 1), 3),
+        "ms_per_step": round(1000.0 * main_leg["seconds"] / max(main_leg["accepted_steps"], 1), 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -198,19 +258,13 @@ def main() -> None:
         "data": "synthetic",
         "config": {"workload": f"quasi-static sphere drape by PTC, n{args.columns} "
                                f"({arrays.V} vertices, {arrays.T} triangles), isotropic E=1e6 nu=0.3 t=0.5mm, "
-                               "sphere r=0.15 m, dhat=1 mm",
-                   "preconditioner": args.preconditioner, "self_contact": args.self_contact,
-                   "pcg_per_newton": round(pcg_per_newton, 2),
-                   "newton_iterations": res.newton_iterations, "pcg_iterations": res.pcg_iterations,
-                   "accepted_steps": res.accepted_steps, "phase_ms": {k: round(v, 2) for k, v in zip(D.PHASES, res.phase_ms)},
-                   "fallbacks": res.fallbacks, "wall_s": round(wall, 3),
-                   "coarse_rows": layout[0] if layout else None,
-                   "factorisations": layout[3][0] if layout else None,
-                   "factorisation_ms": round(layout[3][3] / 1000.0, 2) if layout else None,
-                   "two_level_learned": comparisons.get("learned"),
-                   "two_level_bilinear": comparisons.get("two_level"),
-                   "block_jacobi": comparisons.get("block_jacobi")},
-        "roofline": roofline,
+                               "sphere r=0.15 m, dhat=1 mm; PTC steps warmup+1..warmup+steps from the flat start, "
+                               "every leg from the same state and step size",
+                   "preconditioner": args.preconditioner, "policy": args.policy, "self_contact": args.self_contact,
+                   "pcg_per_newton": main_leg["pcg_per_newton"],
+                   "block_jacobi_over_measured": ratios, "legs": legs},
+        "roofline": kernels[0],
+        "rooflines": kernels[1:],
     }
     if args.roofline_columns > 0:
         line["roofline_large_grid"] = large_grid_spmv(args.roofline_columns)

@@ -164,6 +164,7 @@ def lib():
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
         L.drape_ptc_probe.argtypes = [vp, f32p, C.c_double, C.c_int32, C.POINTER(Record), C.c_int32, i32p, f32p,
                                       f32p]
+        L.drape_kernel_timing.argtypes = [vp, C.c_int32, C.c_int32, f64p, f64p]
         L.drape_set_kappa.argtypes = [vp, C.c_double]
         L.drape_kappa.argtypes = [vp, f64p]
         L.drape_kappa_history.argtypes = [vp, f64p, C.c_int32, i32p]
@@ -444,6 +445,14 @@ class Drape:
         n = C.c_int32(launches)
         _check(lib().drape_pcg_timing(self.h, C.byref(spmv), C.byref(solve), C.byref(n)))
         return spmv.value, solve.value
+
+    KERNELS = {"k_pcg_spmv": 0, "k_pcg_dir_spmv": 1, "k_tri_gemv_lower": 2, "k_tri_gemv_upper": 3}
+
+    def kernel_timing(self, kernel, launches=200):
+        """(average ms per launch, algorithmic bytes per launch) of one PCG kernel (drape_kernel_timing)."""
+        ms, nbytes = C.c_double(), C.c_double()
+        _check(lib().drape_kernel_timing(self.h, self.KERNELS[kernel], launches, C.byref(ms), C.byref(nbytes)))
+        return ms.value, nbytes.value
 
     def set_preconditioner(self, kind, weights=None):
         """'block_jacobi' or 'two_level' (optionally with parent-slot weights [4R])."""

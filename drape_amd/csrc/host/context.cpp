@@ -769,4 +769,29 @@ int drape_pcg_timing(drape_ctx* p, double* spmv_ms, double* solve_ms, int32_t* s
   return DRAPE_OK;
 }
 
+int drape_kernel_timing(drape_ctx* p, int32_t kernel, int32_t launches, double* avg_ms, double* bytes) {
+  Context* c = ctx_of(p);
+  if (!c || !avg_ms || !bytes || launches <= 0 || kernel < 0 || kernel > 3) {
+    set_error("kernel timing: invalid argument");
+    return DRAPE_E_INVALID;
+  }
+  const double R = c->R, B = c->B;
+  switch (kernel) {
+    case 0:
+      TRY(spmv_timing(*c, launches, avg_ms));
+      *bytes = 40.0 * B + 28.0 * R + 4.0;
+      return DRAPE_OK;
+    case 1:
+      TRY(dir_spmv_timing(*c, launches, avg_ms));
+      *bytes = 40.0 * B + 52.0 * R + 4.0;
+      return DRAPE_OK;
+    default:
+      if (!c->two_level) {
+        set_error("kernel timing: the coarse GEMVs need the two-level preconditioner");
+        return DRAPE_E_INVALID;
+      }
+      return two_level_gemv_timing(*c, kernel == 2, launches, avg_ms, bytes);
+  }
+}
+
 }  // extern "C"

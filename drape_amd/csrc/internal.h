@@ -203,6 +203,7 @@ int launch_towards(Context& c);                     // dir = x_hat - pos (all ve
 int launch_step_all(Context& c, float alpha);       // pos += alpha dir (all vertices)
 int launch_ptc_prepare(Context& c, const float4* base, double h);  // x_hat, M/h^2 of a PTC attempt
 int spmv_timing(Context& c, int launches, double* avg_ms);
+int dir_spmv_timing(Context& c, int launches, double* avg_ms);  // k_pcg_dir_spmv alone
 int copy_timing(size_t bytes, int launches, double* avg_ms);  // known-size copy kernel
 int launch_set_float(Context& c, float* dst, float v);  // device scalar write by kernel argument
 int launch_dyn_predict(Context& c, const float4* vel, double h);  // x_hat = x + h v + h^2 g on the free rows
@@ -251,6 +252,7 @@ int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32 =
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host
+int two_level_gemv_timing(Context& c, bool lower, int launches, double* avg_ms, double* bytes);
 int two_level_set_model(Context& c, const float* theta, int count);  // learned prolongation (NULL: none)
 // Factorises the assembled system with the current weights (the network is not run), unconditionally.
 int two_level_factor_weights(Context& c, bool* ok);

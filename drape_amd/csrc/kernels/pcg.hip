@@ -781,6 +781,41 @@ int spmv_timing(Context& c, int launches, double* avg_ms) {
   return DRAPE_OK;
 }
 
+// The fused direction update + SpMV of a PCG iteration alone, `launches` times back to back between two events
+// on the context's stream (z, p -> p2, q), against a state of its own that never stops; average ms per launch.
+int dir_spmv_timing(Context& c, int launches, double* avg_ms) {
+  SellView A = view_of(c);
+  hipStream_t s = c.stream;
+  float* part_pq = c.partials + 2 * c.G;
+  float* part_rz = c.partials + 3 * c.G;
+  PcgState* st = nullptr;
+  HIP_TRY(hipMalloc((void**)&st, sizeof(PcgState)));
+  PcgState init{};
+  init.rz[0] = init.rz[1] = 1.0f;
+  init.rr0 = 1.0f;
+  init.tol2 = -1.0f;                      // rr <= tol2 rr0 never holds
+  init.max_iterations = 0x7fffffff;
+  init.diverge2 = INFINITY;
+  int rc = DRAPE_OK;
+  auto run = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(st, &init, sizeof init, hipMemcpyHostToDevice, s));
+    k_pcg_dir_spmv<<<c.G, 256, 0, s>>>(A, part_rz, c.G, st, c.z, c.p, c.p2, c.q, part_pq, c.R, 1);
+    HIP_TRY(hipEventRecord(c.ev_spmv0, s));
+    for (int i = 0; i < launches; ++i)
+      k_pcg_dir_spmv<<<c.G, 256, 0, s>>>(A, part_rz, c.G, st, c.z, c.p, c.p2, c.q, part_pq, c.R, 1);
+    HIP_TRY(hipEventRecord(c.ev_spmv1, s));
+    HIP_TRY(hipEventSynchronize(c.ev_spmv1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c.ev_spmv0, c.ev_spmv1));
+    *avg_ms = ms / launches;
+    return DRAPE_OK;
+  };
+  rc = run();
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(st);
+  return rc;
+}
+
 // A plain copy of n float4 (16 bytes each), grid-stride: the known-size kernel that scales hardware
 // byte counters.
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {

@@ -1486,6 +1486,38 @@ static void coarse_solve(Context& c, const TwoLevel* t, int guard, bool fp32) {
   }
 }
 
+// One of the coarse solve's triangular GEMVs (lower: t = B r_c, else y_c = B^T t) of the current factorisation,
+// `launches` times back to back between two events on the context's stream: average ms per launch and the
+// algorithmic bytes of one launch (the triangle in its storage precision, x read and y written once).
+int two_level_gemv_timing(Context& c, bool lower, int launches, double* avg_ms, double* bytes) {
+  TRY_RC(adopt_pending(c));
+  TwoLevel* t = c.two_level;
+  hipStream_t s = c.stream;
+  const int blocks = (t->N3 + 3) / 4;
+  const bool half = t->half_valid && use_half(c, t);
+  const double tri = (double)t->N3 * (t->N3 + 1) / 2.0;
+  *bytes = tri * (half ? 2.0 : 4.0) + 8.0 * t->N3;
+  auto launch = [&]() {
+    if (half) {
+      if (lower) k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, t->N3, t->N3p, t->d_t, c.state, 2);
+      else k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, t->N3, t->N3p, t->d_yc, c.state, 2);
+    } else {
+      if (lower) k_tri_gemv<true><<<blocks, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, 2);
+      else k_tri_gemv<false><<<blocks, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, 2);
+    }
+  };
+  launch();
+  HIP_TRY(hipEventRecord(c.ev_spmv0, s));
+  for (int i = 0; i < launches; ++i) launch();
+  HIP_TRY(hipEventRecord(c.ev_spmv1, s));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventSynchronize(c.ev_spmv1));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, c.ev_spmv0, c.ev_spmv1));
+  *avg_ms = ms / launches;
+  return DRAPE_OK;
+}
+
 // z = D^-1 r + P B^T B P^T r for the current r (guard: 0 running, 1 replacement gate, 2 setup with p = z).
 int two_level_apply(Context& c, float* partials, int guard) {
   TwoLevel* t = c.two_level;

@@ -397,6 +397,13 @@ int drape_solution_fp64(drape_ctx* ctx, double* solution);
  * and the whole solve, in milliseconds. */
 int drape_pcg_timing(drape_ctx* ctx, double* spmv_ms, double* solve_ms, int32_t* spmv_launches);
 
+/* Device timing of one kernel of the PCG iteration on the last assembled system (HIP events on the context's
+ * stream, `launches` back-to-back launches): kernel 0 the plain SpMV (k_pcg_spmv), 1 the direction update fused
+ * into the SpMV (k_pcg_dir_spmv), 2 / 3 the lower / upper triangular GEMV of the two-level coarse solve
+ * (k_tri_gemv, current factorisation). Average ms per launch and the kernel's algorithmic bytes per launch.
+ * No reference counterpart. */
+int drape_kernel_timing(drape_ctx* ctx, int32_t kernel, int32_t launches, double* avg_ms, double* bytes);
+
 /* A plain device-to-device copy of `bytes` (16-byte units), `launches` times between HIP events on a stream of
  * its own: the average kernel time in ms. No reference counterpart: the known-size kernel by which hardware
  * byte counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) are scaled. */
