@@ -154,53 +154,70 @@ __device__ inline float hinge_angle(float3 x1, float3 x2, float3 x3, float3 x4) 
   return atan2f(sine, dot3(u1, u2));
 }
 
-__global__ void __launch_bounds__(128) k_hinge(const int4* __restrict__ hin, const float* __restrict__ stiff,
-                                               const float4* __restrict__ pos, int H, int exact,
-                                               float* __restrict__ grad, float* __restrict__ hess) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= H) return;
-  const int4 v = hin[h];
-  const float3 x[4] = {xyz(pos[v.x]), xyz(pos[v.y]), xyz(pos[v.z]), xyz(pos[v.w])};
-  const float k = stiff[h];
-  const float theta = hinge_angle(x[0], x[1], x[2], x[3]);
-  V3<float> pf[4], gf[4];
+// One wavefront of 64 hinges per workgroup. Each lane forms its hinge's 12 gradient and 144 Hessian floats
+// in LDS (padded rows, so lanes hit distinct banks); the wavefront then writes its 64 consecutive records
+// (9,216 floats, contiguous in the element-local layout) with coalesced stores.
+constexpr int HINGE_WG = 64;
+constexpr int HINGE_HSTRIDE = 145;  // 144 floats + 1: lane * 145 covers every bank
+constexpr int HINGE_GSTRIDE = 13;   // 12 floats + 1
+
+__global__ void __launch_bounds__(HINGE_WG) k_hinge(const int4* __restrict__ hin, const float* __restrict__ stiff,
+                                                    const float4* __restrict__ pos, int H, int exact,
+                                                    float* __restrict__ grad, float* __restrict__ hess) {
+  __shared__ float stage_h[HINGE_WG * HINGE_HSTRIDE];
+  __shared__ float stage_g[HINGE_WG * HINGE_GSTRIDE];
+  const int h0 = blockIdx.x * HINGE_WG;
+  const int h = h0 + threadIdx.x;
+  if (h < H) {
+    const int4 v = hin[h];
+    const float3 x[4] = {xyz(pos[v.x]), xyz(pos[v.y]), xyz(pos[v.z]), xyz(pos[v.w])};
+    const float k = stiff[h];
+    const float theta = hinge_angle(x[0], x[1], x[2], x[3]);
+    V3<float> pf[4], gf[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) pf[i] = {x[i].x, x[i].y, x[i].z};
-  angle_gradient<float>(pf, gf);
-  const float factor = 2.0f * k;
-  const float gfac = factor * theta;
-  float* g = grad + 12 * (size_t)h;
+    for (int i = 0; i < 4; ++i) pf[i] = {x[i].x, x[i].y, x[i].z};
+    angle_gradient<float>(pf, gf);
+    const float factor = 2.0f * k;
+    const float gfac = factor * theta;
+    float* g = stage_g + threadIdx.x * HINGE_GSTRIDE;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    g[3 * i] = gf[i].x * gfac; g[3 * i + 1] = gf[i].y * gfac; g[3 * i + 2] = gf[i].z * gfac;
-  }
-  float* out = hess + 144 * (size_t)h;
-  // column (l, c) of the Hessian: Gauss-Newton part factor*g_k g_l^T, plus factor*theta*d(grad theta)/dx_lc
-  for (int l = 0; l < 4; ++l)
-    for (int c = 0; c < 3; ++c) {
-      const float glc = c == 0 ? gf[l].x : (c == 1 ? gf[l].y : gf[l].z);
-      float col[12];
-      if (exact) {
-        V3<D> pd[4], gd[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          pd[i] = {{pf[i].x, (i == l && c == 0) ? 1.0f : 0.0f}, {pf[i].y, (i == l && c == 1) ? 1.0f : 0.0f},
-                   {pf[i].z, (i == l && c == 2) ? 1.0f : 0.0f}};
-        angle_gradient<D>(pd, gd);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          col[3 * i] = gd[i].x.t; col[3 * i + 1] = gd[i].y.t; col[3 * i + 2] = gd[i].z.t;
-        }
-      }
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          const float gkr = r == 0 ? gf[kk].x : (r == 1 ? gf[kk].y : gf[kk].z);
-          const float value = exact ? factor * (gkr * glc + theta * col[3 * kk + r]) : gkr * glc * factor;
-          out[9 * (4 * kk + l) + 3 * r + c] = value;
-        }
+    for (int i = 0; i < 4; ++i) {
+      g[3 * i] = gf[i].x * gfac; g[3 * i + 1] = gf[i].y * gfac; g[3 * i + 2] = gf[i].z * gfac;
     }
+    float* out = stage_h + threadIdx.x * HINGE_HSTRIDE;
+    // column (l, c) of the Hessian: Gauss-Newton part factor*g_k g_l^T, plus factor*theta*d(grad theta)/dx_lc
+    for (int l = 0; l < 4; ++l)
+      for (int c = 0; c < 3; ++c) {
+        const float glc = c == 0 ? gf[l].x : (c == 1 ? gf[l].y : gf[l].z);
+        float col[12];
+        if (exact) {
+          V3<D> pd[4], gd[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            pd[i] = {{pf[i].x, (i == l && c == 0) ? 1.0f : 0.0f}, {pf[i].y, (i == l && c == 1) ? 1.0f : 0.0f},
+                     {pf[i].z, (i == l && c == 2) ? 1.0f : 0.0f}};
+          angle_gradient<D>(pd, gd);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            col[3 * i] = gd[i].x.t; col[3 * i + 1] = gd[i].y.t; col[3 * i + 2] = gd[i].z.t;
+          }
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const float gkr = r == 0 ? gf[kk].x : (r == 1 ? gf[kk].y : gf[kk].z);
+            const float value = exact ? factor * (gkr * glc + theta * col[3 * kk + r]) : gkr * glc * factor;
+            out[9 * (4 * kk + l) + 3 * r + c] = value;
+          }
+      }
+  }
+  __syncthreads();
+  const int count = H - h0 < HINGE_WG ? H - h0 : HINGE_WG;
+  float* hout = hess + 144 * (size_t)h0;
+  for (int i = threadIdx.x; i < 144 * count; i += HINGE_WG) hout[i] = stage_h[(i / 144) * HINGE_HSTRIDE + i % 144];
+  float* gout = grad + 12 * (size_t)h0;
+  for (int i = threadIdx.x; i < 12 * count; i += HINGE_WG) gout[i] = stage_g[(i / 12) * HINGE_GSTRIDE + i % 12];
 }
 
 // ---- rows: inertia and vertex-collider barrier ----------------------------------------------
@@ -269,6 +286,16 @@ __global__ void __launch_bounds__(256) k_rows(const float4* __restrict__ pos, co
   for (int i = 0; i < 9; ++i) ho[i] = h[i];
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: workgroup w runs on XCD w % 8.
+// This bijection of [0, groups) gives each XCD one contiguous run of logical workgroups, so rows that share
+// elements (a vertex and its neighbours one grid row away) gather their local blocks through the same L2.
+constexpr int XCDS = 8;
+__device__ inline int xcd_group(int groups) {
+  const int w = blockIdx.x, x = w % XCDS, k = w / XCDS;
+  const int q = groups / XCDS, r = groups % XCDS;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // ---- assembly (assemble_matrix.wgsl / assemble_force.wgsl, one thread per block or row) -----
 // vals of CSR block b = sum of its local blocks in list order (mass/contact, membranes, hinges),
 // written into the sliced layout: entry e of block b at sell_val[block_dst[b] + 64 e].
@@ -277,7 +304,7 @@ __global__ void __launch_bounds__(256) k_assemble_blocks(const uint32_t* __restr
                                                          const float* __restrict__ lhess,
                                                          const uint32_t* __restrict__ block_dst, int B,
                                                          float* __restrict__ sell_val) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = xcd_group(gridDim.x) * blockDim.x + threadIdx.x;
   if (b >= B) return;
   float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t end = bc_ptr[b + 1];
@@ -294,7 +321,7 @@ __global__ void __launch_bounds__(256) k_assemble_blocks(const uint32_t* __restr
 __global__ void __launch_bounds__(256) k_assemble_rhs(const uint32_t* __restrict__ fc_ptr,
                                                       const uint32_t* __restrict__ fc_idx,
                                                       const float* __restrict__ lgrad, int R, float4* __restrict__ b) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = xcd_group(gridDim.x) * blockDim.x + threadIdx.x;
   if (row >= R) return;
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
   const uint32_t end = fc_ptr[row + 1];
@@ -666,7 +693,7 @@ int launch_evaluate(Context& c, bool exact) {
   hipStream_t s = c.stream;
   if (c.T > 0) k_membrane<<<blocks_for(c.T), 256, 0, s>>>(c.tri, c.geom0, c.geom1, c.pos, c.material, c.T, exact, c.lgrad, c.lhess);
   if (c.H > 0)
-    k_hinge<<<blocks_for(c.H, 128), 128, 0, s>>>(c.hin, c.hstiff, c.pos, c.H, exact, c.lgrad + 9 * (size_t)c.T,
+    k_hinge<<<blocks_for(c.H, HINGE_WG), HINGE_WG, 0, s>>>(c.hin, c.hstiff, c.pos, c.H, exact, c.lgrad + 9 * (size_t)c.T,
                                                   c.lhess + 81 * (size_t)c.T);
   k_rows<<<blocks_for(c.R), 256, 0, s>>>(c.pos, c.xhat, c.rvert, c.rmass, c.mu, c.K > 0 ? c.colliders : nullptr, c.R,
                                          c.lgrad + 9 * (size_t)c.T + 12 * (size_t)c.H,

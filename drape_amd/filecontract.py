@@ -328,6 +328,55 @@ def write_run(gpu_dir, prefix, records):
     table(records, EXTRA_FIELDS).tofile(os.path.join(gpu_dir, f"gpu-{prefix}robust.bin"))
 
 
+def write_probes_input(case_dir, steps, h, positions):
+    """probes.json / probes.bin as the oracle writes them for a cold base solve (e7_run.rs:93-103): the 1-based
+    probe steps (u32), the f32 step size and the f32 start state of each."""
+    arrays = BinaryArrays()
+    arrays.push("probe_steps", "u32", np.asarray(steps, np.uint32))
+    arrays.push("probe_h", "f32", np.asarray(h, np.float32))
+    arrays.push("probe_positions", "f32", np.asarray(positions, np.float32))
+    arrays.write(os.path.join(case_dir, "probes.bin"))
+    with open(os.path.join(case_dir, "probes.json"), "w") as f:
+        json.dump({"probes": len(steps), "steps": [float(v) for v in steps], "arrays": arrays.entries}, f)
+
+
+def load_probes(case_dir, vertices):
+    """loadProbes (run-e7.ts:65-81): (steps, h, start states [n, 3V]) from probes.json / probes.bin, or None when
+    the case has none."""
+    try:
+        with open(os.path.join(case_dir, "probes.json")) as f:
+            manifest = json.load(f)
+        with open(os.path.join(case_dir, "probes.bin"), "rb") as f:
+            blob = f.read()
+    except OSError:
+        return None
+    entries = manifest.get("arrays", [])
+    steps = read_array(entries, blob, "probe_steps", "u32")
+    h = read_array(entries, blob, "probe_h", "f32")
+    positions = read_array(entries, blob, "probe_positions", "f32")
+    if h.size != steps.size or positions.size != 3 * vertices * steps.size:
+        raise DrapeError("probes.bin: probe arrays do not match the problem's vertices")
+    return steps, h, positions.reshape(steps.size, 3 * vertices)
+
+
+def run_probes(ctx, probes):
+    """runPtcProbes (drape-e7.ts:183-196): one PTC attempt from each probe state at its step size, keeping every
+    record and the positions at each record. Returns (records, iterates [n, 3V])."""
+    records, iterates = [], []
+    for step, h, start in zip(*probes):
+        out = ctx.ptc_probe(start, float(h), int(step), cap=256, iterates=True)
+        records += out["records"]
+        iterates.append(out["iterates"])
+    return records, np.concatenate(iterates) if iterates else np.zeros((0, 0), np.float32)
+
+
+def write_probes(gpu_dir, records, iterates):
+    """writeProbes (run-e7.ts:147-150): gpu-probe-iterations.bin, gpu-probe-robust.bin and
+    gpu-probe-iterates.bin (the positions at each record, f32)."""
+    write_run(gpu_dir, "probe-", records)
+    np.ascontiguousarray(iterates, "<f4").tofile(os.path.join(gpu_dir, "gpu-probe-iterates.bin"))
+
+
 def cap_for(columns):
     """capFor (ptc.ts:20): 1000 accepted steps up to n33, 400 above."""
     return 1000 if columns <= 33 else 400
@@ -434,11 +483,14 @@ def run_e7(case_dir, mode="cold", start=None, checks=False, preconditioner="bloc
     try:
         gpu_dir = os.path.join(case_dir, f"gpu-{mode}")
         os.makedirs(gpu_dir, exist_ok=True)
+        if preconditioner != "block_jacobi":
+            ctx.set_preconditioner(preconditioner)
         if checks:
             write_checks(gpu_dir, run_checks(ctx, problem))
             ctx.set_state(problem.array("rest_positions"), None, 0.0, 0.0)
-        if preconditioner != "block_jacobi":
-            ctx.set_preconditioner(preconditioner)
+            probes = load_probes(case_dir, problem.V)
+            if probes is not None:
+                write_probes(gpu_dir, *run_probes(ctx, probes))
         if start is None:
             x0, h0 = problem.array("rest_positions"), problem.ptc["h0"]
         else:

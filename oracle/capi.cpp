@@ -328,6 +328,52 @@ double or_static_floor(or_drape* d, const double* x) {
   return static_floor(dr, vec3s(x, dr.mesh.rest.size()));
 }
 
+int32_t or_simulate_states(or_drape* d, int32_t steps, const double* x0, const double* v0, double kappa_start,
+                           double tolerance, int32_t max_iterations, int32_t regularized, double floor_factor,
+                           int32_t adaptive, double kappa_distance, double kappa_max_factor, int32_t* iterations,
+                           double* kappa, double* min_distance, const int32_t* keep, int32_t keep_count,
+                           double* positions, double* velocities) {
+  Drape& dr = d->drape;
+  NewtonSettings s{tolerance, max_iterations, 1e-4, 30};
+  auto x = x0 ? vec3s(x0, dr.mesh.rest.size()) : dr.rest_positions();
+  std::vector<Vec3> v = v0 ? vec3s(v0, dr.mesh.rest.size()) : std::vector<Vec3>(x.size(), Vec3{0, 0, 0});
+  const double kappa_scene = dr.contact.barrier.kappa, dhat = dr.contact.barrier.dhat;
+  const double kappa0 = kappa_scene;
+  if (kappa_start > 0) dr.contact.barrier.kappa = kappa_start;
+  double previous = INFINITY;
+  const size_t nv = x.size();
+  auto store = [&](int state) {  // state k (0 = the start) when listed in keep
+    for (int32_t j = 0; j < keep_count; ++j)
+      if (keep[j] == state)
+        for (size_t i = 0; i < nv; ++i)
+          for (int k = 0; k < 3; ++k) {
+            if (positions) positions[(size_t)j * 3 * nv + 3 * i + k] = x[i][k];
+            if (velocities) velocities[(size_t)j * 3 * nv + 3 * i + k] = v[i][k];
+          }
+  };
+  store(0);
+  for (int n = 0; n < steps; ++n) {
+    if (kappa) kappa[n] = dr.contact.barrier.kappa;
+    TimeStep t = time_step(dr, x, v, s, regularized != 0, floor_factor);
+    if (iterations) iterations[n] = (int32_t)t.solution.searches.size();
+    x = t.solution.positions;
+    v = t.velocity;
+    double dmin = INFINITY;
+    for (int i : dr.free_) dmin = std::fmin(dmin, dr.contact.min_distance(x[i]));
+    if (min_distance) min_distance[n] = dmin;
+    // the IPC stiffness update (Li et al. 2020, section 4.3) once per step, as the library's rules.adaptive_kappa
+    if (adaptive && dr.has_contact()) {
+      const double cap = kappa_max_factor * kappa0;
+      if (dmin < kappa_distance * dhat && dmin < previous && dr.contact.barrier.kappa < cap)
+        dr.contact.barrier.kappa = std::fmin(2.0 * dr.contact.barrier.kappa, cap);
+      previous = dmin;
+    }
+    store(n + 1);
+  }
+  dr.contact.barrier.kappa = kappa_scene;
+  return 0;
+}
+
 int32_t or_simulate(or_drape* d, int32_t steps, double tolerance, int32_t max_iterations, int32_t regularized,
                     double floor_factor, int32_t* iterations, double* history, int64_t history_cap, int64_t* offsets,
                     double* final_positions) {

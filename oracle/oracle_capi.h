@@ -80,6 +80,18 @@ int32_t or_simulate(or_drape* d, int32_t steps, double tolerance, int32_t max_it
                     double floor_factor, int32_t* iterations, double* history, int64_t history_cap, int64_t* offsets,
                     double* final_positions);
 
+/* A trajectory from x0 / v0 (NULL: rest, zero velocity) with the barrier stiffness kappa_start (<= 0: the scene's),
+ * keeping the states listed in keep[keep_count] (state k after k steps, 0 the start): positions and velocities
+ * [keep_count][3V]; per step the Newton iterations, the stiffness used and the smallest vertex-collider distance
+ * after it. adaptive != 0 applies the IPC stiffness update after each step (kappa doubles when that distance
+ * is below kappa_distance * dhat and below the previous step's, up to kappa_max_factor times the scene's
+ * stiffness), as the library's rules.adaptive_kappa. The scene's stiffness is restored on return. */
+int32_t or_simulate_states(or_drape* d, int32_t steps, const double* x0, const double* v0, double kappa_start,
+                           double tolerance, int32_t max_iterations, int32_t regularized, double floor_factor,
+                           int32_t adaptive, double kappa_distance, double kappa_max_factor, int32_t* iterations,
+                           double* kappa, double* min_distance, const int32_t* keep, int32_t keep_count,
+                           double* positions, double* velocities);
+
 /* One Newton solve of one PTC attempt at step size h from x (e7_ptc.rs:129-134), for timing. */
 int32_t or_ptc_inner_solve(or_drape* d, const double* x, double h, double* out_positions, int32_t* iterations);
 

@@ -146,3 +146,39 @@ def replays(problem, gpu_dir):
     out["rules_replayed_ok"] = (forcing <= 1e-5 and out["termination_replay_ok"] and out["controller_replay_ok"]
                                 and out["rejection_replay_ok"])
     return out
+
+
+ONE_STEP_LIMIT = 1e-3        # e7_compare.rs:16
+FLOOR_FACTOR = 2.0           # e7_compare.rs:17
+STATUS_RESIDUAL = 1.0        # e7_compare.rs:24
+
+
+def probe_results(gpu_dir, vertices, previous, following):
+    """e7_compare.rs:170-190 probes: per probed step (records grouped by step), the GPU's last iterate against the
+    oracle's next state, relative to the oracle's step: |x_gpu - x_s| / |x_s - x_{s-1}|; the f32 floor (the least
+    relative residual of the step's records) and the stop status of its last record. `previous` / `following`
+    hold the oracle's states x_{s-1} and x_s of each probed step, in the order of the probe files."""
+    iterates = read_iterates(gpu_dir, "probe-")
+    states = np.fromfile(os.path.join(gpu_dir, "gpu-probe-iterates.bin"), "<f4").reshape(-1, 3 * vertices)
+    if len(states) != len(iterates):
+        raise ValueError("gpu-probe-iterates.bin does not hold one state per record")
+    groups = []
+    for i, it in enumerate(iterates):
+        if groups and groups[-1][0] == it["step"]:
+            groups[-1][1].append(i)
+        else:
+            groups.append((it["step"], [i]))
+    out = []
+    for k, (step, idx) in enumerate(groups):
+        x = states[idx[-1]].astype(np.float64)
+        prev, nxt = np.asarray(previous[k], np.float64), np.asarray(following[k], np.float64)
+        out.append({"step": step, "error": float(np.linalg.norm(x - nxt) / np.linalg.norm(nxt - prev)),
+                    "f32_floor": min(iterates[i]["relative"] for i in idx), "status": iterates[idx[-1]]["status"],
+                    "last": x})
+    return out
+
+
+def probe_stops_ok(status, f32_floor, representation_floor):
+    """e7_compare.rs:160-163 Probe::stops_ok: the step stopped on its residual, or by the update rule at a floor
+    within twice the representation floor."""
+    return status == STATUS_RESIDUAL or (status == STATUS_UPDATE and f32_floor <= FLOOR_FACTOR * representation_floor)

