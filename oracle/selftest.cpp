@@ -220,6 +220,159 @@ static void resting_kappa_balances_gravity() {
   double kappa = Barrier::resting_kappa(5e-4, 2e-5, 9.81);
   CHECK(std::fabs(Barrier{5e-4, kappa}.first(2.5e-4) + 2e-5 * 9.81) < 1e-18, "%g", kappa);
 }
+// e5_checks.rs:233-241: the contact state puts every free vertex inside the barrier (in [0.05 dhat, dhat)) and
+// some on each collider.
+static void contact_state_inside_the_barrier_on_both_colliders() {
+  Drape d(reference_scene(9, true, 1.0, 1.0, 1.0, false));
+  auto x = contact_state(d);
+  const double dhat = d.contact.barrier.dhat;
+  int sphere = 0, floor = 0;
+  for (int i : d.free_) {
+    const double m = d.contact.min_distance(x[i]);
+    CHECK(m >= 0.05 * dhat * 0.999 && m < dhat, "vertex %d at %g", i, m);
+    for (const auto& c : d.contact.colliders)
+      if (c.distance(x[i]) < dhat) (c.kind == 2 ? sphere : floor)++;
+  }
+  CHECK(sphere > 0 && floor > 0, "%d %d", sphere, floor);
+}
+
+// e5_checks.rs:109-200 contact_derivative_check, patch directions with the Phi step scaled to the patch's
+// smallest distance (patch_scaled_phi_step): Phi gradient and Hessian, barrier gradient and Hessian, barrier
+// energy-change identity.
+namespace e5fd {
+constexpr double FD_STEP = 1e-7, BARRIER_FD_FRACTION = 1e-3;
+constexpr int DIRECTIONS = 6, PATCH = 16;
+std::vector<Vec3> shifted(const std::vector<Vec3>& x, const std::vector<Vec3>& v, double h) {
+  std::vector<Vec3> out(x.size());
+  for (size_t i = 0; i < x.size(); ++i)
+    for (int k = 0; k < 3; ++k) out[i][k] = x[i][k] + h * v[i][k];
+  return out;
+}
+double local_value(const DrapeStep& s, const std::vector<Vec3>& x, const std::vector<int>& patch) {  // drape.rs:273-283
+  const Drape& d = *s.drape;
+  std::vector<bool> touched(d.mesh.rest.size(), false);
+  for (int i : patch) touched[i] = true;
+  double inertia = 0.0;
+  for (int i : patch) {
+    double o2 = 0.0;
+    for (int k = 0; k < 3; ++k) o2 += (x[i][k] - s.x_hat[i][k]) * (x[i][k] - s.x_hat[i][k]);
+    inertia += 0.5 * d.mass_over_h2(i) * o2 + d.contact.energy(x[i]);
+  }
+  double membrane = 0.0, hinge = 0.0;
+  for (const auto& e : d.membranes)
+    if (touched[e.vertices[0]] || touched[e.vertices[1]] || touched[e.vertices[2]]) membrane += e.energy(x, d.material);
+  for (const auto& h : d.hinges)
+    if (touched[h.vertices[0]] || touched[h.vertices[1]] || touched[h.vertices[2]] || touched[h.vertices[3]])
+      hinge += h.energy(x);
+  return inertia + membrane + hinge;
+}
+std::vector<double> barrier_rows(const Drape& d, const std::vector<Vec3>& x) {
+  std::vector<double> g;
+  for (int i : d.free_) {
+    Vec3 v = d.contact.gradient(x[i]);
+    g.insert(g.end(), v.begin(), v.end());
+  }
+  return g;
+}
+std::vector<double> barrier_hessian_times(const Drape& d, const std::vector<Vec3>& x, const std::vector<Vec3>& v) {
+  std::vector<double> out;
+  for (int i : d.free_) {
+    Mat3 h = d.contact.hessian_with(x[i], true);
+    for (int r = 0; r < 3; ++r) out.push_back(h[r][0] * v[i][0] + h[r][1] * v[i][1] + h[r][2] * v[i][2]);
+  }
+  return out;
+}
+double dot(const std::vector<double>& a, const std::vector<double>& b) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+double directional(const std::function<double(const std::vector<Vec3>&)>& f, const std::vector<double>& g,
+                   const std::vector<double>& rows, const std::vector<Vec3>& x, const std::vector<Vec3>& v, double h) {
+  const double fd = (f(shifted(x, v, h)) - f(shifted(x, v, -h))) / (2.0 * h);
+  const double exact = dot(g, rows);
+  return std::fabs(fd - exact) / std::fabs(exact);
+}
+double hessian(const std::function<std::vector<double>(const std::vector<Vec3>&)>& g, const std::vector<double>& exact,
+               const std::vector<Vec3>& x, const std::vector<Vec3>& v, double h) {
+  auto p = g(shifted(x, v, h)), m = g(shifted(x, v, -h));
+  std::vector<double> fd(p.size());
+  for (size_t i = 0; i < p.size(); ++i) fd[i] = (p[i] - m[i]) / (2.0 * h);
+  return difference_norm(fd, exact) / vector_norm(exact);
+}
+}  // namespace e5fd
+
+static void active_barrier_derivatives_match_fd() {
+  using namespace e5fd;
+  Drape d(reference_scene(9, true, 1.0, 1.0, 1.0, false));
+  Lcg rng(59);
+  const auto x = contact_state(d);
+  std::vector<Vec3> x_hat = x;
+  for (auto& p : x_hat)
+    for (double& v : p) v += 1e-4 * rng.uniform();
+  DrapeStep step{&d, x_hat};
+  const int span = (int)d.free_.size() - PATCH;
+  double worst[5] = {0, 0, 0, 0, 0}, worst_fixed[5] = {0, 0, 0, 0, 0};
+  for (int k = 0; k < DIRECTIONS; ++k) {
+    const int start = (int)(((rng.uniform() + 1.0) / 2.0) * span);
+    std::vector<int> patch(d.free_.begin() + start, d.free_.begin() + start + PATCH);
+    std::vector<Vec3> v(d.mesh.rest.size(), Vec3{0, 0, 0});
+    for (int i : patch) v[i] = {rng.uniform(), rng.uniform(), rng.uniform()};
+    const auto rows = step.gather(v);
+    double smallest = INFINITY;
+    for (int i : patch) smallest = std::fmin(smallest, d.contact.min_distance(x[i]));
+    const double barrier_step = BARRIER_FD_FRACTION * smallest;
+    auto local_barrier = [&](const std::vector<Vec3>& z) {
+      double e = 0.0;
+      for (int i : patch) e += d.contact.energy(z[i]);
+      return e;
+    };
+    const double alpha = 0.5 * std::fmin(step.max_step(x, v), d.contact.barrier.dhat);
+    const double direct = local_barrier(shifted(x, v, alpha)) - local_barrier(x);
+    const double identity = std::fabs(step.barrier_change(x, v, alpha) - direct) / std::fabs(direct);
+    for (int scaled = 0; scaled < 2; ++scaled) {
+      const double phi_step = scaled ? barrier_step : FD_STEP;
+      const double e[5] = {
+          directional([&](const std::vector<Vec3>& z) { return local_value(step, z, patch); }, step.residual(x), rows, x,
+                      v, phi_step),
+          hessian([&](const std::vector<Vec3>& z) { return step.residual(z); }, step.tangent(x).multiply(rows), x, v,
+                  phi_step),
+          directional(local_barrier, barrier_rows(d, x), rows, x, v, barrier_step),
+          hessian([&](const std::vector<Vec3>& z) { return barrier_rows(d, z); }, barrier_hessian_times(d, x, v), x, v,
+                  barrier_step),
+          identity};
+      double* w = scaled ? worst : worst_fixed;
+      for (int j = 0; j < 5; ++j) w[j] = std::fmax(w[j], e[j]);
+    }
+  }
+  CHECK(worst[0] <= 1e-6 && worst[2] <= 1e-6, "%g %g", worst[0], worst[2]);
+  CHECK(worst[1] <= 1e-5 && worst[3] <= 1e-5, "%g %g", worst[1], worst[3]);
+  CHECK(worst[4] <= 1e-12, "%g", worst[4]);
+  CHECK(worst_fixed[2] <= 1e-6 && worst_fixed[3] <= 1e-5, "%g %g", worst_fixed[2], worst_fixed[3]);
+}
+
+// newton.rs:302-314 (armijo_backtracks_an_overlong_newton_step), restated on the drape step system (the
+// reference's test uses its E2 coupon, which is not restated here): four times the Newton direction of the
+// first backward-Euler step is backtracked at least once, and the accepted step satisfies Armijo.
+static void armijo_backtracks_an_overlong_newton_step() {
+  Drape d(reference_scene(9, false, 1.0, 1.0, 1.0, false));
+  std::vector<Vec3> x = d.rest_positions(), v(x.size(), Vec3{0, 0, 0});
+  DrapeStep step{&d, d.prediction(x, v)};
+  const auto start = step.x_hat;
+  const auto g = step.residual(start);
+  BandCholesky f;
+  CHECK(f.factor(step.regularized_tangent(start, 0.0)), "factor");
+  std::vector<double> neg(g.size());
+  for (size_t i = 0; i < g.size(); ++i) neg[i] = -4.0 * g[i];
+  const auto direction = step.scatter(f.solve(neg));
+  LineSearch ls;
+  NewtonSettings settings{1e-12, 30, 1e-4, 30};
+  CHECK(line_search(step, start, g, direction, settings, &ls), "line search failed");
+  CHECK(ls.halvings >= 1, "%d", ls.halvings);
+  const double slope = e5fd::dot(g, step.gather(direction));
+  CHECK(step.energy_change(start, direction, ls.step_length) <= 1e-4 * ls.step_length * slope, "armijo");
+}
+
 static void contact_gradient_and_hessian_match_fd() {
   Contact c = test_contact();
   Vec3 x = {0.27, 0.24, 0.2502};
@@ -468,6 +621,9 @@ int main(int argc, char** argv) {
       {"barrier_derivatives_match_fd", barrier_derivatives_match_fd},
       {"resting_kappa_balances_gravity", resting_kappa_balances_gravity},
       {"contact_gradient_and_hessian_match_fd", contact_gradient_and_hessian_match_fd},
+      {"contact_state_inside_the_barrier_on_both_colliders", contact_state_inside_the_barrier_on_both_colliders},
+      {"active_barrier_derivatives_match_fd", active_barrier_derivatives_match_fd},
+      {"armijo_backtracks_an_overlong_newton_step", armijo_backtracks_an_overlong_newton_step},
       {"max_step_keeps_a_tenth", max_step_keeps_a_tenth},
       {"residual_is_gradient_of_incremental_potential", residual_is_gradient_of_incremental_potential},
       {"feasible_start_keeps_a_tenth", feasible_start_keeps_a_tenth},

@@ -13,9 +13,10 @@ is relative to the displacement, so 2^-24 of it is far below the 1e-3 gate), plu
 floor (the f64 step residual of the f32-rounded oracle step over the step's scale, e7_compare.rs:180-190).
 
 The sheet rests on a frictionless sphere with no floor, so its symmetric equilibrium is unstable: the f64 solve
-comes within r/L ~1e-5 of it and then slides off (n33: r/L 8.9e-6 at step 20, free fall from step ~70). The
-solve therefore stops at the first state with r/L <= --stop-below (1e-5), and the equilibrium numbers are
-those of the state of least r/L.
+comes close to it and then slides off (n33: r/L 8.9e-6 at step 20, free fall from step ~70; n129: r/L 4.2e-5
+at step 18, 0.44 after step 19). The solve therefore stops at the first state with r/L <= --stop-below
+(1e-5), or once it has left the equilibrium (r/L above 1000 times its least value after that fell below
+1e-3), and the equilibrium numbers are those of the state of least r/L.
 
 The oracle runs one accepted step per call (its step-size rule depends only on the last step), so the solve
 checkpoints to --work after every step and resumes from there.
@@ -103,6 +104,12 @@ def main():
         print(f"resuming at {len(rows)} steps", flush=True)
     else:
         states, rows, h, done, reason = [od.rest_positions()], [], h0, False, -1
+    def left(rows):
+        least = min(r[1] for r in rows)
+        return least <= 1e-3 and rows[-1][1] > 1e3 * least
+
+    if rows and not done and left(rows):
+        done, reason = True, 6
     t0 = time.time()
     while not done and len(rows) < args.max_steps:
         k, nxt, row, st = one_step(L, od, states[-1], h)
@@ -116,6 +123,8 @@ def main():
                   f"mu={row[4]:g} next r/L={st.final_relative_residual:.3e} ({time.time() - t0:.0f} s)", flush=True)
             if st.final_relative_residual <= args.stop_below:
                 done, reason = True, 5
+            elif left(rows + [[0.0, st.final_relative_residual]]):
+                done, reason = True, 6
         np.savez(ckpt, states=np.array(states), rows=np.array(rows).reshape(-1, 5), h_next=h, done=done,
                  reason=reason)
     if not done:
@@ -128,7 +137,7 @@ def main():
     picks = [1, first(lambda s: contacts[s] > 0), first(lambda s: rl[s] <= 1e-1), first(lambda s: rl[s] <= 1e-2)]
     steps = sorted({s for s in picks if s is not None})
     print(f"oracle: {len(rows)} steps, {int(rows[:, 2].sum())} Newton, stop reason {reason} (5: r/L below "
-          f"--stop-below), final r/L {rl[-1]:.3e}; probe steps {steps}", flush=True)
+          f"--stop-below, 6: left the equilibrium), final r/L {rl[-1]:.3e}; probe steps {steps}", flush=True)
     best = int(np.argmin(rl))
     eq = states[best]
     eq32 = eq.astype(np.float32).astype(np.float64)
