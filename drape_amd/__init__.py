@@ -54,7 +54,8 @@ class Rules(C.Structure):
                 ("fallback", C.c_int32), ("coarse_shift", C.c_double), ("smoother", C.c_double),
                 ("coarse_half", C.c_int32), ("refresh_rate", C.c_double), ("refresh_on_breakdown", C.c_int32),
                 ("self_contact", C.c_int32), ("coarse_async", C.c_int32), ("block_jacobi_truncate", C.c_int32),
-                ("adaptive_kappa", C.c_int32), ("kappa_distance", C.c_double), ("kappa_max_factor", C.c_double)]
+                ("adaptive_kappa", C.c_int32), ("kappa_distance", C.c_double), ("kappa_max_factor", C.c_double),
+                ("coarse_factor", C.c_int32)]
 
 
 class PcgStats(C.Structure):
@@ -167,7 +168,7 @@ def lib():
         L.drape_kernel_timing.argtypes = [vp, C.c_int32, C.c_int32, f64p, f64p]
         L.drape_set_kappa.argtypes = [vp, C.c_double]
         L.drape_kappa.argtypes = [vp, f64p]
-        L.drape_kappa_history.argtypes = [vp, f64p, C.c_int32, i32p]
+        L.drape_kappa_history.argtypes = [vp, f64p, f32p, C.c_int32, i32p]
         L.drape_scene_build.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_build_device.argtypes = [C.POINTER(Scene), C.POINTER(vp)]
         L.drape_scene_contributions.argtypes = [vp, C.POINTER(C.c_int64), u32p, u32p, u32p, u32p, u32p]
@@ -588,11 +589,13 @@ class Drape:
         _check(lib().drape_set_kappa(self.h, kappa))
 
     def kappa_history(self):
+        """(kappa of each step of the last simulate, smallest collider distance after it)."""
         n = C.c_int32()
-        _check(lib().drape_kappa_history(self.h, None, 0, C.byref(n)))
+        _check(lib().drape_kappa_history(self.h, None, None, 0, C.byref(n)))
         out = np.zeros(n.value)
-        _check(lib().drape_kappa_history(self.h, _ptr(out, f64p), n.value, C.byref(n)))
-        return out
+        dist = np.zeros(n.value, np.float32)
+        _check(lib().drape_kappa_history(self.h, _ptr(out, f64p), _ptr(dist, f32p), n.value, C.byref(n)))
+        return out, dist
 
     def solve_equilibrium(self, start=None, h_start=None, cap=400, budget_s=0.0, keep=20000, keep_states=False):
         """solveEquilibrium (drape-e7.ts:156-181). "states" holds the start and every accepted state when

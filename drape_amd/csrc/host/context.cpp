@@ -340,6 +340,7 @@ void drape_default_rules(drape_rules* r) {
   r->adaptive_kappa = 0;
   r->kappa_distance = 0.1;
   r->kappa_max_factor = 100.0;
+  r->coarse_factor = 0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }
@@ -505,13 +506,15 @@ int drape_kappa(drape_ctx* p, double* kappa) {
   return DRAPE_OK;
 }
 
-int drape_kappa_history(drape_ctx* p, double* kappa, int32_t cap, int32_t* count) {
+int drape_kappa_history(drape_ctx* p, double* kappa, float* min_distance, int32_t cap, int32_t* count) {
   Context* c = ctx_of(p);
   if (!c || c->linear || cap < 0) { set_error("invalid argument"); return DRAPE_E_INVALID; }
   const int32_t n = (int32_t)c->kappa_history.size();
   if (count) *count = n;
-  if (kappa)
-    for (int32_t i = 0; i < n && i < cap; ++i) kappa[i] = c->kappa_history[i];
+  for (int32_t i = 0; i < n && i < cap; ++i) {
+    if (kappa) kappa[i] = c->kappa_history[i];
+    if (min_distance) min_distance[i] = i < (int32_t)c->step_distance.size() ? c->step_distance[i] : INFINITY;
+  }
   return DRAPE_OK;
 }
 

@@ -500,9 +500,7 @@ int ptc_solve(Context& c, double h_start, int cap, double budget_s, drape_ptc_re
 // The barrier-stiffness update of Incremental Potential Contact (Li et al. 2020, section 4.3), applied after
 // each time step: when the smallest vertex-collider distance is below kappa_distance * dhat and smaller than
 // after the previous step, kappa doubles (exactly, in FP32), up to kappa_max_factor times its start value.
-int adapt_kappa(Context& c, float* previous_min) {
-  float d = INFINITY;
-  TRY_RC(launch_min_distance(c, &d));
+int adapt_kappa(Context& c, float d, float* previous_min) {
   const float dhat = c.colliders_host[1];
   const float kappa = c.colliders_host[2];
   const float cap = (float)(c.rules.kappa_max_factor * c.kappa0);
@@ -520,6 +518,7 @@ int simulate(Context& c, int steps, std::vector<int32_t>& updates, std::vector<f
   Run run(c);
   const double h = c.problem.time_step;
   c.kappa_history.clear();
+  c.step_distance.clear();
   float previous_min = INFINITY;
   for (int n = 0; n < steps; ++n) {
     c.kappa_history.push_back(c.K > 0 ? (double)c.colliders_host[2] : 0.0);
@@ -545,7 +544,12 @@ int simulate(Context& c, int steps, std::vector<int32_t>& updates, std::vector<f
     relative.push_back(run.records.empty() ? NAN : (float)run.records.back().relative_residual);
     rc = launch_dyn_velocity(c, c.vel, h);
     if (rc != DRAPE_OK) return rc;
-    if (c.rules.adaptive_kappa && c.K > 0) TRY_RC(adapt_kappa(c, &previous_min));
+    if (c.K > 0) {
+      float d = INFINITY;
+      TRY_RC(launch_min_distance(c, &d));
+      c.step_distance.push_back(d);
+      if (c.rules.adaptive_kappa) TRY_RC(adapt_kappa(c, d, &previous_min));
+    }
   }
   records_out = std::move(run.records);
   return DRAPE_OK;

@@ -128,6 +128,7 @@ struct Context {
   // drape_simulate
   double kappa0 = 0.0;
   std::vector<double> kappa_history;
+  std::vector<float> step_distance;  // smallest vertex-collider distance after each step of the last drape_simulate
 
   // Newton iterates kept by a probe (positions after each accepted update, [count][3V]) when keep_iterates
   bool keep_iterates = false;

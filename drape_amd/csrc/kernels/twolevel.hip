@@ -65,6 +65,9 @@ struct TwoLevel {
   double* d_band = nullptr;         // [nt][kb + 1][64 x 64]: tile (k + d, k) of A_c, then of L, row-major
   double* d_linv = nullptr;         // [nt][64 x 64]: inverse of each diagonal tile of L
   double* d_lpanel = nullptr;       // [nt][kb][64 x 64]: tile (k + d, k) of L, d = 1..kb
+  double* d_xring = nullptr;        // [kb + 1][64][N3p]: the last tile rows of L^-1 in FP64 (inverse sweep)
+  hipStream_t inv_stream = nullptr; // the inverse sweep, behind each panel of the factorisation chain
+  hipEvent_t ev_panel = nullptr, ev_inverse = nullptr;
   double* d_Ac = nullptr;           // [N3 * N3] dense A_c for the check entry point (allocated on demand)
   float* d_B = nullptr;             // [N3 * N3p] row-major B = L^-1 (zero above the diagonal)
   float* d_BT = nullptr;            // [N3 * N3p] row-major B^T (zero below the diagonal)
@@ -113,6 +116,11 @@ void two_level_destroy(TwoLevel* t) {
   if (!t) return;
   TwoLevel::Pending& q = t->pending;
   if (q.side) (void)hipStreamSynchronize(q.side);
+  if (t->inv_stream) (void)hipStreamSynchronize(t->inv_stream);
+  for (hipEvent_t e : {t->ev_panel, t->ev_inverse})
+    if (e) (void)hipEventDestroy(e);
+  if (t->inv_stream) (void)hipStreamDestroy(t->inv_stream);
+  if (t->d_xring) (void)hipFree(t->d_xring);
   for (void* p : {(void*)q.w, (void*)q.linv, (void*)q.B, (void*)q.BT, (void*)q.Bh, (void*)q.BTh})
     if (p) (void)hipFree(p);
   if (q.h_info) (void)hipHostFree(q.h_info);
@@ -623,6 +631,71 @@ __global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__
   }
 }
 
+// Tile row I of B = L^-1 as a sweep over tile rows, so that row I can run as soon as the panel of tile column I
+// has produced L_II^-1 (the factor tiles L(I, I - d) came from earlier TRSMs): X(I, J) = L_II^-1 (E(I, J) -
+// sum_{d=1..min(kb, I-J)} L(I, I - d) X(I - d, J)) for the 64 (I + 1) columns of B's lower triangle, 16 per
+// workgroup. X keeps the last kb + 1 tile rows in FP64 ([kb + 1][64][N3p], slot I % (kb + 1)); B (row-major),
+// B^T and, when Bh is not null, the FP16 copies B~ = S^-1 B (S = diag(B), from L_II^-1) and its transpose get
+// the rows and columns below N3. Thread (row r, four columns): FP64 sums in a fixed order (d, then m).
+__global__ void __launch_bounds__(256) k_inverse_row(const double* __restrict__ lpanel, const double* __restrict__ linv,
+                                                     int I, int kb, int N3, int N3p, double* __restrict__ X,
+                                                     float* __restrict__ B, float* __restrict__ BT,
+                                                     _Float16* __restrict__ Bh, _Float16* __restrict__ BTh,
+                                                     const int* info) {
+  __shared__ double Ls[TILE * LSTR];
+  __shared__ double Xs[TILE * 17];
+  if (*info != 0) return;
+  const int tid = threadIdx.x, r = tid >> 2, cq = (tid & 3) * 4;
+  const int j0 = INV_COLS * blockIdx.x, J = j0 / TILE;
+  const int dmax = I - J < kb ? I - J : kb;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int d = 1; d <= dmax; ++d) {
+    __syncthreads();
+    tile_to_lds<TILE>(lpanel + (((size_t)(I - d) * kb + (d - 1)) << 12), Ls);
+    const double* xs = X + (size_t)((I - d) % (kb + 1)) * TILE * N3p + j0;
+    for (int e = tid; e < TILE * INV_COLS; e += 256) Xs[(e / INV_COLS) * 17 + e % INV_COLS] = xs[(size_t)(e / INV_COLS) * N3p + e % INV_COLS];
+    __syncthreads();
+#pragma unroll 8
+    for (int m = 0; m < TILE; ++m) {
+      const double l = Ls[r * LSTR + m];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] -= l * Xs[m * 17 + cq + u];
+    }
+  }
+  __syncthreads();
+  tile_to_lds<TILE>(linv + ((size_t)I << 12), Ls);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int col = j0 + cq + u;
+    Xs[r * 17 + cq + u] = (col == TILE * I + r ? 1.0 : 0.0) + acc[u];
+  }
+  __syncthreads();
+  double out[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+  for (int m = 0; m < TILE; ++m) {
+    const double l = Ls[r * LSTR + m];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) out[u] += l * Xs[m * 17 + cq + u];
+  }
+  const int R = TILE * I + r;
+  double* xo = X + (size_t)(I % (kb + 1)) * TILE * N3p + (size_t)r * N3p + j0 + cq;
+  const double sr = Ls[r * LSTR + r];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    xo[u] = out[u];
+    const int C = j0 + cq + u;
+    if (R < N3 && C < N3 && C <= R) {
+      B[(size_t)R * N3p + C] = (float)out[u];
+      BT[(size_t)C * N3p + R] = (float)out[u];
+      if (Bh) {
+        const _Float16 h = (_Float16)fmin(fmax(out[u] / sr, -65504.0), 65504.0);
+        Bh[(size_t)R * N3p + C] = h;
+        BTh[(size_t)C * N3p + R] = h;
+      }
+    }
+  }
+}
+
 // dense A_c (both triangles, row-major) for the check entry point
 __global__ void __launch_bounds__(256) k_galerkin_dense(const uint32_t* __restrict__ pair_ptr,
                                                         const uint32_t* __restrict__ pair_key,
@@ -915,15 +988,26 @@ __global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restric
 }
 
 // ---- host ------------------------------------------------------------------------------------
-// Coarse layout of a grid sheet with `columns` vertices per side ((columns - 1) divisible by 4): parents,
+// Coarse layout of a grid sheet with `columns` vertices per side ((columns - 1) divisible by the factor): parents,
 // bilinear weights, children and the Galerkin term lists.
+// The coarsening factor of rules.coarse_factor; 0 picks the smallest power of two from 4 up that keeps the coarse
+// grid at most 33 vertices a side (n129: 4, n257: 8), so that the dense inverse factor stays near 43 MB.
+int coarse_factor_of(const drape_rules& r, int columns) {
+  if (r.coarse_factor > 0) return r.coarse_factor;
+  int f = 4;
+  while ((columns - 1) / f + 1 > 33 && (columns - 1) % (2 * f) == 0) f *= 2;
+  return f;
+}
+
 int two_level_create(Context& c, int columns) {
-  if (columns < 9 || (columns - 1) % 4 != 0 || (size_t)columns * columns != (size_t)c.V) {
-    set_error("two-level preconditioner: the sheet must be a grid with (columns - 1) divisible by 4");
+  const int f = coarse_factor_of(c.rules, columns);
+  if (columns < 2 * f + 1 || f < 2 || (columns - 1) % f != 0 || (size_t)columns * columns != (size_t)c.V) {
+    set_error("two-level preconditioner: the sheet must be a grid with (columns - 1) divisible by the coarsening "
+              "factor (" + std::to_string(f) + ") and at least three coarse vertices a side");
     return DRAPE_E_INVALID;
   }
   TwoLevel* t = new TwoLevel();
-  const int n = columns, nc = (n - 1) / 4 + 1;
+  const int n = columns, nc = (n - 1) / f + 1;
   t->nc = nc;
   std::vector<int> row_of(c.V, -1);
   for (int row = 0; row < c.R; ++row) row_of[c.row_vertex[row]] = row;
@@ -932,7 +1016,7 @@ int two_level_create(Context& c, int columns) {
   int Nc = 0;
   for (int b = 0; b < nc; ++b)
     for (int a = 0; a < nc; ++a)
-      if (row_of[(4 * b) * n + 4 * a] >= 0) coarse_of[b * nc + a] = Nc++;
+      if (row_of[(f * b) * n + f * a] >= 0) coarse_of[b * nc + a] = Nc++;
   t->Nc = Nc;
   t->N3 = 3 * Nc;
   t->N3p = (t->N3 + 63) / 64 * 64;
@@ -941,8 +1025,8 @@ int two_level_create(Context& c, int columns) {
   for (int row = 0; row < c.R; ++row) {
     const int v = (int)c.row_vertex[row];
     const int x = v % n, y = v / n;
-    const int a0 = std::min(x / 4, nc - 2), b0 = std::min(y / 4, nc - 2);
-    const double fx = (x - 4.0 * a0) / 4.0, fy = (y - 4.0 * b0) / 4.0;
+    const int a0 = std::min(x / f, nc - 2), b0 = std::min(y / f, nc - 2);
+    const double fx = (x - (double)f * a0) / f, fy = (y - (double)f * b0) / f;
     const int ca[4] = {a0, a0 + 1, a0, a0 + 1}, cb[4] = {b0, b0, b0 + 1, b0 + 1};
     const double wt[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
     for (int k = 0; k < 4; ++k) {
@@ -956,8 +1040,8 @@ int two_level_create(Context& c, int columns) {
   for (int row = 0; row < c.R; ++row) {
     const int v = (int)c.row_vertex[row];
     const int x = v % n, y = v / n;
-    const int a0 = std::min(x / 4, nc - 2), b0 = std::min(y / 4, nc - 2);
-    fxy[row] = make_float2((float)((x - 4.0 * a0) / 4.0), (float)((y - 4.0 * b0) / 4.0));
+    const int a0 = std::min(x / f, nc - 2), b0 = std::min(y / f, nc - 2);
+    fxy[row] = make_float2((float)((x - (double)f * a0) / f), (float)((y - (double)f * b0) / f));
   }
   // children of each coarse node, in fine-row order
   std::vector<std::vector<uint32_t>> kids(Nc);
@@ -1024,6 +1108,10 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_band, (size_t)t->nt * (t->kb + 1) * TILE * TILE);
   TL_ALLOC(t->d_linv, (size_t)t->nt * TILE * TILE);
   TL_ALLOC(t->d_lpanel, (size_t)t->nt * std::max(t->kb, 1) * TILE * TILE);
+  TL_ALLOC(t->d_xring, (size_t)(t->kb + 1) * TILE * t->N3p);
+  if ((e = hipStreamCreateWithFlags(&t->inv_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
+  if ((e = hipEventCreateWithFlags(&t->ev_panel, hipEventDisableTiming)) != hipSuccess) return fail(e);
+  if ((e = hipEventCreateWithFlags(&t->ev_inverse, hipEventDisableTiming)) != hipSuccess) return fail(e);
   TL_ALLOC(t->d_B, (size_t)t->N3 * t->N3p);
   TL_ALLOC(t->d_BT, (size_t)t->N3 * t->N3p);
   TL_ALLOC(t->d_Bh, (size_t)t->N3 * t->N3p);
@@ -1418,15 +1506,22 @@ static int factorise_chain(Context& c, hipStream_t s, const float* w, double* li
   if (after_product) HIP_TRY(hipEventRecord(after_product, s));
   k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(t->d_band, t->kb, t->nt, t->N3, 1.0 + c.rules.coarse_shift);
   const int updates = t->kb * (t->kb + 1) / 2;
+  // The panel chain is latency-bound on one workgroup at a time; tile row k of B = L^-1 runs on the inverse
+  // stream as soon as panel k is done, beside the chain's later tile columns, and s waits for the last row.
   for (int k = 0; k < t->nt; ++k) {
     k_chol_panel<<<1, 256, 0, s>>>(t->d_band, linv, k, t->kb, t->d_info);
+    HIP_TRY(hipEventRecord(t->ev_panel, s));
+    HIP_TRY(hipStreamWaitEvent(t->inv_stream, t->ev_panel, 0));
+    k_inverse_row<<<(k + 1) * (TILE / INV_COLS), 256, 0, t->inv_stream>>>(
+        t->d_lpanel, linv, k, t->kb, t->N3, t->N3p, t->d_xring, B, BT, half ? Bh : nullptr, half ? BTh : nullptr,
+        t->d_info);
     if (k + 1 < t->nt && updates > 0) {
       k_chol_trsm<<<4 * t->kb, 256, 0, s>>>(t->d_band, t->d_lpanel, linv, k, t->nt, t->kb, t->d_info);
       k_chol_update<<<4 * updates, 256, 0, s>>>(t->d_band, t->d_lpanel, k, t->nt, t->kb, t->d_info);
     }
   }
-  k_band_inverse<<<t->nt * (TILE / INV_COLS), 256, 0, s>>>(t->d_lpanel, linv, t->nt, t->kb, t->N3, t->N3p, B, BT,
-                                                           half ? Bh : nullptr, half ? BTh : nullptr, t->d_info);
+  HIP_TRY(hipEventRecord(t->ev_inverse, t->inv_stream));
+  HIP_TRY(hipStreamWaitEvent(s, t->ev_inverse, 0));
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }

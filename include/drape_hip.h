@@ -128,6 +128,9 @@ typedef struct {
                              0 (default) keeps the start value (the reference's fixed kappa) */
   double kappa_distance;   /* default 0.1 */
   double kappa_max_factor; /* default 100 */
+  int32_t coarse_factor;   /* two-level: the coarse vertices are every coarse_factor-th grid vertex each way
+                              ((grid_columns - 1) divisible by it); 0 (default) picks the smallest power of two
+                              from 4 up that keeps the coarse grid at most 33 vertices a side (n129: 4, n257: 8) */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0
@@ -282,12 +285,13 @@ int drape_ptc_probe(drape_ctx* ctx, const float* start, double h, int32_t step, 
                     int32_t* count, float* iterates, float* final_positions);
 
 /* The barrier stiffness kappa the kernels use (the colliders' kappa, problem.colliders[2]): drape_set_kappa
- * writes it (kappa > 0), drape_kappa reads it. drape_kappa_history gives the kappa of each step of the last
- * drape_simulate ([count], up to cap; NULL queries the count): the values rules.adaptive_kappa produced.
- * The reference's kappa is a constant of the problem (drape.rs:81); these serve the adaptive update. */
+ * writes it (kappa > 0), drape_kappa reads it. drape_kappa_history gives, for each step of the last
+ * drape_simulate ([count], up to cap; NULL arrays query the count), the kappa the step used (the values
+ * rules.adaptive_kappa produced) and the smallest vertex-collider distance after it (FP32, +inf without
+ * colliders). The reference's kappa is a constant of the problem (drape.rs:81); these serve the update. */
 int drape_set_kappa(drape_ctx* ctx, double kappa);
 int drape_kappa(drape_ctx* ctx, double* kappa);
-int drape_kappa_history(drape_ctx* ctx, double* kappa, int32_t cap, int32_t* count);
+int drape_kappa_history(drape_ctx* ctx, double* kappa, float* min_distance, int32_t cap, int32_t* count);
 
 /* Prolongation of a coarse state to the next grid as a PTC start (e7_prolong.rs:9-47, the oracle's
  * e7-prolong): `coarse` [3 coarse_columns^2] f32 -> `fine` [3 (2 coarse_columns - 1)^2] f32, computed in f64.
@@ -337,8 +341,8 @@ int drape_linear_create(int32_t rows, int32_t blocks, const uint32_t* row_ptr, c
                         const float* vals, const float* rhs, const drape_rules* rules, drape_ctx** out);
 
 /* Selects the preconditioner of later solves (DRAPE_PRECOND_*). DRAPE_PRECOND_TWO_LEVEL adds a Galerkin
- * coarse correction on every fourth grid vertex to block-Jacobi; it needs problem.grid_columns with
- * (grid_columns - 1) divisible by 4. `weights` [4R] are the prolongation weights of the four parent slots
+ * coarse correction on every f-th grid vertex (f = rules.coarse_factor) to block-Jacobi; it needs
+ * problem.grid_columns with (grid_columns - 1) divisible by f. `weights` [4R] are the prolongation weights of the four parent slots
  * of each row (NULL: bilinear); slots without a parent ignore theirs. No reference counterpart: the
  * reference's solver is block-Jacobi only (solver.ts:109-191). */
 int drape_set_preconditioner(drape_ctx* ctx, int32_t kind, const float* weights);

@@ -328,6 +328,8 @@ double or_static_floor(or_drape* d, const double* x) {
   return static_floor(dr, vec3s(x, dr.mesh.rest.size()));
 }
 
+void or_drape_set_kappa(or_drape* d, double kappa) { d->drape.contact.barrier.kappa = kappa; }
+
 int32_t or_simulate_states(or_drape* d, int32_t steps, const double* x0, const double* v0, double kappa_start,
                            double tolerance, int32_t max_iterations, int32_t regularized, double floor_factor,
                            int32_t adaptive, double kappa_distance, double kappa_max_factor, int32_t* iterations,

@@ -80,6 +80,9 @@ int32_t or_simulate(or_drape* d, int32_t steps, double tolerance, int32_t max_it
                     double floor_factor, int32_t* iterations, double* history, int64_t history_cap, int64_t* offsets,
                     double* final_positions);
 
+/* Sets the barrier stiffness of every collider term (the adaptive update's value at a probe step). */
+void or_drape_set_kappa(or_drape* d, double kappa);
+
 /* A trajectory from x0 / v0 (NULL: rest, zero velocity) with the barrier stiffness kappa_start (<= 0: the scene's),
  * keeping the states listed in keep[keep_count] (state k after k steps, 0 the start): positions and velocities
  * [keep_count][3V]; per step the Newton iterations, the stiffness used and the smallest vertex-collider distance

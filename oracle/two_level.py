@@ -12,12 +12,12 @@ here is pinned by mathematics instead (tests/test_two_level_oracle.py):
   - two-level PCG converges to the direct solution and in exact arithmetic never takes more iterations
     than the dimension; with a zero coarse space it is block-Jacobi PCG step for step.
 
-Layout (shared with the HIP library): coarse vertices are every fourth grid vertex in each direction of a
-sheet with `columns` vertices per side ((columns - 1) % 4 == 0); coarse rows are the free coarse vertices
-in vertex order; each fine row has four parent slots k = 0..3 = the corners (a0, b0), (a0 + 1, b0),
-(a0, b0 + 1), (a0 + 1, b0 + 1) of its coarse cell with a0 = min(x // 4, nc - 2), weights
-(1 - fx)(1 - fy), fx (1 - fy), (1 - fx) fy, fx fy; a slot whose corner is pinned, or whose weight is 0,
-is empty (parent -1, weight 0).
+Layout (shared with the HIP library): coarse vertices are every f-th grid vertex in each direction of a
+sheet with `columns` vertices per side ((columns - 1) % f == 0; f = 4 unless given); coarse rows are the free
+coarse vertices in vertex order; each fine row has four parent slots k = 0..3 = the corners (a0, b0),
+(a0 + 1, b0), (a0, b0 + 1), (a0 + 1, b0 + 1) of its coarse cell with a0 = min(x // f, nc - 2),
+fx = (x - f a0) / f, weights (1 - fx)(1 - fy), fx (1 - fy), (1 - fx) fy, fx fy; a slot whose corner is
+pinned, or whose weight is 0, is empty (parent -1, weight 0).
 """
 from __future__ import annotations
 
@@ -25,12 +25,12 @@ import numpy as np
 import scipy.sparse as sp
 
 
-def coarse_layout(columns: int, row_vertex: np.ndarray):
-    """(Nc, parent [R, 4] int32, weights [R, 4] float64) of the stride-4 coarse grid."""
-    n = int(columns)
-    if n < 9 or (n - 1) % 4:
-        raise ValueError("the sheet needs (columns - 1) divisible by 4")
-    nc = (n - 1) // 4 + 1
+def coarse_layout(columns: int, row_vertex: np.ndarray, factor: int = 4):
+    """(Nc, parent [R, 4] int32, weights [R, 4] float64) of the stride-`factor` coarse grid."""
+    n, f = int(columns), int(factor)
+    if f < 2 or n < 2 * f + 1 or (n - 1) % f:
+        raise ValueError(f"the sheet needs (columns - 1) divisible by {f}")
+    nc = (n - 1) // f + 1
     row_vertex = np.asarray(row_vertex, np.int64)
     free = np.zeros(n * n, bool)
     free[row_vertex] = True
@@ -38,7 +38,7 @@ def coarse_layout(columns: int, row_vertex: np.ndarray):
     count = 0
     for b in range(nc):
         for a in range(nc):
-            if free[(4 * b) * n + 4 * a]:
+            if free[(f * b) * n + f * a]:
                 coarse_of[b, a] = count
                 count += 1
     R = len(row_vertex)
@@ -46,8 +46,8 @@ def coarse_layout(columns: int, row_vertex: np.ndarray):
     weights = np.zeros((R, 4), np.float64)
     for row, v in enumerate(row_vertex):
         x, y = int(v) % n, int(v) // n
-        a0, b0 = min(x // 4, nc - 2), min(y // 4, nc - 2)
-        fx, fy = (x - 4.0 * a0) / 4.0, (y - 4.0 * b0) / 4.0
+        a0, b0 = min(x // f, nc - 2), min(y // f, nc - 2)
+        fx, fy = (x - float(f) * a0) / f, (y - float(f) * b0) / f
         corners = [(a0, b0), (a0 + 1, b0), (a0, b0 + 1), (a0 + 1, b0 + 1)]
         wts = [(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy]
         for k, ((a, b), w) in enumerate(zip(corners, wts)):

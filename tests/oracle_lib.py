@@ -95,6 +95,7 @@ def lib():
         L.or_simulate.restype = C.c_int32
         L.or_simulate.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_double, i32p,
                                   f64p, C.c_int64, i64p, f64p]
+        L.or_drape_set_kappa.argtypes = [C.c_void_p, C.c_double]
         L.or_ptc_steps.restype = C.c_int32
         L.or_ptc_steps.argtypes = [C.c_void_p, f64p, C.c_double, C.c_int32, f64p, C.c_int32, f64p,
                                    C.POINTER(PtcStats)]
@@ -154,6 +155,9 @@ class Drape:
     @property
     def kappa(self):
         return lib().or_drape_kappa(self.h)
+
+    def set_kappa(self, kappa):
+        lib().or_drape_set_kappa(self.h, kappa)
 
     @property
     def gravity_load(self):
