@@ -55,7 +55,7 @@ class Rules(C.Structure):
                 ("coarse_half", C.c_int32), ("refresh_rate", C.c_double), ("refresh_on_breakdown", C.c_int32),
                 ("self_contact", C.c_int32), ("coarse_async", C.c_int32), ("block_jacobi_truncate", C.c_int32),
                 ("adaptive_kappa", C.c_int32), ("kappa_distance", C.c_double), ("kappa_max_factor", C.c_double),
-                ("coarse_factor", C.c_int32)]
+                ("coarse_factor", C.c_int32), ("coarse_split", C.c_int32)]
 
 
 class PcgStats(C.Structure):
@@ -163,6 +163,7 @@ def lib():
         L.drape_simulate.argtypes = [vp, C.c_int32, f32p, f32p, f32p, f32p, i32p, f32p, C.POINTER(Record), C.c_int32,
                                      i32p]
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
+        L.drape_two_level_ordering.argtypes = [vp, i32p]
         L.drape_ptc_probe.argtypes = [vp, f32p, C.c_double, C.c_int32, C.POINTER(Record), C.c_int32, i32p, f32p,
                                       f32p]
         L.drape_kernel_timing.argtypes = [vp, C.c_int32, C.c_int32, f64p, f64p]
@@ -535,10 +536,18 @@ class Drape:
         return out.reshape(-1, 4)
 
     def two_level_inverse_factor(self):
-        """B = L^-1 of the current coarse factorisation (FP32, dense lower triangular)."""
+        """B = L^-1 of the current coarse factorisation (FP32, dense lower triangular) in the factor's ordering
+        (two_level_ordering)."""
         nc = self.two_level_layout()[0]
         out = np.zeros((3 * nc, 3 * nc), np.float32)
         _check(lib().drape_two_level_inverse_factor(self.h, _ptr(out, f32p)))
+        return out
+
+    def two_level_ordering(self):
+        """The natural coarse unknown 3 I + q at each position of the factor's ordering [3Nc]."""
+        nc = self.two_level_layout()[0]
+        out = np.zeros(3 * nc, np.int32)
+        _check(lib().drape_two_level_ordering(self.h, _ptr(out, i32p)))
         return out
 
     # ---- the step and its caller ----

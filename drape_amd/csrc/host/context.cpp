@@ -341,6 +341,7 @@ void drape_default_rules(drape_rules* r) {
   r->kappa_distance = 0.1;
   r->kappa_max_factor = 100.0;
   r->coarse_factor = 0;
+  r->coarse_split = 1;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }
@@ -695,6 +696,13 @@ int drape_two_level_inverse_factor(drape_ctx* p, float* factor) {
   if (!c || !factor) { set_error("null argument"); return DRAPE_E_INVALID; }
   if (!c->two_level) { set_error("no two-level layout on this context"); return DRAPE_E_INVALID; }
   return two_level_inverse_factor(*c, factor);
+}
+
+int drape_two_level_ordering(drape_ctx* p, int32_t* order) {
+  Context* c = ctx_of(p);
+  if (!c || !order) { set_error("null argument"); return DRAPE_E_INVALID; }
+  if (!c->two_level) { set_error("no two-level layout on this context"); return DRAPE_E_INVALID; }
+  return two_level_ordering(*c, order);
 }
 
 int drape_set_prolongation_model(drape_ctx* p, const float* params, int32_t count) {

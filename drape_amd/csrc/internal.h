@@ -244,7 +244,7 @@ int two_level_iterate(Context& c, const float* partial_pq, const float4* p, floa
 // The coarse correction's parts for the V-cycle: yc = B^T B P^T r (guarded like two_level_apply) and the
 // device arrays the prolongation reads.
 struct CoarseView {
-  const int32_t* parent;  // [4R] coarse row of each parent slot, -1 if none
+  const int32_t* ppos;    // [4R] position of each parent slot's node in yc (the coarse factor's ordering), -1 if none
   const float* w;         // [4R] prolongation weights
   const float* yc;        // [3Nc] the coarse solution
 };
@@ -253,6 +253,7 @@ int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32 =
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host
+int two_level_ordering(Context& c, int32_t* order);    // the natural unknown at each position of the factor
 int two_level_gemv_timing(Context& c, bool lower, int launches, double* avg_ms, double* bytes);
 int two_level_set_model(Context& c, const float* theta, int count);  // learned prolongation (NULL: none)
 // Factorises the assembled system with the current weights (the network is not run), unconditionally.

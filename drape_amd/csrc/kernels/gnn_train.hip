@@ -135,8 +135,8 @@ __global__ void __launch_bounds__(256) k_spmv_dot(SellView A, const float4* __re
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// eps = e - P c
-__global__ void __launch_bounds__(256) k_eps(const int32_t* __restrict__ parent, const float* __restrict__ w,
+// eps = e - P c (ppos: the position of each parent slot's node in c, -1 when empty)
+__global__ void __launch_bounds__(256) k_eps(const int32_t* __restrict__ ppos, const float* __restrict__ w,
                                              const float* __restrict__ yc, const float4* __restrict__ e,
                                              float4* __restrict__ eps, int R) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -144,12 +144,12 @@ __global__ void __launch_bounds__(256) k_eps(const int32_t* __restrict__ parent,
   float4 v = e[i];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int I = parent[4 * i + k];
-    if (I >= 0) {
+    const int P = ppos[4 * i + k];
+    if (P >= 0) {
       const float ww = w[4 * i + k];
-      v.x = v.x - ww * yc[3 * I];
-      v.y = v.y - ww * yc[3 * I + 1];
-      v.z = v.z - ww * yc[3 * I + 2];
+      v.x = v.x - ww * yc[P];
+      v.y = v.y - ww * yc[P + 1];
+      v.z = v.z - ww * yc[P + 2];
     }
   }
   eps[i] = v;
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_sum_pair(const float* __restrict__ part
 }
 
 // gw[i, k] (+)= -2 / (n |e|_A^2) (A eps)_i . c[parent(i, k)]  (the first vector writes, the others add)
-__global__ void __launch_bounds__(256) k_weight_grad(const int32_t* __restrict__ parent, const float* __restrict__ yc,
+__global__ void __launch_bounds__(256) k_weight_grad(const int32_t* __restrict__ ppos, const float* __restrict__ yc,
                                                      const float4* __restrict__ aeps, const double* __restrict__ sums,
                                                      int n, int first, float* __restrict__ gw, int R) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -191,9 +191,9 @@ __global__ void __launch_bounds__(256) k_weight_grad(const int32_t* __restrict__
   const float4 a = aeps[i];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int I = parent[4 * i + k];
+    const int P = ppos[4 * i + k];
     float v = 0.0f;
-    if (I >= 0) v = scale * (a.x * yc[3 * I] + a.y * yc[3 * I + 1] + a.z * yc[3 * I + 2]);
+    if (P >= 0) v = scale * (a.x * yc[P] + a.y * yc[P + 1] + a.z * yc[P + 2]);
     gw[4 * i + k] = first ? v : gw[4 * i + k] + v;
   }
 }
@@ -489,10 +489,10 @@ int gnn_train_gradient(Context& c, const float* params, int count, double* loss,
     const float4* E = g->d_E + (size_t)e * R;
     k_spmv_dot<<<c.G, 256, 0, s>>>(A, E, g->d_q, g->d_partial);
     TRY_RC(two_level_restrict_solve(c, g->d_q, 2, true));
-    k_eps<<<blocks_for(R), 256, 0, s>>>(cv.parent, cv.w, cv.yc, E, g->d_eps, R);
+    k_eps<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.w, cv.yc, E, g->d_eps, R);
     k_spmv_dot<<<c.G, 256, 0, s>>>(A, g->d_eps, g->d_q, g->d_partial + c.G);
     k_sum_pair<<<1, 256, 0, s>>>(g->d_partial, c.G, g->d_sums + 2 * e);
-    k_weight_grad<<<blocks_for(R), 256, 0, s>>>(cv.parent, cv.yc, g->d_q, g->d_sums + 2 * e, n, e == 0, g->d_gw, R);
+    k_weight_grad<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.yc, g->d_q, g->d_sums + 2 * e, n, e == 0, g->d_gw, R);
   }
   // backward
   const float* theta = a.theta;

@@ -379,12 +379,12 @@ __device__ inline float4 vc_z2(const float4* __restrict__ z1, const CoarseView& 
   float4 v = z1[row];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int I = cv.parent[4 * row + k];
-    if (I >= 0) {
+    const int P = cv.ppos[4 * row + k];
+    if (P >= 0) {
       const float ww = cv.w[4 * row + k];
-      v.x = v.x + ww * cv.yc[3 * I];
-      v.y = v.y + ww * cv.yc[3 * I + 1];
-      v.z = v.z + ww * cv.yc[3 * I + 2];
+      v.x = v.x + ww * cv.yc[P];
+      v.y = v.y + ww * cv.yc[P + 1];
+      v.z = v.z + ww * cv.yc[P + 2];
     }
   }
   return v;

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <vector>
@@ -60,19 +61,31 @@ struct TwoLevel {
   uint32_t* d_pair_ptr = nullptr;   // [npairs+1]
   uint32_t* d_pair_key = nullptr;   // [npairs] I * Nc + J
   uint4* d_pair_terms = nullptr;    // (block b, slot ik, slot jl, 0)
-  int nt = 0;                       // 64-row tiles of the coarse matrix (padded with identity rows)
-  int kb = 0;                       // sub-diagonal tiles in the band
+  int nt = 0;                       // 64-row tiles of the coarse matrix (segments padded with identity rows)
+  int kb = 0;                       // sub-diagonal tiles in a segment's band
+  int ts = 0;                       // separator tiles (0 without the split)
+  int seg[4] = {0, 0, 0, 0};        // first tile of the first domain, the second domain, the separator; nt
+  int bstart[2] = {0, 0};           // first border tile of each domain
+  int real[3] = {0, 0, 0};          // unknowns of each segment
+  double tri_entries = 0.0;         // stored entries of B on or below the diagonal (between real unknowns)
+  std::vector<int32_t> order;       // [N3] natural unknown 3 I + q at each real position of the factor's ordering
+  int32_t* d_pos = nullptr;         // [Nc] position of each coarse node's first unknown in the factor's ordering
+  int32_t* d_ppos = nullptr;        // [4R] position of each parent slot's node, -1 when empty
   double* d_band = nullptr;         // [nt][kb + 1][64 x 64]: tile (k + d, k) of A_c, then of L, row-major
   double* d_linv = nullptr;         // [nt][64 x 64]: inverse of each diagonal tile of L
   double* d_lpanel = nullptr;       // [nt][kb][64 x 64]: tile (k + d, k) of L, d = 1..kb
-  double* d_xring = nullptr;        // [kb + 1][64][N3p]: the last tile rows of L^-1 in FP64 (inverse sweep)
+  double* d_border = nullptr;       // [2][kb + 1][ts][64 x 64]: separator rows of A_c against each domain's border
+  double* d_lborder = nullptr;      // the same tiles of L
+  double* d_sacc = nullptr;         // [ts][ts][64 x 64]: the second domain's Schur update of the separator block
+  double* d_xring = nullptr;        // [3][kb + 1][64][N3p]: the last tile rows of L^-1 per segment in FP64
   hipStream_t inv_stream = nullptr; // the inverse sweep, behind each panel of the factorisation chain
   hipEvent_t ev_panel = nullptr, ev_inverse = nullptr;
   double* d_Ac = nullptr;           // [N3 * N3] dense A_c for the check entry point (allocated on demand)
-  float* d_B = nullptr;             // [N3 * N3p] row-major B = L^-1 (zero above the diagonal)
-  float* d_BT = nullptr;            // [N3 * N3p] row-major B^T (zero below the diagonal)
-  _Float16* d_Bh = nullptr;         // [N3 * N3p] FP16 B~ = S^-1 B, S = diag(B) (zero above the diagonal)
-  _Float16* d_BTh = nullptr;        // [N3 * N3p] its transpose (the same values)
+  float* d_B = nullptr;             // [N3p * N3p] row-major B = L^-1 in the factor's ordering (zero above the
+                                    // diagonal, between the domains and at padding)
+  float* d_BT = nullptr;            // [N3p * N3p] row-major B^T
+  _Float16* d_Bh = nullptr;         // [N3p * N3p] FP16 B~ = S^-1 B, S = diag(B)
+  _Float16* d_BTh = nullptr;        // [N3p * N3p] its transpose (the same values)
   bool half_valid = false;          // the FP16 copies belong to the current factorisation
   // coarse_reuse 4: the factorisation of the step size used before the current one, kept for when the
   // step size returns to it (PTC alternates between neighbouring step sizes)
@@ -129,7 +142,8 @@ void two_level_destroy(TwoLevel* t) {
   if (q.side) (void)hipStreamDestroy(q.side);
   for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_wbil, (void*)t->d_fxy, (void*)t->d_theta,
                   (void*)t->d_gnn, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
-                  (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT, (void*)t->d_Bh, (void*)t->d_BTh,
+                  (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel,
+                  (void*)t->d_border, (void*)t->d_lborder, (void*)t->d_sacc, (void*)t->d_pos, (void*)t->d_ppos, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT, (void*)t->d_Bh, (void*)t->d_BTh,
                   (void*)t->saved.linv, (void*)t->saved.B, (void*)t->saved.BT, (void*)t->saved.Bh, (void*)t->saved.BTh,
                   (void*)t->d_t, (void*)t->d_rc, (void*)t->d_yc, (void*)t->d_info})
     if (p) (void)hipFree(p);
@@ -141,10 +155,60 @@ void two_level_destroy(TwoLevel* t) {
 // ---- banded FP64 Cholesky and inverse on 64 x 64 tiles ----------------------------------------------
 constexpr int TILE = 64;
 constexpr int MAX_KB = 8;   // sub-diagonal tiles: coarse grids up to 83 vertices a side
+constexpr int MAX_TS = 8;   // separator tiles
 constexpr int LSTR = 65;    // LDS row stride of column-accessed tiles (bank spread)
 
-__device__ inline double* band_tile(double* band, int kb, int k, int d) {
-  return band + (((size_t)k * (kb + 1) + d) << 12);
+// The factor's ordering of the coarse unknowns: up to three segments, each padded to whole 64-row tiles.
+// With the split (rules.coarse_split), the first domain holds the coarse grid rows above a separator of as
+// many rows as the coarse operator's coupling reaches, in row order; the second the rows below it, in reverse
+// order, so that its rows next to the separator come last; the separator follows both (one level of nested
+// dissection). Without it the first segment holds every unknown in row order. A_c and L then have nonzero
+// tiles only inside each segment's band and between the separator and the last tiles of each domain (its
+// border): the two domains factor side by side, and B = L^-1 is zero between them.
+struct Tiles {
+  double* band;     // [nt][kb + 1]: tile (k + d, k) inside a segment, of A_c, then the factor's diagonal tiles
+  double* lpanel;   // [nt][kb]: factor tile (k + d, k), d = 1..kb, inside a segment
+  double* border;   // [2][kb + 1][ts]: tile (separator tile a, border tile bstart[e] + j) of A_c
+  double* lborder;  // the same tiles of the factor
+  double* sacc;     // [ts][ts]: the second domain's Schur update of the separator block (below the diagonal)
+  int kb, nt, ts;
+  int seg[4];       // first tile of each segment (first domain, second domain, separator), seg[3] = nt
+  int bstart[2];    // first border tile of each domain (= its segment's end without a separator)
+  int real[3];      // unknowns of each segment; the rest of its tiles is padding with a unit diagonal
+};
+
+__device__ inline int seg_of(const Tiles& t, int T) { return T < t.seg[1] ? 0 : (T < t.seg[2] ? 1 : 2); }
+__device__ inline bool real_row(const Tiles& t, int R) {
+  const int s = seg_of(t, R >> 6);
+  return R - TILE * t.seg[s] < t.real[s];
+}
+__device__ inline double* band_tile(const Tiles& t, int k, int d) {
+  return t.band + (((size_t)k * (t.kb + 1) + d) << 12);
+}
+__device__ inline size_t border_offset(const Tiles& t, int e, int J, int a) {
+  return ((size_t)(e * (t.kb + 1) + J - t.bstart[e]) * t.ts + a) << 12;
+}
+// Tile (I, J), I >= J, of A_c (inside a segment or on a border).
+__device__ inline double* a_tile(const Tiles& t, int I, int J) {
+  const int sJ = seg_of(t, J);
+  if (seg_of(t, I) == sJ) return band_tile(t, J, I - J);
+  return t.border + border_offset(t, sJ, J, I - t.seg[2]);
+}
+// Factor tile (I, J), I > J.
+__device__ inline double* l_tile(const Tiles& t, int I, int J) {
+  const int sJ = seg_of(t, J);
+  if (seg_of(t, I) == sJ) return t.lpanel + (((size_t)J * t.kb + (I - J - 1)) << 12);
+  return t.lborder + border_offset(t, sJ, J, I - t.seg[2]);
+}
+// The tile row of slot j below the diagonal tile of column k, or -1: slots 0 .. kb-1 are the band tiles
+// k + 1 + j of k's segment, slots kb .. kb+ts-1 the separator's tiles when k is a border tile of a domain.
+__device__ inline int below_tile(const Tiles& t, int k, int j) {
+  const int s = seg_of(t, k);
+  if (j < t.kb) {
+    const int I = k + 1 + j;
+    return I < t.seg[s + 1] ? I : -1;
+  }
+  return (s < 2 && k >= t.bstart[s]) ? t.seg[2] + (j - t.kb) : -1;
 }
 
 // Copies a ROWS x 64 row-major FP64 tile into LDS rows of stride LSTR: every load of the thread is issued
@@ -162,16 +226,17 @@ __device__ inline void tile_to_lds(const double* __restrict__ src, double* dst) 
   }
 }
 
-// The coarse block pair (I, J) into the band, entries on or below the diagonal: one wavefront per pair,
-// lane l summing the pair's terms l, l + 64, ... for all nine entries, then a fixed-order wavefront tree, so
-// the sum is reproducible. (One thread per entry walking a pair's few hundred terms in sequence was
-// latency-bound: 248 us per product at n129.)
+// The coarse block pair (I, J) into the tiles, entries on or below the diagonal of the factor's ordering
+// (pos: the position of each coarse node's first unknown): one wavefront per pair, lane l summing the pair's
+// terms l, l + 64, ... for all nine entries, then a fixed-order wavefront tree, so the sum is reproducible.
+// (One thread per entry walking a pair's few hundred terms in sequence was latency-bound: 248 us per product
+// at n129.)
 __global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restrict__ pair_ptr,
                                                        const uint32_t* __restrict__ pair_key,
                                                        const uint4* __restrict__ terms, const float* __restrict__ w,
                                                        const float* __restrict__ sell_val,
                                                        const uint32_t* __restrict__ block_dst, int npairs, int Nc,
-                                                       int kb, double* __restrict__ band) {
+                                                       const int32_t* __restrict__ pos, Tiles tl) {
   const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (p >= npairs) return;  // uniform per wavefront
@@ -193,22 +258,30 @@ __global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restric
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_down(acc[q], o, 64);
   if (lane == 0) {
+    const int pI = pos[I], pJ = pos[J];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-      const int R = 3 * I + q / 3, C = 3 * J + q % 3;
+      const int R = pI + q / 3, C = pJ + q % 3;
       if (R < C) continue;
-      const int k = C >> 6, d = (R >> 6) - k;
-      band_tile(band, kb, k, d)[((R & 63) << 6) + (C & 63)] = acc[q];
+      a_tile(tl, R >> 6, C >> 6)[((R & 63) << 6) + (C & 63)] = acc[q];
     }
   }
 }
 
-// Diagonal of A_c scaled by `factor`; the padding rows past N3 get a unit diagonal.
-__global__ void k_band_diagonal(double* __restrict__ band, int kb, int nt, int N3, double factor) {
+// Diagonal of A_c scaled by `factor`; the padding rows of each segment get a unit diagonal.
+__global__ void k_band_diagonal(Tiles tl, double factor) {
   const int R = blockIdx.x * blockDim.x + threadIdx.x;
-  if (R >= nt * TILE) return;
-  double* a = band_tile(band, kb, R >> 6, 0) + ((R & 63) << 6) + (R & 63);
-  *a = R < N3 ? *a * factor : 1.0;
+  if (R >= tl.nt * TILE) return;
+  double* a = band_tile(tl, R >> 6, 0) + ((R & 63) << 6) + (R & 63);
+  *a = real_row(tl, R) ? *a * factor : 1.0;
+}
+
+// The second domain's Schur update into the separator block (tiles on or below its diagonal).
+__global__ void k_add_separator(Tiles tl) {
+  const int tile = blockIdx.x >> 4, e = ((blockIdx.x & 15) << 8) + threadIdx.x;
+  const int a = tile / tl.ts, b = tile % tl.ts;
+  if (b > a) return;
+  band_tile(tl, tl.seg[2] + b, a - b)[e] += tl.sacc[((size_t)tile << 12) + e];
 }
 
 // LDS traffic between the lanes of one wavefront: LDS instructions of a wavefront execute in order, so a
@@ -224,15 +297,14 @@ __device__ inline double rsqrt_f64(double x) {
   return y;
 }
 
-// Diagonal tile of tile column k, blocked by 16-column panels. Wavefront 0 factors each panel four columns
+// Diagonal tile of tile column k (cols.x, or cols.y for the second workgroup), blocked by 16-column panels. Wavefront 0 factors each panel four columns
 // at a time: the 4 x 4 diagonal block is broadcast through LDS and factored redundantly by every lane, each
 // lane (= row) solves its four factor entries and broadcasts them, and the lanes apply the rank-4 update to
 // the rest of their panel row. All four wavefronts then apply the panel's rank-16 update to the trailing
 // lower triangle. The tile's inverse follows blockwise: the four 16 x 16 diagonal blocks in parallel, then
 // the off-diagonal blocks by distance from the diagonal, X_ab = -X_aa sum_c L_ac X_cb. A pivot that is not
 // positive (or not finite) records 64 k + column + 1 in *info and stops every later kernel.
-__global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ band, double* __restrict__ linv, int k,
-                                                    int kb, int* info) {
+__global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict__ linv, int2 cols, int* info) {
   __shared__ double Ls[TILE * LSTR];
   __shared__ double Xs[TILE * LSTR];
   __shared__ double Ys[3 * 16 * 17];
@@ -241,8 +313,9 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ band, d
   __shared__ double rdiag[TILE];
   __shared__ int fail;
   if (*info != 0) return;
+  const int k = blockIdx.x ? cols.y : cols.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double* tile = band_tile(band, kb, k, 0);
+  double* tile = band_tile(t, k, 0);
   tile_to_lds<TILE>(tile, Ls);
   for (int e = tid; e < TILE * TILE; e += 256) Xs[(e >> 6) * LSTR + (e & 63)] = 0.0;
   if (tid == 0) fail = 0;
@@ -417,18 +490,20 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ band, d
   for (int e = tid; e < TILE * TILE; e += 256) inv[e] = Xs[(e >> 6) * LSTR + (e & 63)];
 }
 
-// Sub-diagonal factor tiles of tile column k: L(k + d, k) = A(k + d, k) L_kk^-T. Workgroup (d, quarter)
-// forms 16 rows of tile d.
-__global__ void __launch_bounds__(256) k_chol_trsm(const double* __restrict__ band, double* __restrict__ lpanel,
-                                                   const double* __restrict__ linv, int k, int nt, int kb,
+// Factor tiles below the diagonal of tile column k (cols.x, or cols.y for the second half of the grid):
+// L(I, k) = A(I, k) L_kk^-T for every below slot of k; workgroup (slot, quarter) forms 16 rows of the tile.
+__global__ void __launch_bounds__(256) k_chol_trsm(Tiles t, const double* __restrict__ linv, int2 cols,
                                                    const int* info) {
   __shared__ double Xs[TILE * LSTR];
   __shared__ double As[16 * LSTR];
   if (*info != 0) return;
-  const int d = blockIdx.x / 4 + 1, quarter = blockIdx.x % 4;
-  if (d > kb || k + d >= nt) return;
+  const int slots = t.kb + t.ts, per = 4 * slots;
+  const int k = blockIdx.x < per ? cols.x : cols.y;
+  const int j = (blockIdx.x % per) / 4, quarter = blockIdx.x % 4;
+  const int I = below_tile(t, k, j);
+  if (I < 0) return;
   const int tid = threadIdx.x;
-  const double* a = band_tile(const_cast<double*>(band), kb, k, d) + (quarter * 16 << 6);
+  const double* a = a_tile(t, I, k) + (quarter * 16 << 6);
   const double* inv = linv + ((size_t)k << 12);
   tile_to_lds<TILE>(inv, Xs);
   tile_to_lds<16>(a, As);
@@ -442,29 +517,32 @@ __global__ void __launch_bounds__(256) k_chol_trsm(const double* __restrict__ ba
 #pragma unroll
     for (int v = 0; v < 4; ++v) o[v] += av * Xs[(c0 + 16 * v) * LSTR + m];
   }
-  double* l = lpanel + (((size_t)k * kb + (d - 1)) << 12) + ((quarter * 16 + r) << 6) + c0;
+  double* l = l_tile(t, I, k) + ((quarter * 16 + r) << 6) + c0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) l[16 * v] = o[v];
 }
 
-// Trailing update after tile column k: tile (k + d1, k + d2) -= L(k + d1, k) L(k + d2, k)^T for
-// 1 <= d2 <= d1 <= kb; workgroup (pair, quarter) updates 16 rows of the tile.
-__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ band, const double* __restrict__ lpanel,
-                                                     int k, int nt, int kb, const int* info) {
+// Trailing update after tile column k (cols.x, or cols.y for the second half of the grid): tile (I, J) -=
+// L(I, k) L(J, k)^T for every pair of below slots i >= j of k; workgroup (pair, quarter) updates 16 rows of
+// the tile. Separator tiles updated from the second domain go to the separate Schur block, so that the two
+// domains' columns never write the same tile.
+__global__ void __launch_bounds__(256) k_chol_update(Tiles t, int2 cols, const int* info) {
   __shared__ double Ps[16 * LSTR];
   __shared__ double Qs[TILE * LSTR];
   if (*info != 0) return;
-  int d1 = 1, idx = blockIdx.x / 4;
+  const int slots = t.kb + t.ts, per = 4 * (slots * (slots + 1) / 2);
+  const int k = blockIdx.x < per ? cols.x : cols.y;
+  int i = 0, idx = (blockIdx.x % per) / 4;
   const int quarter = blockIdx.x % 4;
-  while (idx >= d1) {
-    idx -= d1;
-    ++d1;
+  while (idx > i) {
+    idx -= i + 1;
+    ++i;
   }
-  const int d2 = idx + 1;
-  if (d1 > kb || k + d1 >= nt) return;
+  const int I = below_tile(t, k, i), J = below_tile(t, k, idx);
+  if (I < 0 || J < 0) return;
   const int tid = threadIdx.x;
-  const double* p = lpanel + (((size_t)k * kb + (d1 - 1)) << 12) + (quarter * 16 << 6);
-  const double* q = lpanel + (((size_t)k * kb + (d2 - 1)) << 12);
+  const double* p = l_tile(t, I, k) + (quarter * 16 << 6);
+  const double* q = l_tile(t, J, k);
   tile_to_lds<TILE>(q, Qs);
   tile_to_lds<16>(p, Ps);
   __syncthreads();
@@ -476,183 +554,47 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ band, 
 #pragma unroll
     for (int v = 0; v < 4; ++v) o[v] += pv * Qs[(c0 + 16 * v) * LSTR + m];
   }
-  double* out = band_tile(band, kb, k + d2, d1 - d2) + ((quarter * 16 + r) << 6) + c0;
+  double* tile;
+  if (seg_of(t, I) == 2 && seg_of(t, J) == 2 && seg_of(t, k) == 1)
+    tile = t.sacc + ((size_t)((I - t.seg[2]) * t.ts + (J - t.seg[2])) << 12);
+  else
+    tile = a_tile(t, I, J);
+  double* out = tile + ((quarter * 16 + r) << 6) + c0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) out[16 * v] -= o[v];
 }
 
-// B = L^-1 by blocked forward substitution: workgroup (J, g) owns the 16 columns 64 J + 16 g .. + 15 and
-// walks the row tiles I = J, J + 1, ...: X_I = Linv_II (E_I - sum_d L(I, I - d) X_{I - d}), the last kb
-// tiles of X kept in LDS. The factor tiles stream through one LDS tile buffer, the next two prefetched into
-// registers while the current one is used. Each product splits its 64-long sums over the four wavefronts
-// (16 terms each, a 4 x 4 block of outputs per lane), and the partial sums meet in LDS in a fixed order.
-// FP32 B (row-major) and B^T (row-major) receive the rows and columns below N3, and so do the FP16 copies
-// B~ = S^-1 B and its transpose when Bh is not null, S = diag(B) = the reciprocal factor diagonal (the
-// diagonal of linv), with |B~| clamped to the FP16 range.
-constexpr int INV_COLS = 16;
-__global__ void __launch_bounds__(256) k_band_inverse(const double* __restrict__ lpanel,
-                                                      const double* __restrict__ linv, int nt, int kb, int N3,
-                                                      int N3p, float* __restrict__ B, float* __restrict__ BT,
-                                                      _Float16* __restrict__ Bh, _Float16* __restrict__ BTh,
-                                                      const int* info) {
-  __shared__ double Xh[(MAX_KB + 1) * TILE * INV_COLS];
-  __shared__ double T[TILE * INV_COLS];
-  __shared__ double Part[4 * TILE * INV_COLS];
-  __shared__ double Lb[TILE * LSTR];
-  if (*info != 0) return;
-  const int J = blockIdx.x / (TILE / INV_COLS), g = blockIdx.x % (TILE / INV_COLS);
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int r0 = (lane >> 2) * 4, c0 = (lane & 3) * 4, m0 = 16 * wave;  // 4 x 4 outputs, 16 terms per wave
-  const int j0 = TILE * J + INV_COLS * g;                                // first global column of the group
-  // stage (I, d): d = 1 .. min(I - J, kb) are the factor tiles L(I, I - d), d = 0 the diagonal inverse
-  auto tile_of = [&](int I, int d) -> const double* {
-    return d == 0 ? linv + ((size_t)I << 12) : lpanel + (((size_t)(I - d) * kb + (d - 1)) << 12);
-  };
-  auto first_stage = [&](int I) { return (I - J < kb ? I - J : kb) > 0 ? 1 : 0; };
-  auto next_stage = [&](int I, int d, int* nI, int* nd) {
-    const int dmax = I - J < kb ? I - J : kb;
-    if (d != 0 && d < dmax) {
-      *nI = I;
-      *nd = d + 1;
-    } else if (d != 0) {
-      *nI = I;
-      *nd = 0;
-    } else {
-      *nI = I + 1;
-      *nd = first_stage(I + 1);
-    }
-  };
-  // two stages of factor tiles in flight in registers
-  double pre[16], pre2[16];
-  int I = J, d = first_stage(J);
-  int I2, d2;
-  next_stage(I, d, &I2, &d2);
-  {
-    const double* src = tile_of(I, d);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) pre[i] = src[tid + 256 * i];
-    if (I2 < nt) {
-      const double* src2 = tile_of(I2, d2);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) pre2[i] = src2[tid + 256 * i];
-    }
-  }
-  double acc[4][4] = {};   // this wave's part of -sum_d L(I, I - d) X_{I - d}
-  while (I < nt) {
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int e = tid + 256 * i;
-      Lb[(e >> 6) * LSTR + (e & 63)] = pre[i];
-    }
-    if (d == 0) {
-      // partial sums of the factor tiles, then T = E_I + their sum
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Part[wave * TILE * INV_COLS + (r0 + u) * INV_COLS + c0 + v] = acc[u][v];
-    }
-    __syncthreads();
-    int nI, nd, nI2 = nt, nd2 = 0;
-    next_stage(I, d, &nI, &nd);
-    if (nI < nt) next_stage(nI, nd, &nI2, &nd2);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) pre[i] = pre2[i];
-    if (nI2 < nt) {
-      const double* src = tile_of(nI2, nd2);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) pre2[i] = src[tid + 256 * i];
-    }
-    if (d != 0) {
-      const double* x = Xh + ((I - d) % (kb + 1)) * TILE * INV_COLS;
-#pragma unroll
-      for (int m = m0; m < m0 + 16; ++m) {
-        double lv[4], xv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) lv[u] = Lb[(r0 + u) * LSTR + m];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) xv[v] = x[m * INV_COLS + c0 + v];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[u][v] -= lv[u] * xv[v];
-      }
-    } else {
-      // T = E_I + sum over the four partials
-      for (int e = tid; e < TILE * INV_COLS; e += 256) {
-        const int rr = e / INV_COLS, cc = e % INV_COLS;
-        const double eye = (I == J && rr == INV_COLS * g + cc) ? 1.0 : 0.0;
-        T[e] = eye + ((Part[e] + Part[TILE * INV_COLS + e]) +
-                      (Part[2 * TILE * INV_COLS + e] + Part[3 * TILE * INV_COLS + e]));
-      }
-      __syncthreads();
-      double xo[4][4] = {};
-#pragma unroll
-      for (int m = m0; m < m0 + 16; ++m) {
-        double lv[4], tv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) lv[u] = Lb[(r0 + u) * LSTR + m];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) tv[v] = T[m * INV_COLS + c0 + v];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) xo[u][v] += lv[u] * tv[v];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Part[wave * TILE * INV_COLS + (r0 + u) * INV_COLS + c0 + v] = xo[u][v];
-      __syncthreads();
-      double* slot = Xh + (I % (kb + 1)) * TILE * INV_COLS;
-      for (int e = tid; e < TILE * INV_COLS; e += 256) {
-        const double xv = (Part[e] + Part[TILE * INV_COLS + e]) +
-                          (Part[2 * TILE * INV_COLS + e] + Part[3 * TILE * INV_COLS + e]);
-        slot[e] = xv;
-        const int R = TILE * I + e / INV_COLS, C = j0 + e % INV_COLS;
-        if (R < N3 && C < N3) {
-          B[(size_t)R * N3p + C] = (float)xv;
-          BT[(size_t)C * N3p + R] = (float)xv;
-          if (Bh) {
-            const double sr = linv[((size_t)(R >> 6) << 12) + (R & 63) * 65];
-            const _Float16 h = (_Float16)fmin(fmax(xv / sr, -65504.0), 65504.0);
-            Bh[(size_t)R * N3p + C] = h;
-            BTh[(size_t)C * N3p + R] = h;
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-    }
-    I = nI;
-    d = nd;
-  }
-}
-
-// Tile row I of B = L^-1 as a sweep over tile rows, so that row I can run as soon as the panel of tile column I
-// has produced L_II^-1 (the factor tiles L(I, I - d) came from earlier TRSMs): X(I, J) = L_II^-1 (E(I, J) -
-// sum_{d=1..min(kb, I-J)} L(I, I - d) X(I - d, J)) for the 64 (I + 1) columns of B's lower triangle, 16 per
-// workgroup. X keeps the last kb + 1 tile rows in FP64 ([kb + 1][64][N3p], slot I % (kb + 1)); B (row-major),
+// Tile rows of B = L^-1 as a sweep over tile rows, so that row I can run as soon as the panel of tile column I
+// has produced L_II^-1 (the factor tiles L(I, K) came from earlier TRSMs): X(I, J) = L_II^-1 (E(I, J) -
+// sum_K L(I, K) X(K, J)) over the structurally nonzero X(K, J): K >= J in I's segment band, and for a separator
+// row the border tiles of J's domain. One launch serves tile row rows.x (the first w1 workgroups) and rows.y;
+// 16 columns per workgroup, over the row's nonzero columns (the second domain's rows start at its first
+// column). X keeps the last kb + 1 tile rows of each segment in FP64 ([3][kb + 1][64][N3p]); B (row-major),
 // B^T and, when Bh is not null, the FP16 copies B~ = S^-1 B (S = diag(B), from L_II^-1) and its transpose get
-// the rows and columns below N3. Thread (row r, four columns): FP64 sums in a fixed order (d, then m).
-__global__ void __launch_bounds__(256) k_inverse_row(const double* __restrict__ lpanel, const double* __restrict__ linv,
-                                                     int I, int kb, int N3, int N3p, double* __restrict__ X,
-                                                     float* __restrict__ B, float* __restrict__ BT,
-                                                     _Float16* __restrict__ Bh, _Float16* __restrict__ BTh,
-                                                     const int* info) {
+// the entries between real unknowns. Thread (row r, four columns): FP64 sums in a fixed order.
+constexpr int INV_COLS = 16;
+__device__ inline int inverse_first_column(const Tiles& t, int I) { return seg_of(t, I) == 1 ? TILE * t.seg[1] : 0; }
+__global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __restrict__ linv, int2 rows, int w1,
+                                                     int N3p, double* __restrict__ X, float* __restrict__ B,
+                                                     float* __restrict__ BT, _Float16* __restrict__ Bh,
+                                                     _Float16* __restrict__ BTh, const int* info) {
   __shared__ double Ls[TILE * LSTR];
   __shared__ double Xs[TILE * 17];
   if (*info != 0) return;
+  const bool second = (int)blockIdx.x >= w1;
+  const int I = second ? rows.y : rows.x;
+  const int s = seg_of(t, I);
   const int tid = threadIdx.x, r = tid >> 2, cq = (tid & 3) * 4;
-  const int j0 = INV_COLS * blockIdx.x, J = j0 / TILE;
-  const int dmax = I - J < kb ? I - J : kb;
+  const int j0 = inverse_first_column(t, I) + INV_COLS * (second ? blockIdx.x - w1 : blockIdx.x), J = j0 / TILE;
+  auto ring = [&](int K) -> double* {
+    const int sk = seg_of(t, K);
+    return X + (size_t)(sk * (t.kb + 1) + (K - t.seg[sk]) % (t.kb + 1)) * TILE * N3p;
+  };
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int d = 1; d <= dmax; ++d) {
+  auto term = [&](int K) {
     __syncthreads();
-    tile_to_lds<TILE>(lpanel + (((size_t)(I - d) * kb + (d - 1)) << 12), Ls);
-    const double* xs = X + (size_t)((I - d) % (kb + 1)) * TILE * N3p + j0;
+    tile_to_lds<TILE>(l_tile(t, I, K), Ls);
+    const double* xs = ring(K) + j0;
     for (int e = tid; e < TILE * INV_COLS; e += 256) Xs[(e / INV_COLS) * 17 + e % INV_COLS] = xs[(size_t)(e / INV_COLS) * N3p + e % INV_COLS];
     __syncthreads();
 #pragma unroll 8
@@ -661,7 +603,14 @@ __global__ void __launch_bounds__(256) k_inverse_row(const double* __restrict__ 
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u] -= l * Xs[m * 17 + cq + u];
     }
+  };
+  if (s == 2) {
+    const int sJ = seg_of(t, J);
+    if (sJ < 2)
+      for (int K = t.bstart[sJ]; K < t.seg[sJ + 1]; ++K)
+        if (K >= J) term(K);
   }
+  for (int d = 1; d <= t.kb && I - d >= t.seg[s] && I - d >= J; ++d) term(I - d);
   __syncthreads();
   tile_to_lds<TILE>(linv + ((size_t)I << 12), Ls);
 #pragma unroll
@@ -678,13 +627,14 @@ __global__ void __launch_bounds__(256) k_inverse_row(const double* __restrict__ 
     for (int u = 0; u < 4; ++u) out[u] += l * Xs[m * 17 + cq + u];
   }
   const int R = TILE * I + r;
-  double* xo = X + (size_t)(I % (kb + 1)) * TILE * N3p + (size_t)r * N3p + j0 + cq;
+  double* xo = ring(I) + (size_t)r * N3p + j0 + cq;
   const double sr = Ls[r * LSTR + r];
+  const bool rreal = real_row(t, R);
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     xo[u] = out[u];
     const int C = j0 + cq + u;
-    if (R < N3 && C < N3 && C <= R) {
+    if (rreal && C <= R && real_row(t, C)) {
       B[(size_t)R * N3p + C] = (float)out[u];
       BT[(size_t)C * N3p + R] = (float)out[u];
       if (Bh) {
@@ -722,12 +672,12 @@ __global__ void __launch_bounds__(256) k_galerkin_dense(const uint32_t* __restri
     for (int c = 0; c < 3; ++c) Ac[(3 * (size_t)I + r) * N3 + 3 * (size_t)J + c] = acc[3 * r + c];
 }
 
-// r_c[I] = sum over the children (i, k) of I of w_ik r_i: one wavefront per coarse node, lane j takes
-// child j (a node has at most 49), then a fixed shuffle tree.
+// r_c[I] = sum over the children (i, k) of I of w_ik r_i at the node's position in the factor's ordering
+// (pos): one wavefront per coarse node, lane j takes child j (a node has at most 49), then a fixed shuffle tree.
 __global__ void __launch_bounds__(256) k_restrict(const uint32_t* __restrict__ child_ptr,
                                                   const uint32_t* __restrict__ child, const float* __restrict__ w,
-                                                  const float4* __restrict__ r, int Nc, float* __restrict__ rc,
-                                                  const PcgState* st, int guard) {
+                                                  const int32_t* __restrict__ pos, const float4* __restrict__ r,
+                                                  int Nc, float* __restrict__ rc, const PcgState* st, int guard) {
   if (guard == 0 && st->status != 0) return;
   if (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
   const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -750,41 +700,69 @@ __global__ void __launch_bounds__(256) k_restrict(const uint32_t* __restrict__ c
     a2 = a2 + __shfl_down(a2, o, 64);
   }
   if (lane == 0) {
-    rc[3 * I] = a0;
-    rc[3 * I + 1] = a1;
-    rc[3 * I + 2] = a2;
+    const int P = pos[I];
+    rc[P] = a0;
+    rc[P + 1] = a1;
+    rc[P + 2] = a2;
   }
 }
 
-// y = T x for a triangular T stored row-major in full (zeros outside the triangle): one wavefront per row
-// over the float4 groups that meet the triangle, lane sums then a fixed shuffle tree.
+// Row ranges of B = L^-1 in the factor's ordering: d2 and sep are the first positions of the second domain
+// and of the separator (both N3p without the split), r1 .. r3 the real unknowns of the three segments. A row
+// of B (LOWER) spans [0, R], or [d2, R] in the second domain; a row of B^T spans [C, N3p), without the second
+// domain [d2, sep) when C lies in the first; padding rows are skipped (their y stays zero).
+struct Split {
+  int d2, sep, r1, r2, r3;
+};
+__device__ inline bool split_real(const Split& p, int R) {
+  return R < p.d2 ? R < p.r1 : (R < p.sep ? R - p.d2 < p.r2 : R - p.sep < p.r3);
+}
+// [lo, hi) in units of `width` elements, without [hs, he)
 template <bool LOWER>
-__global__ void __launch_bounds__(256) k_tri_gemv(const float* __restrict__ T, const float* __restrict__ x,
-                                                  int N3, int N3p, float* __restrict__ y, const PcgState* st,
-                                                  int guard) {
+__device__ inline void split_range(const Split& p, int row, int N3p, int width, int* lo, int* hi, int* hs, int* he) {
+  if (LOWER) {
+    *lo = (row >= p.d2 && row < p.sep) ? p.d2 / width : 0;
+    *hi = row / width + 1;
+    *hs = *he = *hi;
+  } else {
+    *lo = row / width;
+    *hi = N3p / width;
+    *hs = row < p.d2 ? p.d2 / width : *hi;
+    *he = row < p.d2 ? p.sep / width : *hi;
+  }
+}
+
+// y = T x for a triangular T stored row-major in full (zeros outside the structure): one wavefront per row
+// over the float4 groups of its range, lane sums then a fixed shuffle tree.
+template <bool LOWER>
+__global__ void __launch_bounds__(256) k_tri_gemv(const float* __restrict__ T, const float* __restrict__ x, Split sp,
+                                                  int N3p, float* __restrict__ y, const PcgState* st, int guard) {
   if (guard == 0 && st->status != 0) return;
   if (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= N3) return;
+  if (row >= N3p || !split_real(sp, row)) return;
   const float4* m = reinterpret_cast<const float4*>(T + (size_t)row * N3p);
   const float4* v = reinterpret_cast<const float4*>(x);
-  const int lo = LOWER ? 0 : row / 4, hi = LOWER ? row / 4 + 1 : N3p / 4;
+  int lo, hi, hs, he;
+  split_range<LOWER>(sp, row, N3p, 4, &lo, &hi, &hs, &he);
+  const int count = hi - lo - (he - hs);
   float acc = 0.0f;
   // eight independent loads in flight per lane before the sequential accumulation
-  for (int j0 = lo + lane; j0 < hi; j0 += 8 * 64) {
+  for (int v0 = lane; v0 < count; v0 += 8 * 64) {
     float4 a[8], b[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int j = j0 + 64 * u;
-      if (j < hi) {
+      const int vv = v0 + 64 * u;
+      if (vv < count) {
+        const int j = lo + vv < hs ? lo + vv : lo + vv + (he - hs);
         a[u] = m[j];
         b[u] = v[j];
       }
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u)
-      if (j0 + 64 * u < hi) acc = acc + (a[u].x * b[u].x + a[u].y * b[u].y + a[u].z * b[u].z + a[u].w * b[u].w);
+      if (v0 + 64 * u < count) acc = acc + (a[u].x * b[u].x + a[u].y * b[u].y + a[u].z * b[u].z + a[u].w * b[u].w);
   }
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) acc = acc + __shfl_down(acc, s, 64);
@@ -796,24 +774,27 @@ __global__ void __launch_bounds__(256) k_tri_gemv(const float* __restrict__ T, c
 // diagonal of B, read from linv), so that the pair computes (S B~)^T (S B~) x.
 template <bool LOWER>
 __global__ void __launch_bounds__(256) k_tri_gemv_h(const _Float16* __restrict__ T, const float* __restrict__ x,
-                                                    const double* __restrict__ linv, int N3, int N3p,
+                                                    const double* __restrict__ linv, Split sp, int N3p,
                                                     float* __restrict__ y, const PcgState* st, int guard) {
   if (guard == 0 && st->status != 0) return;
   if (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= N3) return;
+  if (row >= N3p || !split_real(sp, row)) return;
   const uint4* m = reinterpret_cast<const uint4*>(T + (size_t)row * N3p);
   const float4* v = reinterpret_cast<const float4*>(x);
-  const int lo = LOWER ? 0 : row / 8, hi = LOWER ? row / 8 + 1 : N3p / 8;
+  int lo, hi, hs, he;
+  split_range<LOWER>(sp, row, N3p, 8, &lo, &hi, &hs, &he);
+  const int count = hi - lo - (he - hs);
   float acc = 0.0f;
-  for (int j0 = lo + lane; j0 < hi; j0 += 4 * 64) {
+  for (int v0 = lane; v0 < count; v0 += 4 * 64) {
     uint4 a[4];
     float4 b0[4], b1[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int j = j0 + 64 * u;
-      if (j < hi) {
+      const int vv = v0 + 64 * u;
+      if (vv < count) {
+        const int j = lo + vv < hs ? lo + vv : lo + vv + (he - hs);
         a[u] = m[j];
         b0[u] = v[2 * j];
         b1[u] = v[2 * j + 1];
@@ -821,7 +802,7 @@ __global__ void __launch_bounds__(256) k_tri_gemv_h(const _Float16* __restrict__
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (j0 + 64 * u < hi) {
+      if (v0 + 64 * u < count) {
         union {
           uint4 q;
           _Float16 h[8];
@@ -844,8 +825,9 @@ __global__ void __launch_bounds__(256) k_tri_gemv_h(const _Float16* __restrict__
   }
 }
 
-// z = D^-1 r + sum_k w_ik y_c[parent(i, k)], partial (r.z, r.r); mode 2 (setup) also sets p = z.
-__global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ parent, const float* __restrict__ w,
+// z = D^-1 r + sum_k w_ik y_c[parent(i, k)], partial (r.z, r.r); mode 2 (setup) also sets p = z. ppos holds the
+// position of each parent slot's node in the factor's ordering (-1: empty slot).
+__global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ ppos, const float* __restrict__ w,
                                                  const float* __restrict__ yc, const float* __restrict__ dinv,
                                                  const float4* __restrict__ r, float4* __restrict__ z,
                                                  float4* __restrict__ p, float* __restrict__ partials, int R,
@@ -863,12 +845,12 @@ __global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ par
     float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int I = parent[4 * row + k];
-      if (I >= 0) {
+      const int P = ppos[4 * row + k];
+      if (P >= 0) {
         const float ww = w[4 * row + k];
-        z0 = z0 + ww * yc[3 * I];
-        z1 = z1 + ww * yc[3 * I + 1];
-        z2 = z2 + ww * yc[3 * I + 2];
+        z0 = z0 + ww * yc[P];
+        z1 = z1 + ww * yc[P + 1];
+        z2 = z2 + ww * yc[P + 2];
       }
     }
     const float4 zz = make_float4(z0, z1, z2, 0.0f);
@@ -891,7 +873,8 @@ __global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ par
 // so the iterates are bit-identical to it.
 __global__ void __launch_bounds__(256) k_restrict_update(const uint32_t* __restrict__ child_ptr,
                                                          const uint32_t* __restrict__ child,
-                                                         const float* __restrict__ w, const float4* __restrict__ r,
+                                                         const float* __restrict__ w, const int32_t* __restrict__ pos,
+                                                         const float4* __restrict__ r,
                                                          const float4* __restrict__ q, int Nc, float* __restrict__ rc,
                                                          const float* __restrict__ partial_pq, int G, PcgState* st,
                                                          int it) {
@@ -928,13 +911,14 @@ __global__ void __launch_bounds__(256) k_restrict_update(const uint32_t* __restr
     a2 = a2 + __shfl_down(a2, o, 64);
   }
   if (lane == 0) {
-    rc[3 * I] = a0;
-    rc[3 * I + 1] = a1;
-    rc[3 * I + 2] = a2;
+    const int P = pos[I];
+    rc[P] = a0;
+    rc[P + 1] = a1;
+    rc[P + 2] = a2;
   }
 }
 
-__global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restrict__ parent, const float* __restrict__ w,
+__global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restrict__ ppos, const float* __restrict__ w,
                                                         const float* __restrict__ yc, const float* __restrict__ dinv,
                                                         const float4* __restrict__ p, const float4* __restrict__ q,
                                                         float4* __restrict__ x, float4* __restrict__ r,
@@ -968,12 +952,12 @@ __global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restric
     float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int I = parent[4 * row + k];
-      if (I >= 0) {
+      const int P = ppos[4 * row + k];
+      if (P >= 0) {
         const float ww = w[4 * row + k];
-        z0 = z0 + ww * yc[3 * I];
-        z1 = z1 + ww * yc[3 * I + 1];
-        z2 = z2 + ww * yc[3 * I + 2];
+        z0 = z0 + ww * yc[P];
+        z1 = z1 + ww * yc[P + 1];
+        z2 = z2 + ww * yc[P + 2];
       }
     }
     z[row] = make_float4(z0, z1, z2, 0.0f);
@@ -1019,7 +1003,10 @@ int two_level_create(Context& c, int columns) {
       if (row_of[(f * b) * n + f * a] >= 0) coarse_of[b * nc + a] = Nc++;
   t->Nc = Nc;
   t->N3 = 3 * Nc;
-  t->N3p = (t->N3 + 63) / 64 * 64;
+  std::vector<int> node_b(Nc);
+  for (int b = 0; b < nc; ++b)
+    for (int a = 0; a < nc; ++a)
+      if (coarse_of[b * nc + a] >= 0) node_b[coarse_of[b * nc + a]] = b;
   t->parent.assign(4 * (size_t)c.R, -1);
   t->bilinear.assign(4 * (size_t)c.R, 0.0f);
   for (int row = 0; row < c.R; ++row) {
@@ -1076,17 +1063,90 @@ int two_level_create(Context& c, int columns) {
     pair_ptr.push_back((uint32_t)terms.size());
   }
   t->npairs = (int)pair_key.size();
-  // band of the coarse matrix in 64-row tiles
-  t->nt = (t->N3 + TILE - 1) / TILE;
-  for (uint32_t key : pair_key) {
-    const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
-    if (I >= J) t->kb = std::max(t->kb, (3 * I + 2) / TILE - (3 * J) / TILE);
+  // The factor's ordering. The separator is as many coarse grid rows as the pairs reach across rows (two for
+  // the hinges' two-cell stencil); the first domain takes the rows above it in row order, the second the rows
+  // below it in reverse order. A layout whose separator or borders outgrow the tile structure falls back to
+  // one segment in row order.
+  int reach = 0;
+  for (uint32_t key : pair_key)
+    reach = std::max(reach, std::abs(node_b[key / (uint32_t)Nc] - node_b[key % (uint32_t)Nc]));
+  std::vector<int32_t> pos(Nc);
+  auto layout = [&](bool split) -> bool {
+    const int m0 = (nc - reach) / 2;
+    split = split && reach >= 1 && m0 >= 1 && nc - m0 - reach >= 1;
+    auto seg_of_row = [&](int b) { return !split ? 0 : (b < m0 ? 0 : (b < m0 + reach ? 2 : 1)); };
+    int count[3] = {0, 0, 0};
+    std::vector<int> idx(Nc);
+    for (int b = 0; b < nc; ++b)  // rows in order (the second domain is numbered from its far end below)
+      for (int a = 0; a < nc; ++a) {
+        const int I = coarse_of[b * nc + a];
+        if (I >= 0 && seg_of_row(b) != 1) idx[I] = count[seg_of_row(b)]++;
+      }
+    for (int b = nc - 1; b >= 0; --b)
+      for (int a = nc - 1; a >= 0; --a) {
+        const int I = coarse_of[b * nc + a];
+        if (I >= 0 && seg_of_row(b) == 1) idx[I] = count[1]++;
+      }
+    int tiles[3];
+    for (int g = 0; g < 3; ++g) {
+      t->real[g] = 3 * count[g];
+      tiles[g] = (t->real[g] + TILE - 1) / TILE;
+    }
+    t->seg[0] = 0;
+    t->seg[1] = tiles[0];
+    t->seg[2] = tiles[0] + tiles[1];
+    t->seg[3] = t->nt = tiles[0] + tiles[1] + tiles[2];
+    t->ts = tiles[2];
+    for (int I = 0; I < Nc; ++I) pos[I] = TILE * t->seg[seg_of_row(node_b[I])] + 3 * idx[I];
+    auto seg_of_tile = [&](int T) { return T < t->seg[1] ? 0 : (T < t->seg[2] ? 1 : 2); };
+    t->kb = 0;
+    t->bstart[0] = t->seg[1];
+    t->bstart[1] = t->seg[2];
+    for (uint32_t key : pair_key) {
+      const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
+      for (int q = 0; q < 9; ++q) {
+        const int R = pos[I] + q / 3, C = pos[J] + q % 3;
+        if (R < C) continue;
+        const int sR = seg_of_tile(R / TILE), sC = seg_of_tile(C / TILE);
+        if (sR == sC) t->kb = std::max(t->kb, R / TILE - C / TILE);
+        else if (sR == 2) t->bstart[sC] = std::min(t->bstart[sC], C / TILE);
+        else return false;  // the domains touch: no separator
+      }
+    }
+    if (t->ts > MAX_TS || (t->ts > 0 && t->ts - 1 > t->kb)) return false;
+    for (int e = 0; e < 2; ++e)
+      if (t->seg[e + 1] - t->bstart[e] > t->kb + 1) return false;
+    return true;
+  };
+  if (!layout(c.rules.coarse_split != 0) && !layout(false)) {
+    set_error("two-level preconditioner: no tile layout for the coarse operator");
+    delete t;
+    return DRAPE_E_INVALID;
   }
   if (t->kb > MAX_KB) {
     set_error("two-level preconditioner: the coarse grid is too large for the banded coarse solve");
     delete t;
     return DRAPE_E_INVALID;
   }
+  t->N3p = TILE * t->nt;
+  // the factor's order of the real unknowns, and the entries of B's structure on or below its diagonal (a row
+  // meets the real unknowns before it in its own segment, and a separator row those of both domains too)
+  t->order.assign(t->N3, 0);
+  {
+    std::vector<int32_t> at(t->N3p, -1);
+    for (int I = 0; I < Nc; ++I)
+      for (int q = 0; q < 3; ++q) at[pos[I] + q] = 3 * I + q;
+    int n = 0;
+    for (int P = 0; P < t->N3p; ++P)
+      if (at[P] >= 0) t->order[n++] = at[P];
+    double entries = 0.0;
+    for (int g = 0; g < 3; ++g)
+      for (int i = 0; i < t->real[g]; ++i) entries += i + 1 + (g == 2 ? t->real[0] + t->real[1] : 0);
+    t->tri_entries = entries;
+  }
+  std::vector<int32_t> ppos(4 * (size_t)c.R, -1);
+  for (size_t i = 0; i < ppos.size(); ++i)
+    if (t->parent[i] >= 0) ppos[i] = pos[t->parent[i]];
   hipStream_t s = c.stream;
   auto fail = [&](hipError_t e) {
     set_error(std::string("two-level setup: ") + hipGetErrorString(e));
@@ -1105,17 +1165,22 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_pair_ptr, pair_ptr.size());
   TL_ALLOC(t->d_pair_key, pair_key.size());
   TL_ALLOC(t->d_pair_terms, terms.size());
+  TL_ALLOC(t->d_pos, pos.size());
+  TL_ALLOC(t->d_ppos, ppos.size());
   TL_ALLOC(t->d_band, (size_t)t->nt * (t->kb + 1) * TILE * TILE);
   TL_ALLOC(t->d_linv, (size_t)t->nt * TILE * TILE);
   TL_ALLOC(t->d_lpanel, (size_t)t->nt * std::max(t->kb, 1) * TILE * TILE);
-  TL_ALLOC(t->d_xring, (size_t)(t->kb + 1) * TILE * t->N3p);
+  TL_ALLOC(t->d_border, (size_t)2 * (t->kb + 1) * std::max(t->ts, 1) * TILE * TILE);
+  TL_ALLOC(t->d_lborder, (size_t)2 * (t->kb + 1) * std::max(t->ts, 1) * TILE * TILE);
+  TL_ALLOC(t->d_sacc, (size_t)std::max(t->ts * t->ts, 1) * TILE * TILE);
+  TL_ALLOC(t->d_xring, (size_t)3 * (t->kb + 1) * TILE * t->N3p);
   if ((e = hipStreamCreateWithFlags(&t->inv_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
   if ((e = hipEventCreateWithFlags(&t->ev_panel, hipEventDisableTiming)) != hipSuccess) return fail(e);
   if ((e = hipEventCreateWithFlags(&t->ev_inverse, hipEventDisableTiming)) != hipSuccess) return fail(e);
-  TL_ALLOC(t->d_B, (size_t)t->N3 * t->N3p);
-  TL_ALLOC(t->d_BT, (size_t)t->N3 * t->N3p);
-  TL_ALLOC(t->d_Bh, (size_t)t->N3 * t->N3p);
-  TL_ALLOC(t->d_BTh, (size_t)t->N3 * t->N3p);
+  TL_ALLOC(t->d_B, (size_t)t->N3p * t->N3p);
+  TL_ALLOC(t->d_BT, (size_t)t->N3p * t->N3p);
+  TL_ALLOC(t->d_Bh, (size_t)t->N3p * t->N3p);
+  TL_ALLOC(t->d_BTh, (size_t)t->N3p * t->N3p);
   TL_ALLOC(t->d_t, (size_t)t->N3p);
   TL_ALLOC(t->d_rc, (size_t)t->N3p);
   TL_ALLOC(t->d_yc, (size_t)t->N3p);
@@ -1126,6 +1191,8 @@ int two_level_create(Context& c, int columns) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
   };
   if ((e = up(t->d_parent, t->parent.data(), t->parent.size() * 4)) != hipSuccess ||
+      (e = up(t->d_pos, pos.data(), pos.size() * 4)) != hipSuccess ||
+      (e = up(t->d_ppos, ppos.data(), ppos.size() * 4)) != hipSuccess ||
       (e = up(t->d_w, t->bilinear.data(), t->bilinear.size() * 4)) != hipSuccess ||
       (e = up(t->d_wbil, t->bilinear.data(), t->bilinear.size() * 4)) != hipSuccess ||
       (e = up(t->d_fxy, fxy.data(), fxy.size() * sizeof(float2))) != hipSuccess ||
@@ -1137,10 +1204,10 @@ int two_level_create(Context& c, int columns) {
       (e = hipMemsetAsync(t->d_rc, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_yc, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_t, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
-      (e = hipMemsetAsync(t->d_B, 0, (size_t)t->N3 * t->N3p * 4, s)) != hipSuccess ||
-      (e = hipMemsetAsync(t->d_BT, 0, (size_t)t->N3 * t->N3p * 4, s)) != hipSuccess ||
-      (e = hipMemsetAsync(t->d_Bh, 0, (size_t)t->N3 * t->N3p * 2, s)) != hipSuccess ||
-      (e = hipMemsetAsync(t->d_BTh, 0, (size_t)t->N3 * t->N3p * 2, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_B, 0, (size_t)t->N3p * t->N3p * 4, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_BT, 0, (size_t)t->N3p * t->N3p * 4, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_Bh, 0, (size_t)t->N3p * t->N3p * 2, s)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_BTh, 0, (size_t)t->N3p * t->N3p * 2, s)) != hipSuccess ||
       (e = hipStreamSynchronize(s)) != hipSuccess)
     return fail(e);
   if (c.two_level) two_level_destroy(c.two_level);
@@ -1227,10 +1294,30 @@ GnnArgs two_level_gnn_args(Context& c) {
 
 static int run_model(Context& c) { return gnn_infer(two_level_gnn_args(c)); }
 
+static Tiles tiles_of(const TwoLevel* t) {
+  Tiles v;
+  v.band = t->d_band;
+  v.lpanel = t->d_lpanel;
+  v.border = t->d_border;
+  v.lborder = t->d_lborder;
+  v.sacc = t->d_sacc;
+  v.kb = t->kb;
+  v.nt = t->nt;
+  v.ts = t->ts;
+  for (int i = 0; i < 4; ++i) v.seg[i] = t->seg[i];
+  for (int i = 0; i < 2; ++i) v.bstart[i] = t->bstart[i];
+  for (int i = 0; i < 3; ++i) v.real[i] = t->real[i];
+  return v;
+}
+
+static Split split_of(const TwoLevel* t) {
+  return {TILE * t->seg[1], TILE * t->seg[2], t->real[0], t->real[1], t->real[2]};
+}
+
 // The FP16 copies are formed and applied under rules.coarse_half = 1, and under 2 (auto) once the FP32 pair
 // B, B^T outgrows half of the 256 MB Infinity Cache.
 static bool use_half(const Context& c, const TwoLevel* t) {
-  const bool big = (size_t)8 * t->N3 * t->N3p > ((size_t)128 << 20);
+  const bool big = 8.0 * t->tri_entries > (double)((size_t)128 << 20);
   return c.rules.coarse_half == 1 || (c.rules.coarse_half == 2 && big);
 }
 
@@ -1248,7 +1335,7 @@ static int swap_saved(Context& c, bool keep) {
   TwoLevel* t = c.two_level;
   TwoLevel::Saved& v = t->saved;
   if (!v.B) {
-    const size_t nb = (size_t)t->N3 * t->N3p;
+    const size_t nb = (size_t)t->N3p * t->N3p;
     HIP_TRY(hipMalloc((void**)&v.linv, (size_t)t->nt * TILE * TILE * sizeof(double)));
     HIP_TRY(hipMalloc((void**)&v.B, nb * sizeof(float)));
     HIP_TRY(hipMalloc((void**)&v.BT, nb * sizeof(float)));
@@ -1361,7 +1448,7 @@ static int factorise_async(Context& c) {
   TwoLevel* t = c.two_level;
   TwoLevel::Pending& q = t->pending;
   if (!q.side) {
-    const size_t nb = (size_t)t->N3 * t->N3p;
+    const size_t nb = (size_t)t->N3p * t->N3p;
     HIP_TRY(hipStreamCreateWithFlags(&q.side, hipStreamNonBlocking));
     for (hipEvent_t* e : {&q.ready, &q.read, &q.done}) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     for (hipEvent_t* e : {&q.began, &q.ended}) HIP_TRY(hipEventCreate(e));
@@ -1499,26 +1586,48 @@ static int factorise(Context& c, bool model, bool* ok) {
 static int factorise_chain(Context& c, hipStream_t s, const float* w, double* linv, float* B, float* BT,
                            _Float16* Bh, _Float16* BTh, bool half, hipEvent_t after_product) {
   TwoLevel* t = c.two_level;
-  HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * TILE * TILE * sizeof(double), s));
+  const Tiles tl = tiles_of(t);
+  const size_t tile_bytes = (size_t)TILE * TILE * sizeof(double);
+  HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * tile_bytes, s));
+  if (t->ts > 0) {
+    HIP_TRY(hipMemsetAsync(t->d_border, 0, (size_t)2 * (t->kb + 1) * t->ts * tile_bytes, s));
+    HIP_TRY(hipMemsetAsync(t->d_sacc, 0, (size_t)t->ts * t->ts * tile_bytes, s));
+  }
   HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
   k_galerkin_band<<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, w,
-                                                        c.sell_val, c.block_dst, t->npairs, t->Nc, t->kb, t->d_band);
+                                                        c.sell_val, c.block_dst, t->npairs, t->Nc, t->d_pos, tl);
   if (after_product) HIP_TRY(hipEventRecord(after_product, s));
-  k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(t->d_band, t->kb, t->nt, t->N3, 1.0 + c.rules.coarse_shift);
-  const int updates = t->kb * (t->kb + 1) / 2;
-  // The panel chain is latency-bound on one workgroup at a time; tile row k of B = L^-1 runs on the inverse
-  // stream as soon as panel k is done, beside the chain's later tile columns, and s waits for the last row.
-  for (int k = 0; k < t->nt; ++k) {
-    k_chol_panel<<<1, 256, 0, s>>>(t->d_band, linv, k, t->kb, t->d_info);
+  k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(tl, 1.0 + c.rules.coarse_shift);
+  const int slots = t->kb + t->ts;
+  const int pairs = slots * (slots + 1) / 2;
+  auto inverse_columns = [&](int I) { return (TILE * (I + 1) - (I >= t->seg[1] && I < t->seg[2] ? TILE * t->seg[1] : 0)) / INV_COLS; };
+  // One step of the chain: the panels of one or two tile columns (one per domain), then tile rows of B = L^-1
+  // on the inverse stream as soon as their panels are done, beside the chain's later columns, then the
+  // columns' TRSM and trailing update (none after the last tile column).
+  auto column = [&](int k1, int k2) -> int {
+    const int n = k2 >= 0 ? 2 : 1;
+    const int2 cols = make_int2(k1, k2);
+    k_chol_panel<<<n, 256, 0, s>>>(tl, linv, cols, t->d_info);
     HIP_TRY(hipEventRecord(t->ev_panel, s));
     HIP_TRY(hipStreamWaitEvent(t->inv_stream, t->ev_panel, 0));
-    k_inverse_row<<<(k + 1) * (TILE / INV_COLS), 256, 0, t->inv_stream>>>(
-        t->d_lpanel, linv, k, t->kb, t->N3, t->N3p, t->d_xring, B, BT, half ? Bh : nullptr, half ? BTh : nullptr,
-        t->d_info);
-    if (k + 1 < t->nt && updates > 0) {
-      k_chol_trsm<<<4 * t->kb, 256, 0, s>>>(t->d_band, t->d_lpanel, linv, k, t->nt, t->kb, t->d_info);
-      k_chol_update<<<4 * updates, 256, 0, s>>>(t->d_band, t->d_lpanel, k, t->nt, t->kb, t->d_info);
+    const int w1 = inverse_columns(k1), w2 = k2 >= 0 ? inverse_columns(k2) : 0;
+    k_inverse_row<<<w1 + w2, 256, 0, t->inv_stream>>>(tl, linv, cols, w1, t->N3p, t->d_xring, B, BT,
+                                                     half ? Bh : nullptr, half ? BTh : nullptr, t->d_info);
+    if (k1 + 1 < t->nt && slots > 0) {
+      k_chol_trsm<<<4 * slots * n, 256, 0, s>>>(tl, linv, cols, t->d_info);
+      k_chol_update<<<4 * pairs * n, 256, 0, s>>>(tl, cols, t->d_info);
     }
+    return DRAPE_OK;
+  };
+  // the two domains side by side, then the separator
+  const int t1 = t->seg[1], t2 = t->seg[2] - t->seg[1];
+  for (int k = 0; k < std::max(t1, t2); ++k) {
+    const int k1 = k < t1 ? k : -1, k2 = k < t2 ? t->seg[1] + k : -1;
+    TRY_RC(k1 >= 0 ? column(k1, k2) : column(k2, -1));
+  }
+  if (t->ts > 0) {
+    k_add_separator<<<16 * t->ts * t->ts, 256, 0, s>>>(tl);
+    for (int k = t->seg[2]; k < t->nt; ++k) TRY_RC(column(k, -1));
   }
   HIP_TRY(hipEventRecord(t->ev_inverse, t->inv_stream));
   HIP_TRY(hipStreamWaitEvent(s, t->ev_inverse, 0));
@@ -1543,13 +1652,27 @@ int two_level_galerkin(Context& c, double* out) {
   return DRAPE_OK;
 }
 
-// B = L^-1 of the current factorisation (FP32, row-major [3Nc x 3Nc]) into host memory.
+// B = L^-1 of the current factorisation (FP32, row-major [3Nc x 3Nc]) between the real unknowns, in the
+// factor's ordering (two_level_ordering), into host memory.
 int two_level_inverse_factor(Context& c, float* out) {
   TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
-  HIP_TRY(hipMemcpy2DAsync(out, (size_t)t->N3 * sizeof(float), t->d_B, (size_t)t->N3p * sizeof(float),
-                           (size_t)t->N3 * sizeof(float), t->N3, hipMemcpyDeviceToHost, c.stream));
+  std::vector<float> full((size_t)t->N3p * t->N3p);
+  HIP_TRY(hipMemcpyAsync(full.data(), t->d_B, full.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
+  std::vector<int> real_pos;
+  for (int g = 0; g < 3; ++g)
+    for (int i = 0; i < t->real[g]; ++i) real_pos.push_back(TILE * t->seg[g] + i);
+  const size_t n = real_pos.size();
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < n; ++j) out[i * n + j] = full[(size_t)real_pos[i] * t->N3p + real_pos[j]];
+  return DRAPE_OK;
+}
+
+// The factor's ordering: order[i] = 3 I + q, the natural coarse unknown at its i-th real position.
+int two_level_ordering(Context& c, int32_t* order) {
+  const TwoLevel* t = c.two_level;
+  std::copy(t->order.begin(), t->order.end(), order);
   return DRAPE_OK;
 }
 
@@ -1571,13 +1694,14 @@ void two_level_observe(Context& c, int iterations, double relative_residual) {
 // factorisation (and not `fp32`), else from FP32 B.
 static void coarse_solve(Context& c, const TwoLevel* t, int guard, bool fp32) {
   hipStream_t s = c.stream;
-  const int blocks = (t->N3 + 3) / 4;
+  const int blocks = t->N3p / 4;
+  const Split sp = split_of(t);
   if (t->half_valid && use_half(c, t) && !fp32) {
-    k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, t->N3, t->N3p, t->d_t, c.state, guard);
-    k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, t->N3, t->N3p, t->d_yc, c.state, guard);
+    k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, sp, t->N3p, t->d_t, c.state, guard);
+    k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, sp, t->N3p, t->d_yc, c.state, guard);
   } else {
-    k_tri_gemv<true><<<blocks, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, guard);
-    k_tri_gemv<false><<<blocks, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, guard);
+    k_tri_gemv<true><<<blocks, 256, 0, s>>>(t->d_B, t->d_rc, sp, t->N3p, t->d_t, c.state, guard);
+    k_tri_gemv<false><<<blocks, 256, 0, s>>>(t->d_BT, t->d_t, sp, t->N3p, t->d_yc, c.state, guard);
   }
 }
 
@@ -1588,17 +1712,17 @@ int two_level_gemv_timing(Context& c, bool lower, int launches, double* avg_ms, 
   TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
-  const int blocks = (t->N3 + 3) / 4;
+  const int blocks = t->N3p / 4;
+  const Split sp = split_of(t);
   const bool half = t->half_valid && use_half(c, t);
-  const double tri = (double)t->N3 * (t->N3 + 1) / 2.0;
-  *bytes = tri * (half ? 2.0 : 4.0) + 8.0 * t->N3;
+  *bytes = t->tri_entries * (half ? 2.0 : 4.0) + 8.0 * t->N3;
   auto launch = [&]() {
     if (half) {
-      if (lower) k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, t->N3, t->N3p, t->d_t, c.state, 2);
-      else k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, t->N3, t->N3p, t->d_yc, c.state, 2);
+      if (lower) k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, sp, t->N3p, t->d_t, c.state, 2);
+      else k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, sp, t->N3p, t->d_yc, c.state, 2);
     } else {
-      if (lower) k_tri_gemv<true><<<blocks, 256, 0, s>>>(t->d_B, t->d_rc, t->N3, t->N3p, t->d_t, c.state, 2);
-      else k_tri_gemv<false><<<blocks, 256, 0, s>>>(t->d_BT, t->d_t, t->N3, t->N3p, t->d_yc, c.state, 2);
+      if (lower) k_tri_gemv<true><<<blocks, 256, 0, s>>>(t->d_B, t->d_rc, sp, t->N3p, t->d_t, c.state, 2);
+      else k_tri_gemv<false><<<blocks, 256, 0, s>>>(t->d_BT, t->d_t, sp, t->N3p, t->d_yc, c.state, 2);
     }
   };
   launch();
@@ -1617,9 +1741,10 @@ int two_level_gemv_timing(Context& c, bool lower, int launches, double* avg_ms, 
 int two_level_apply(Context& c, float* partials, int guard) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
-  k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, c.r, t->Nc, t->d_rc, c.state, guard);
+  k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_pos, c.r, t->Nc, t->d_rc, c.state,
+                                             guard);
   coarse_solve(c, t, guard, false);
-  k_prolong<<<c.G, 256, 0, s>>>(t->d_parent, t->d_w, t->d_yc, c.dinv, c.r, c.z, c.p, partials, c.R, c.state, guard);
+  k_prolong<<<c.G, 256, 0, s>>>(t->d_ppos, t->d_w, t->d_yc, c.dinv, c.r, c.z, c.p, partials, c.R, c.state, guard);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }
@@ -1627,10 +1752,10 @@ int two_level_apply(Context& c, float* partials, int guard) {
 int two_level_iterate(Context& c, const float* partial_pq, const float4* p, float* partial_rz, int it) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
-  k_restrict_update<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, c.r, c.q, t->Nc, t->d_rc,
-                                                    partial_pq, c.G, c.state, it);
+  k_restrict_update<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_pos, c.r, c.q, t->Nc,
+                                                    t->d_rc, partial_pq, c.G, c.state, it);
   coarse_solve(c, t, 0, false);
-  k_prolong_update<<<c.G, 256, 0, s>>>(t->d_parent, t->d_w, t->d_yc, c.dinv, p, c.q, c.x, c.r, c.z, partial_rz, c.R,
+  k_prolong_update<<<c.G, 256, 0, s>>>(t->d_ppos, t->d_w, t->d_yc, c.dinv, p, c.q, c.x, c.r, c.z, partial_rz, c.R,
                                        partial_pq, c.G, c.state, it);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
@@ -1638,13 +1763,14 @@ int two_level_iterate(Context& c, const float* partial_pq, const float4* p, floa
 
 CoarseView two_level_view(const Context& c) {
   const TwoLevel* t = c.two_level;
-  return {t->d_parent, t->d_w, t->d_yc};
+  return {t->d_ppos, t->d_w, t->d_yc};
 }
 
 int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
-  k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, r, t->Nc, t->d_rc, c.state, guard);
+  k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_pos, r, t->Nc, t->d_rc, c.state,
+                                             guard);
   coarse_solve(c, t, guard, fp32);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;

@@ -131,6 +131,10 @@ typedef struct {
   int32_t coarse_factor;   /* two-level: the coarse vertices are every coarse_factor-th grid vertex each way
                               ((grid_columns - 1) divisible by it); 0 (default) picks the smallest power of two
                               from 4 up that keeps the coarse grid at most 33 vertices a side (n129: 4, n257: 8) */
+  int32_t coarse_split;    /* two-level: 1 (default) orders the coarse unknowns for the factorisation as two
+                              domains (the coarse grid rows above and below a middle separator as wide as the
+                              operator's coupling) and the separator, so that the domains factor side by side
+                              and the inverse factor is zero between them; 0 keeps row order (one band) */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0
@@ -390,8 +394,14 @@ int drape_self_contact_pairs(drape_ctx* ctx, int32_t* n_pt, int32_t* n_ee, int32
 int drape_two_level_coarse(drape_ctx* ctx, double* coarse);
 
 /* B = L^-1 (A_c = L L^T) of the current coarse factorisation as FP32, row-major [3Nc x 3Nc] (zero above the
- * diagonal), for checking; the coarse solve applies B^T B. */
+ * diagonal), in the factor's ordering of the coarse unknowns (drape_two_level_ordering), for checking; the
+ * coarse solve applies B^T B. No reference counterpart (the two-level preconditioner is new). */
 int drape_two_level_inverse_factor(drape_ctx* ctx, float* factor);
+
+/* The factor's ordering of the coarse unknowns: order[i] = 3 I + q (coarse node I, component q) at position i
+ * of drape_two_level_inverse_factor ([3Nc]), so that B^T B = (Q A_c Q^T)^-1 with Q the permutation that picks
+ * the rows order[0], order[1], ... No reference counterpart. */
+int drape_two_level_ordering(drape_ctx* ctx, int32_t* order);
 
 /* The FP64 iterate [3R] of the last drape_pcg under rules.refine_fp64 (drape_pcg's FP32 solution rounds it).
  * No reference counterpart (FP64 refinement is new). */
