@@ -292,37 +292,178 @@ __device__ inline void wave_order() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "
 // 1/sqrt(x) to double precision: the hardware estimate refined by two Newton steps.
 __device__ inline double rsqrt_f64(double x) {
   double y = __builtin_amdgcn_rsq(x);
-  y = y * (1.5 - 0.5 * x * y * y);
-  y = y * (1.5 - 0.5 * x * y * y);
+  // y += y (1 - x y^2) / 2, twice, with fused products
+  y = fma(0.5 * y, fma(-x, y * y, 1.0), y);
+  y = fma(0.5 * y, fma(-x, y * y, 1.0), y);
   return y;
 }
 
-// Diagonal tile of tile column k (cols.x, or cols.y for the second workgroup), blocked by 16-column panels. Wavefront 0 factors each panel four columns
-// at a time: the 4 x 4 diagonal block is broadcast through LDS and factored redundantly by every lane, each
-// lane (= row) solves its four factor entries and broadcasts them, and the lanes apply the rank-4 update to
-// the rest of their panel row. All four wavefronts then apply the panel's rank-16 update to the trailing
-// lower triangle. The tile's inverse follows blockwise: the four 16 x 16 diagonal blocks in parallel, then
-// the off-diagonal blocks by distance from the diagonal, X_ab = -X_aa sum_c L_ac X_cb. A pivot that is not
-// positive (or not finite) records 64 k + column + 1 in *info and stops every later kernel.
-__global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict__ linv, int2 cols, int* info) {
-  __shared__ double Ls[TILE * LSTR];
-  __shared__ double Xs[TILE * LSTR];
-  __shared__ double Ys[3 * 16 * 17];
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// A 16 x 16 block of X Y^T over columns k0 .. k0 + K - 1 of X and Y (16 rows each, in LDS with stride LSTR),
+// on one wavefront: v_mfma_f64_16x16x4f64 takes A[i][k] from lane 16 k + i and B[k][j] from lane 16 k + j, and
+// returns D[(lane >> 4) + 4 v][lane & 15] in element v.
+template <int K>
+__device__ inline v4d mfma_xyt(const double* X, const double* Y, int k0) {
+  const int l = threadIdx.x & 63;
+  const double* xr = X + (l & 15) * LSTR + k0 + (l >> 4);
+  const double* yr = Y + (l & 15) * LSTR + k0 + (l >> 4);
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < K; k += 4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[k], yr[k], acc, 0, 0, 0);
+  return acc;
+}
+
+// The value of the lane given by the quad permutation CTRL (DPP, within groups of four lanes).
+template <int CTRL>
+__device__ inline double quad_perm(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false));
+}
+
+// X <- X L^-T in place for a 64-row tile X and the lower factor tile L (both in LDS, stride LSTR; rl[j] =
+// 1 / L_jj): forward substitution by the first 256 threads, four per row. Thread s of a row keeps x_m for
+// m = s (mod 4), zero until formed, so every step sums the same fixed terms without a branch (the terms of
+// unformed x are exact zeros); the four partial sums of each x_j meet by two DPP exchanges inside the quad
+// (every lane gets the same, commutative sums), so the result does not depend on scheduling.
+__device__ inline void solve_lt(const double* L, const double* rl, double* X) {
+  const int tid = threadIdx.x;
+  if (tid >= 256) return;
+  const int r = tid >> 2, s = tid & 3;
+  double own[16], cur[16], nxt[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) own[i] = cur[i] = nxt[i] = 0.0;
+  const double* xr = X + r * LSTR;
+  double xv = xr[0], rv = rl[0], xv_n = 0.0, rv_n = 0.0;
+  // the loads of step j + 1 are issued before step j's dependent chain
+#pragma unroll
+  for (int j = 0; j < TILE; ++j) {
+    if (j + 1 < TILE) {
+#pragma unroll
+      for (int i = 0; 4 * i < j + 1; ++i) nxt[i] = L[(j + 1) * LSTR + 4 * i + s];
+      xv_n = xr[j + 1];
+      rv_n = rl[j + 1];
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; 4 * i < j; ++i) acc[i & 3] = fma(own[i], cur[i], acc[i & 3]);
+    double part = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    part += quad_perm<0xB1>(part);  // lanes [1, 0, 3, 2]
+    part += quad_perm<0x4E>(part);  // lanes [2, 3, 0, 1]
+    double xj = (xv - part) * rv;
+    asm volatile("" : "+v"(xj));  // formed by every lane, so that the select below needs no branch
+    own[j >> 2] = (j & 3) == s ? xj : own[j >> 2];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) cur[i] = nxt[i];
+    xv = xv_n;
+    rv = rv_n;
+  }
+  // the row's reads are done: its four threads share one wavefront
+#pragma unroll
+  for (int i = 0; i < 16; ++i) X[r * LSTR + 4 * i + s] = own[i];
+}
+
+// One step of the factorisation chain for up to two tile columns (one per domain), in one launch:
+//  - a panel workgroup per column c = st.c[e]: with p = c - 1 in c's segment, it forms the factor tile
+//    L(c, p) by solving X L_pp^T = A(c, p) (A(c, p) already holds every update from the columns before p),
+//    writes it, subtracts L(c, p) L(c, p)^T from A(c, c) (FP64 MFMA), and factors A(c, c) by 16-column panels:
+//    wavefront 0 factors each panel four columns at a time (the 4 x 4 diagonal block broadcast through LDS and
+//    factored redundantly by every lane, each lane (= row) solving its four factor entries and broadcasting
+//    them for the rank-4 update of the rest of its panel row), then the panel's rank-16 update of the trailing
+//    lower triangle runs on MFMA. A pivot that is not positive (or not finite) records 64 c + column + 1 in
+//    *info and stops every later kernel;
+//  - worker workgroups for column p = st.p[e] (the previous column of a segment, or its last one after its
+//    panel): the factor tiles L(I, p) of the below tiles I other than the panel's; one workgroup per pair of
+//    below tiles (I >= J) other than the panel's (c, c), forming L(I, p) and L(J, p) itself and subtracting
+//    L(I, p) L(J, p)^T from tile (I, J) (the second domain's separator block goes to the separate Schur
+//    block); and one that inverts L_pp blockwise for the inverse sweep (the four 16 x 16 diagonal blocks in
+//    parallel, then the off-diagonal blocks by distance from the diagonal, X_ab = -X_aa sum_c L_ac X_cb).
+// Every tile a workgroup writes is read by no other workgroup of the launch.
+struct Step {
+  int panels, workers;
+  int c[2], p[2];
+};
+constexpr int STEP_THREADS = 512;
+constexpr int STEP_WAVES = STEP_THREADS / 64;
+
+template <int ROWS>
+__device__ inline void tile_to_lds_step(const double* __restrict__ src, double* dst) {
+  constexpr int PER = ROWS * TILE / STEP_THREADS;
+  double v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) v[i] = src[threadIdx.x + STEP_THREADS * i];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + STEP_THREADS * i;
+    dst[(e >> 6) * LSTR + (e & 63)] = v[i];
+  }
+}
+
+// Up to three 64 x 64 tiles into LDS with every load issued before the first store (src2 may be null).
+__device__ inline void tiles_to_lds_step(const double* __restrict__ src0, double* dst0, const double* __restrict__ src1,
+                                         double* dst1, const double* __restrict__ src2, double* dst2) {
+  constexpr int PER = TILE * TILE / STEP_THREADS;
+  double v0[PER], v1[PER], v2[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    v0[i] = src0[threadIdx.x + STEP_THREADS * i];
+    v1[i] = src1[threadIdx.x + STEP_THREADS * i];
+    if (src2) v2[i] = src2[threadIdx.x + STEP_THREADS * i];
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + STEP_THREADS * i, o = (e >> 6) * LSTR + (e & 63);
+    dst0[o] = v0[i];
+    dst1[o] = v1[i];
+    if (src2) dst2[o] = v2[i];
+  }
+}
+
+__device__ inline void lds_to_tile_step(const double* src, double* __restrict__ dst) {
+  for (int e = threadIdx.x; e < TILE * TILE; e += STEP_THREADS) dst[e] = src[(e >> 6) * LSTR + (e & 63)];
+}
+
+__global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __restrict__ linv, Step st, int* info) {
+  __shared__ double sa[TILE * LSTR];
+  __shared__ double sb[TILE * LSTR];
+  __shared__ double sc[TILE * LSTR];
   __shared__ double blk[16];
   __shared__ double colbuf[4 * TILE];
   __shared__ double rdiag[TILE];
   __shared__ int fail;
   if (*info != 0) return;
-  const int k = blockIdx.x ? cols.y : cols.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double* tile = band_tile(t, k, 0);
-  tile_to_lds<TILE>(tile, Ls);
-  for (int e = tid; e < TILE * TILE; e += 256) Xs[(e >> 6) * LSTR + (e & 63)] = 0.0;
-  if (tid == 0) fail = 0;
-  __syncthreads();
-  for (int b = 0; b < 4; ++b) {
-    const int c0 = 16 * b;
-    if (wave == 0) {
+  const int slots = t.kb + t.ts, per = 1 + slots + slots * (slots + 1) / 2;
+  const int bid = blockIdx.x;
+  if (bid < st.panels) {
+    const int k = st.c[bid];
+    const int prev = k - 1 >= t.seg[seg_of(t, k)] ? k - 1 : -1;
+    double* tile = band_tile(t, k, 0);
+    double* Ls = sc;
+    if (prev >= 0) {
+      tiles_to_lds_step(band_tile(t, prev, 0), sa, a_tile(t, k, prev), sb, tile, sc);
+      __syncthreads();
+      if (tid < TILE) rdiag[tid] = 1.0 / sa[tid * LSTR + tid];
+      __syncthreads();
+      solve_lt(sa, rdiag, sb);
+      __syncthreads();
+      // A(k, k) -= L(k, prev) L(k, prev)^T on the ten blocks on or below the diagonal
+      for (int q = wave; q < 10; q += STEP_WAVES) {
+        const int bi = q < 1 ? 0 : q < 3 ? 1 : q < 6 ? 2 : 3;
+        const int bj = q - bi * (bi + 1) / 2;
+        const v4d acc = mfma_xyt<TILE>(sb + 16 * bi * LSTR, sb + 16 * bj * LSTR, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ls[(16 * bi + (lane >> 4) + 4 * v) * LSTR + 16 * bj + (lane & 15)] -= acc[v];
+      }
+    } else {
+      tile_to_lds_step<TILE>(tile, sc);
+    }
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    for (int b = 0; b < 4; ++b) {
+      const int c0 = 16 * b;
+      if (wave == 0) {
       const int row = c0 + lane;
       const bool act = row < TILE;
       double p[16];
@@ -351,7 +492,7 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
           for (int v = 0; v < 4; ++v) {
             double piv = d[v][v];
 #pragma unroll
-            for (int m = 0; m < v; ++m) piv -= l[v][m] * l[v][m];
+            for (int m = 0; m < v; ++m) piv = fma(-l[v][m], l[v][m], piv);
             if (!(piv > 0.0) || !(piv < INFINITY)) {
               if (bad == 0) bad = c0 + q + v + 1;
               piv = 1.0;
@@ -362,7 +503,7 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
             for (int u = v + 1; u < 4; ++u) {
               double a = d[u][v];
 #pragma unroll
-              for (int m = 0; m < v; ++m) a -= l[u][m] * l[v][m];
+              for (int m = 0; m < v; ++m) a = fma(-l[u][m], l[v][m], a);
               l[u][v] = a * rd[v];
             }
           }
@@ -372,7 +513,7 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
           for (int v = 0; v < 4; ++v) {
             double a = p[q + v];
 #pragma unroll
-            for (int m = 0; m < v; ++m) a -= x[m] * l[v][m];
+            for (int m = 0; m < v; ++m) a = fma(-x[m], l[v][m], a);
             x[v] = a * rd[v];
           }
 #pragma unroll
@@ -395,7 +536,7 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
 #pragma unroll
           for (int j = q + 4; j < 16; ++j) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) p[j] -= x[v] * colbuf[v * TILE + j];
+            for (int v = 0; v < 4; ++v) p[j] = fma(-x[v], colbuf[v * TILE + j], p[j]);
           }
           wave_order();
         }
@@ -406,48 +547,51 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
       }
       if (lane == 0 && bad != 0) fail = bad;
     }
-    __syncthreads();
-    if (fail != 0) {
-      if (tid == 0) *info = TILE * k + fail;
-      return;
-    }
-    // trailing update of rows and columns c0 + 16 .. 63: one 4 x 4 block of the lower triangle per thread
-    const int t4 = (TILE - c0 - 16) / 4;
-    if (tid < t4 * t4) {
-      const int bi = tid / t4, bj = tid % t4;
-      if (bj <= bi) {
-        const int i0 = c0 + 16 + 4 * bi, j0 = c0 + 16 + 4 * bj;
-        double o[4][4] = {};
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-          double li[4], lj[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            li[u] = Ls[(i0 + u) * LSTR + c0 + m];
-            lj[u] = Ls[(j0 + u) * LSTR + c0 + m];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) o[u][v] += li[u] * lj[v];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) Ls[(i0 + u) * LSTR + j0 + v] -= o[u][v];
+      __syncthreads();
+      if (fail != 0) {
+        if (tid == 0) *info = TILE * k + fail;
+        return;
       }
+      // rank-16 update of rows and columns c0 + 16 .. 63 (16 x 16 blocks on or below the diagonal)
+      const int nb = 3 - b;
+      for (int q = wave; q < nb * (nb + 1) / 2; q += STEP_WAVES) {
+        const int bi = q < 1 ? 0 : q < 3 ? 1 : 2;
+        const int bj = q - bi * (bi + 1) / 2;
+        const int i0 = c0 + 16 + 16 * bi, j0 = c0 + 16 + 16 * bj;
+        const v4d acc = mfma_xyt<16>(Ls + i0 * LSTR, Ls + j0 * LSTR, c0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ls[(i0 + (lane >> 4) + 4 * v) * LSTR + j0 + (lane & 15)] -= acc[v];
+      }
+      __syncthreads();
     }
+    // the factor back, zero above the diagonal, and L(k, prev) (sb is not written after the solve)
+    for (int e = tid; e < TILE * TILE; e += STEP_THREADS) {
+      const int i = e >> 6, j = e & 63;
+      tile[e] = j <= i ? Ls[i * LSTR + j] : 0.0;
+    }
+    if (prev >= 0) lds_to_tile_step(sb, l_tile(t, k, prev));
+    return;
+  }
+  const int w = bid - st.panels;
+  const int p = st.p[w / per], role = w % per;
+  // the panel of column p + 1 in this launch (same segment) owns tile (p + 1, p) and the update of (p + 1, p + 1)
+  int excl = -1;
+  for (int e = 0; e < st.panels; ++e)
+    if (st.c[e] == p + 1 && seg_of(t, p + 1) == seg_of(t, p)) excl = p + 1;
+  if (role == 0) {
+    // L_pp^-1 for the inverse sweep
+    double* Ls = sa;
+    double* Xs = sb;
+    double* Ys = sc;
+    tile_to_lds_step<TILE>(band_tile(t, p, 0), Ls);
+    for (int e = tid; e < TILE * TILE; e += STEP_THREADS) Xs[(e >> 6) * LSTR + (e & 63)] = 0.0;
     __syncthreads();
-  }
-  // the factor back, zero above the diagonal
-  for (int e = tid; e < TILE * TILE; e += 256) {
-    const int i = e >> 6, j = e & 63;
-    tile[e] = j <= i ? Ls[i * LSTR + j] : 0.0;
-  }
+    if (tid < TILE) rdiag[tid] = 1.0 / Ls[tid * LSTR + tid];
+    __syncthreads();
   // inverse: diagonal 16 x 16 blocks, wavefront w inverts block w (lane j < 16 holds column j)
   {
     const int o = 16 * wave;
-    if (lane < 16) {
+    if (wave < 4 && lane < 16) {
       double x[16];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
@@ -465,7 +609,7 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
 #pragma unroll
   for (int dist = 1; dist < 4; ++dist) {
     const int nblk = 4 - dist;
-    for (int q = tid; q < nblk * 256; q += 256) {
+    for (int q = tid; q < nblk * 256; q += (int)blockDim.x) {
       const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
       const int a = dist + blk, b = blk;
       const double* lrow = Ls + (16 * a + i) * LSTR + 16 * b;
@@ -476,7 +620,7 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
       Ys[blk * 16 * 17 + i * 17 + j] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
     __syncthreads();
-    for (int q = tid; q < nblk * 256; q += 256) {
+    for (int q = tid; q < nblk * 256; q += (int)blockDim.x) {
       const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
       const int a = dist + blk, b = blk;
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -486,82 +630,47 @@ __global__ void __launch_bounds__(256) k_chol_panel(Tiles t, double* __restrict_
     }
     __syncthreads();
   }
-  double* inv = linv + ((size_t)k << 12);
-  for (int e = tid; e < TILE * TILE; e += 256) inv[e] = Xs[(e >> 6) * LSTR + (e & 63)];
-}
-
-// Factor tiles below the diagonal of tile column k (cols.x, or cols.y for the second half of the grid):
-// L(I, k) = A(I, k) L_kk^-T for every below slot of k; workgroup (slot, quarter) forms 16 rows of the tile.
-__global__ void __launch_bounds__(256) k_chol_trsm(Tiles t, const double* __restrict__ linv, int2 cols,
-                                                   const int* info) {
-  __shared__ double Xs[TILE * LSTR];
-  __shared__ double As[16 * LSTR];
-  if (*info != 0) return;
-  const int slots = t.kb + t.ts, per = 4 * slots;
-  const int k = blockIdx.x < per ? cols.x : cols.y;
-  const int j = (blockIdx.x % per) / 4, quarter = blockIdx.x % 4;
-  const int I = below_tile(t, k, j);
-  if (I < 0) return;
-  const int tid = threadIdx.x;
-  const double* a = a_tile(t, I, k) + (quarter * 16 << 6);
-  const double* inv = linv + ((size_t)k << 12);
-  tile_to_lds<TILE>(inv, Xs);
-  tile_to_lds<16>(a, As);
-  __syncthreads();
-  // thread: row r (0..15), columns c0 + 16 v (16 consecutive rows of Xs per load: no bank conflicts)
-  const int r = tid >> 4, c0 = tid & 15;
-  double o[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-  for (int m = 0; m < TILE; ++m) {
-    const double av = As[r * LSTR + m];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) o[v] += av * Xs[(c0 + 16 * v) * LSTR + m];
+    lds_to_tile_step(Xs, linv + ((size_t)p << 12));
+    return;
   }
-  double* l = l_tile(t, I, k) + ((quarter * 16 + r) << 6) + c0;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) l[16 * v] = o[v];
-}
-
-// Trailing update after tile column k (cols.x, or cols.y for the second half of the grid): tile (I, J) -=
-// L(I, k) L(J, k)^T for every pair of below slots i >= j of k; workgroup (pair, quarter) updates 16 rows of
-// the tile. Separator tiles updated from the second domain go to the separate Schur block, so that the two
-// domains' columns never write the same tile.
-__global__ void __launch_bounds__(256) k_chol_update(Tiles t, int2 cols, const int* info) {
-  __shared__ double Ps[16 * LSTR];
-  __shared__ double Qs[TILE * LSTR];
-  if (*info != 0) return;
-  const int slots = t.kb + t.ts, per = 4 * (slots * (slots + 1) / 2);
-  const int k = blockIdx.x < per ? cols.x : cols.y;
-  int i = 0, idx = (blockIdx.x % per) / 4;
-  const int quarter = blockIdx.x % 4;
+  if (role <= slots) {
+    const int I = below_tile(t, p, role - 1);
+    if (I < 0 || I == excl) return;
+    tiles_to_lds_step(band_tile(t, p, 0), sa, a_tile(t, I, p), sb, nullptr, nullptr);
+    __syncthreads();
+    if (tid < TILE) rdiag[tid] = 1.0 / sa[tid * LSTR + tid];
+    __syncthreads();
+    solve_lt(sa, rdiag, sb);
+    __syncthreads();
+    lds_to_tile_step(sb, l_tile(t, I, p));
+    return;
+  }
+  int i = 0, idx = role - 1 - slots;
   while (idx > i) {
     idx -= i + 1;
     ++i;
   }
-  const int I = below_tile(t, k, i), J = below_tile(t, k, idx);
-  if (I < 0 || J < 0) return;
-  const int tid = threadIdx.x;
-  const double* p = l_tile(t, I, k) + (quarter * 16 << 6);
-  const double* q = l_tile(t, J, k);
-  tile_to_lds<TILE>(q, Qs);
-  tile_to_lds<16>(p, Ps);
+  const int I = below_tile(t, p, i), J = below_tile(t, p, idx);
+  if (I < 0 || J < 0 || (I == excl && J == excl)) return;
+  tiles_to_lds_step(band_tile(t, p, 0), sa, a_tile(t, J, p), sb, I != J ? a_tile(t, I, p) : nullptr, sc);
   __syncthreads();
-  const int r = tid >> 4, c0 = tid & 15;
-  double o[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-  for (int m = 0; m < TILE; ++m) {
-    const double pv = Ps[r * LSTR + m];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) o[v] += pv * Qs[(c0 + 16 * v) * LSTR + m];
-  }
-  double* tile;
-  if (seg_of(t, I) == 2 && seg_of(t, J) == 2 && seg_of(t, k) == 1)
-    tile = t.sacc + ((size_t)((I - t.seg[2]) * t.ts + (J - t.seg[2])) << 12);
+  if (tid < TILE) rdiag[tid] = 1.0 / sa[tid * LSTR + tid];
+  __syncthreads();
+  solve_lt(sa, rdiag, sb);
+  if (I != J) solve_lt(sa, rdiag, sc);
+  __syncthreads();
+  const double* LI = I != J ? sc : sb;
+  double* out;
+  if (seg_of(t, I) == 2 && seg_of(t, J) == 2 && seg_of(t, p) == 1)
+    out = t.sacc + ((size_t)((I - t.seg[2]) * t.ts + (J - t.seg[2])) << 12);
   else
-    tile = a_tile(t, I, J);
-  double* out = tile + ((quarter * 16 + r) << 6) + c0;
+    out = a_tile(t, I, J);
+  for (int q = wave; q < 16; q += STEP_WAVES) {
+    const int bi = q >> 2, bj = q & 3;
+    const v4d acc = mfma_xyt<TILE>(LI + 16 * bi * LSTR, sb + 16 * bj * LSTR, 0);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) out[16 * v] -= o[v];
+    for (int v = 0; v < 4; ++v) out[((16 * bi + (lane >> 4) + 4 * v) << 6) + 16 * bj + (lane & 15)] -= acc[v];
+  }
 }
 
 // Tile rows of B = L^-1 as a sweep over tile rows, so that row I can run as soon as the panel of tile column I
@@ -1175,8 +1284,12 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_sacc, (size_t)std::max(t->ts * t->ts, 1) * TILE * TILE);
   TL_ALLOC(t->d_xring, (size_t)3 * (t->kb + 1) * TILE * t->N3p);
   if ((e = hipStreamCreateWithFlags(&t->inv_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
-  if ((e = hipEventCreateWithFlags(&t->ev_panel, hipEventDisableTiming)) != hipSuccess) return fail(e);
-  if ((e = hipEventCreateWithFlags(&t->ev_inverse, hipEventDisableTiming)) != hipSuccess) return fail(e);
+  // the chain's events order the two device streams only: a device-scope release is enough, and spares each
+  // panel the system-scope cache writeback that stalled the next launch of the chain
+  if ((e = hipEventCreateWithFlags(&t->ev_panel, hipEventDisableTiming | hipEventReleaseToDevice)) != hipSuccess)
+    return fail(e);
+  if ((e = hipEventCreateWithFlags(&t->ev_inverse, hipEventDisableTiming | hipEventReleaseToDevice)) != hipSuccess)
+    return fail(e);
   TL_ALLOC(t->d_B, (size_t)t->N3p * t->N3p);
   TL_ALLOC(t->d_BT, (size_t)t->N3p * t->N3p);
   TL_ALLOC(t->d_Bh, (size_t)t->N3p * t->N3p);
@@ -1599,36 +1712,50 @@ static int factorise_chain(Context& c, hipStream_t s, const float* w, double* li
   if (after_product) HIP_TRY(hipEventRecord(after_product, s));
   k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(tl, 1.0 + c.rules.coarse_shift);
   const int slots = t->kb + t->ts;
-  const int pairs = slots * (slots + 1) / 2;
+  const int per = 1 + slots + slots * (slots + 1) / 2;  // worker workgroups per finished column
   auto inverse_columns = [&](int I) { return (TILE * (I + 1) - (I >= t->seg[1] && I < t->seg[2] ? TILE * t->seg[1] : 0)) / INV_COLS; };
-  // One step of the chain: the panels of one or two tile columns (one per domain), then tile rows of B = L^-1
-  // on the inverse stream as soon as their panels are done, beside the chain's later columns, then the
-  // columns' TRSM and trailing update (none after the last tile column).
-  auto column = [&](int k1, int k2) -> int {
-    const int n = k2 >= 0 ? 2 : 1;
-    const int2 cols = make_int2(k1, k2);
-    k_chol_panel<<<n, 256, 0, s>>>(tl, linv, cols, t->d_info);
+  // Tile rows of B = L^-1 go to the inverse stream once their column is finished (its L^-1 tile formed), a few
+  // steps at a time behind one event, beside the chain's later steps.
+  std::vector<int2> rows;
+  int since = 0;
+  auto flush_rows = [&]() -> int {
+    if (rows.empty()) return DRAPE_OK;
     HIP_TRY(hipEventRecord(t->ev_panel, s));
     HIP_TRY(hipStreamWaitEvent(t->inv_stream, t->ev_panel, 0));
-    const int w1 = inverse_columns(k1), w2 = k2 >= 0 ? inverse_columns(k2) : 0;
-    k_inverse_row<<<w1 + w2, 256, 0, t->inv_stream>>>(tl, linv, cols, w1, t->N3p, t->d_xring, B, BT,
-                                                     half ? Bh : nullptr, half ? BTh : nullptr, t->d_info);
-    if (k1 + 1 < t->nt && slots > 0) {
-      k_chol_trsm<<<4 * slots * n, 256, 0, s>>>(tl, linv, cols, t->d_info);
-      k_chol_update<<<4 * pairs * n, 256, 0, s>>>(tl, cols, t->d_info);
+    for (const int2 r : rows) {
+      const int w1 = inverse_columns(r.x), w2 = r.y >= 0 ? inverse_columns(r.y) : 0;
+      k_inverse_row<<<w1 + w2, 256, 0, t->inv_stream>>>(tl, linv, r, w1, t->N3p, t->d_xring, B, BT,
+                                                       half ? Bh : nullptr, half ? BTh : nullptr, t->d_info);
+    }
+    rows.clear();
+    since = 0;
+    return DRAPE_OK;
+  };
+  // step k of a group of segments: the panel of each segment's column k, the workers of its column k - 1
+  auto steps = [&](int first, int last) -> int {
+    int longest = 0;
+    for (int g = first; g <= last; ++g) longest = std::max(longest, t->seg[g + 1] - t->seg[g]);
+    for (int k = 0; k <= longest; ++k) {
+      Step st{};
+      for (int g = first; g <= last; ++g) {
+        const int n = t->seg[g + 1] - t->seg[g];
+        if (k < n) st.c[st.panels++] = t->seg[g] + k;
+        if (k >= 1 && k <= n) st.p[st.workers++] = t->seg[g] + k - 1;
+      }
+      if (st.panels + st.workers == 0) continue;
+      k_chol_step<<<st.panels + st.workers * per, STEP_THREADS, 0, s>>>(tl, linv, st, t->d_info);
+      if (st.workers > 0) rows.push_back(make_int2(st.p[0], st.workers > 1 ? st.p[1] : -1));
+      if (++since == 4) TRY_RC(flush_rows());
     }
     return DRAPE_OK;
   };
   // the two domains side by side, then the separator
-  const int t1 = t->seg[1], t2 = t->seg[2] - t->seg[1];
-  for (int k = 0; k < std::max(t1, t2); ++k) {
-    const int k1 = k < t1 ? k : -1, k2 = k < t2 ? t->seg[1] + k : -1;
-    TRY_RC(k1 >= 0 ? column(k1, k2) : column(k2, -1));
-  }
+  TRY_RC(steps(0, 1));
   if (t->ts > 0) {
     k_add_separator<<<16 * t->ts * t->ts, 256, 0, s>>>(tl);
-    for (int k = t->seg[2]; k < t->nt; ++k) TRY_RC(column(k, -1));
+    TRY_RC(steps(2, 2));
   }
+  TRY_RC(flush_rows());
   HIP_TRY(hipEventRecord(t->ev_inverse, t->inv_stream));
   HIP_TRY(hipStreamWaitEvent(s, t->ev_inverse, 0));
   HIP_TRY(hipGetLastError());
