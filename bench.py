@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part), keyed by (kernel, grid columns); the
 # summaries are under profiles/. (At n129 the bytes are L2 misses served mostly by the Infinity Cache.)
 TRAFFIC = {("k_pcg_spmv", 1025): 523_831_470, ("k_pcg_spmv", 129): 8_638_625, ("k_pcg_dir_spmv", 129): 9_762_528,
-           ("k_tri_gemv_lower", 129): 21_726_692, ("k_tri_gemv_upper", 129): 22_528_454}
+           ("k_tri_gemv_lower", 129): 12_372_435, ("k_tri_gemv_upper", 129): 13_493_029}
 
 
 def spmv_bytes(rows: int, blocks: int) -> int:
@@ -235,7 +235,7 @@ def main() -> None:
 
     # rooflines of the PCG iteration's kernels on the last assembled system of the measured leg, timed with
     # HIP events on the context's stream
-    cache = "n129 working set (matrix 7.7 MB, coarse inverse 2 x 21.3 MB) is resident in the 256 MB Infinity Cache"
+    cache = "n129 working set (matrix 7.7 MB, coarse inverse 2 x 12.0 MB in the split ordering) is resident in the 256 MB Infinity Cache"
     kernels = [kernel_roofline(ctx, "k_pcg_dir_spmv", 400, "direction update fused into the SpMV, every "
                                "iteration after a chunk's first; " + cache),
                kernel_roofline(ctx, "k_pcg_spmv", 400, "plain SpMV, the first iteration of a chunk; " + cache)]

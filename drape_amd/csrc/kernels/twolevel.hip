@@ -680,38 +680,38 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
 // 16 columns per workgroup, over the row's nonzero columns (the second domain's rows start at its first
 // column). X keeps the last kb + 1 tile rows of each segment in FP64 ([3][kb + 1][64][N3p]); B (row-major),
 // B^T and, when Bh is not null, the FP16 copies B~ = S^-1 B (S = diag(B), from L_II^-1) and its transpose get
-// the entries between real unknowns. Thread (row r, four columns): FP64 sums in a fixed order.
+// the entries between real unknowns. Wavefront w forms rows 16 w .. 16 w + 15 with FP64 MFMA (a fixed order).
 constexpr int INV_COLS = 16;
 __device__ inline int inverse_first_column(const Tiles& t, int I) { return seg_of(t, I) == 1 ? TILE * t.seg[1] : 0; }
 __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __restrict__ linv, int2 rows, int w1,
                                                      int N3p, double* __restrict__ X, float* __restrict__ B,
                                                      float* __restrict__ BT, _Float16* __restrict__ Bh,
                                                      _Float16* __restrict__ BTh, const int* info) {
-  __shared__ double Ls[TILE * LSTR];
-  __shared__ double Xs[TILE * 17];
+  __shared__ double Ts[TILE * 17];
   if (*info != 0) return;
   const bool second = (int)blockIdx.x >= w1;
   const int I = second ? rows.y : rows.x;
   const int s = seg_of(t, I);
-  const int tid = threadIdx.x, r = tid >> 2, cq = (tid & 3) * 4;
+  const int lane = threadIdx.x & 63, bi = threadIdx.x >> 6;  // wavefront bi: rows 16 bi .. 16 bi + 15
   const int j0 = inverse_first_column(t, I) + INV_COLS * (second ? blockIdx.x - w1 : blockIdx.x), J = j0 / TILE;
   auto ring = [&](int K) -> double* {
     const int sk = seg_of(t, K);
     return X + (size_t)(sk * (t.kb + 1) + (K - t.seg[sk]) % (t.kb + 1)) * TILE * N3p;
   };
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // acc = -sum_K L(I, K) X(K, j0 ..) for this wavefront's 16 rows: FP64 MFMA with the operands read from L2
+  // (A[i][k] = L(I, K)[16 bi + i][k0 + k] from lane 16 k + i, B[k][j] = X(K)[k0 + k][j0 + j] from lane 16 k + j)
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
   auto term = [&](int K) {
-    __syncthreads();
-    tile_to_lds<TILE>(l_tile(t, I, K), Ls);
-    const double* xs = ring(K) + j0;
-    for (int e = tid; e < TILE * INV_COLS; e += 256) Xs[(e / INV_COLS) * 17 + e % INV_COLS] = xs[(size_t)(e / INV_COLS) * N3p + e % INV_COLS];
-    __syncthreads();
-#pragma unroll 8
-    for (int m = 0; m < TILE; ++m) {
-      const double l = Ls[r * LSTR + m];
+    const double* a = l_tile(t, I, K) + (16 * bi + (lane & 15)) * TILE + (lane >> 4);
+    const double* x = ring(K) + (size_t)(lane >> 4) * N3p + j0 + (lane & 15);
+    double av[16], xv[16];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] -= l * Xs[m * 17 + cq + u];
+    for (int q = 0; q < 16; ++q) {
+      av[q] = a[4 * q];
+      xv[q] = x[(size_t)4 * q * N3p];
     }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[q], xv[q], acc, 0, 0, 0);
   };
   if (s == 2) {
     const int sJ = seg_of(t, J);
@@ -720,34 +720,34 @@ __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __re
         if (K >= J) term(K);
   }
   for (int d = 1; d <= t.kb && I - d >= t.seg[s] && I - d >= J; ++d) term(I - d);
-  __syncthreads();
-  tile_to_lds<TILE>(linv + ((size_t)I << 12), Ls);
+  // T = E(I, j0 ..) + acc in LDS, then X(I, j0 ..) = L_II^-1 T
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int col = j0 + cq + u;
-    Xs[r * 17 + cq + u] = (col == TILE * I + r ? 1.0 : 0.0) + acc[u];
+  for (int v = 0; v < 4; ++v) {
+    const int r = 16 * bi + (lane >> 4) + 4 * v, col = j0 + (lane & 15);
+    Ts[r * 17 + (lane & 15)] = (col == TILE * I + r ? 1.0 : 0.0) + acc[v];
   }
   __syncthreads();
-  double out[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-  for (int m = 0; m < TILE; ++m) {
-    const double l = Ls[r * LSTR + m];
+  const double* li = linv + ((size_t)I << 12) + (16 * bi + (lane & 15)) * TILE + (lane >> 4);
+  double lv[16], tv[16];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) out[u] += l * Xs[m * 17 + cq + u];
+  for (int q = 0; q < 16; ++q) {
+    lv[q] = li[4 * q];
+    tv[q] = Ts[(4 * q + (lane >> 4)) * 17 + (lane & 15)];
   }
-  const int R = TILE * I + r;
-  double* xo = ring(I) + (size_t)r * N3p + j0 + cq;
-  const double sr = Ls[r * LSTR + r];
-  const bool rreal = real_row(t, R);
+  v4d out = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    xo[u] = out[u];
-    const int C = j0 + cq + u;
-    if (rreal && C <= R && real_row(t, C)) {
-      B[(size_t)R * N3p + C] = (float)out[u];
-      BT[(size_t)C * N3p + R] = (float)out[u];
+  for (int q = 0; q < 16; ++q) out = __builtin_amdgcn_mfma_f64_16x16x4f64(lv[q], tv[q], out, 0, 0, 0);
+  double* xo = ring(I);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = 16 * bi + (lane >> 4) + 4 * v, C = j0 + (lane & 15), R = TILE * I + r;
+    xo[(size_t)r * N3p + C] = out[v];
+    if (C <= R && real_row(t, R) && real_row(t, C)) {
+      B[(size_t)R * N3p + C] = (float)out[v];
+      BT[(size_t)C * N3p + R] = (float)out[v];
       if (Bh) {
-        const _Float16 h = (_Float16)fmin(fmax(out[u] / sr, -65504.0), 65504.0);
+        const double sr = linv[((size_t)I << 12) + r * 65];
+        const _Float16 h = (_Float16)fmin(fmax(out[v] / sr, -65504.0), 65504.0);
         Bh[(size_t)R * N3p + C] = h;
         BTh[(size_t)C * N3p + R] = h;
       }
