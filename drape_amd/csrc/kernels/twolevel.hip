@@ -314,71 +314,22 @@ __device__ inline v4d mfma_xyt(const double* X, const double* Y, int k0) {
   return acc;
 }
 
-// The value of the lane given by the quad permutation CTRL (DPP, within groups of four lanes).
-template <int CTRL>
-__device__ inline double quad_perm(double v) {
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false));
-}
-
-// X <- X L^-T in place for a 64-row tile X and the lower factor tile L (both in LDS, stride LSTR; rl[j] =
-// 1 / L_jj): forward substitution by the first 256 threads, four per row. Thread s of a row keeps x_m for
-// m = s (mod 4), zero until formed, so every step sums the same fixed terms without a branch (the terms of
-// unformed x are exact zeros); the four partial sums of each x_j meet by two DPP exchanges inside the quad
-// (every lane gets the same, commutative sums), so the result does not depend on scheduling.
-__device__ inline void solve_lt(const double* L, const double* rl, double* X) {
-  const int tid = threadIdx.x;
-  if (tid >= 256) return;
-  const int r = tid >> 2, s = tid & 3;
-  double own[16], cur[16], nxt[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) own[i] = cur[i] = nxt[i] = 0.0;
-  const double* xr = X + r * LSTR;
-  double xv = xr[0], rv = rl[0], xv_n = 0.0, rv_n = 0.0;
-  // the loads of step j + 1 are issued before step j's dependent chain
-#pragma unroll
-  for (int j = 0; j < TILE; ++j) {
-    if (j + 1 < TILE) {
-#pragma unroll
-      for (int i = 0; 4 * i < j + 1; ++i) nxt[i] = L[(j + 1) * LSTR + 4 * i + s];
-      xv_n = xr[j + 1];
-      rv_n = rl[j + 1];
-    }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int i = 0; 4 * i < j; ++i) acc[i & 3] = fma(own[i], cur[i], acc[i & 3]);
-    double part = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    part += quad_perm<0xB1>(part);  // lanes [1, 0, 3, 2]
-    part += quad_perm<0x4E>(part);  // lanes [2, 3, 0, 1]
-    double xj = (xv - part) * rv;
-    asm volatile("" : "+v"(xj));  // formed by every lane, so that the select below needs no branch
-    own[j >> 2] = (j & 3) == s ? xj : own[j >> 2];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) cur[i] = nxt[i];
-    xv = xv_n;
-    rv = rv_n;
-  }
-  // the row's reads are done: its four threads share one wavefront
-#pragma unroll
-  for (int i = 0; i < 16; ++i) X[r * LSTR + 4 * i + s] = own[i];
-}
-
 // One step of the factorisation chain for up to two tile columns (one per domain), in one launch:
 //  - a panel workgroup per column c = st.c[e]: with p = c - 1 in c's segment, it forms the factor tile
-//    L(c, p) by solving X L_pp^T = A(c, p) (A(c, p) already holds every update from the columns before p),
-//    writes it, subtracts L(c, p) L(c, p)^T from A(c, c) (FP64 MFMA), and factors A(c, c) by 16-column panels:
-//    wavefront 0 factors each panel four columns at a time (the 4 x 4 diagonal block broadcast through LDS and
-//    factored redundantly by every lane, each lane (= row) solving its four factor entries and broadcasting
-//    them for the rank-4 update of the rest of its panel row), then the panel's rank-16 update of the trailing
-//    lower triangle runs on MFMA. A pivot that is not positive (or not finite) records 64 c + column + 1 in
-//    *info and stops every later kernel;
+//    L(c, p) = A(c, p) L_pp^-T (A(c, p) already holds every update from the columns before p; L_pp^-1 came
+//    from p's panel) and subtracts L(c, p) L(c, p)^T from A(c, c), both on FP64 MFMA; then it factors A(c, c)
+//    by 16-column panels: wavefront 0 factors each panel four columns at a time (the 4 x 4 diagonal block
+//    broadcast through LDS and factored redundantly by every lane, each lane (= row) solving its four factor
+//    entries and broadcasting them for the rank-4 update of the rest of its panel row), and the panel's
+//    rank-16 update of the trailing lower triangle runs on MFMA; finally it inverts L_cc blockwise (the four
+//    16 x 16 diagonal blocks by forward substitution in parallel, then the off-diagonal blocks by distance
+//    from the diagonal, X_ab = -X_aa sum_c L_ac X_cb, on MFMA). A pivot that is not positive (or not finite)
+//    records 64 c + column + 1 in *info and stops every later kernel;
 //  - worker workgroups for column p = st.p[e] (the previous column of a segment, or its last one after its
-//    panel): the factor tiles L(I, p) of the below tiles I other than the panel's; one workgroup per pair of
-//    below tiles (I >= J) other than the panel's (c, c), forming L(I, p) and L(J, p) itself and subtracting
-//    L(I, p) L(J, p)^T from tile (I, J) (the second domain's separator block goes to the separate Schur
-//    block); and one that inverts L_pp blockwise for the inverse sweep (the four 16 x 16 diagonal blocks in
-//    parallel, then the off-diagonal blocks by distance from the diagonal, X_ab = -X_aa sum_c L_ac X_cb).
+//    panel): the factor tiles L(I, p) = A(I, p) L_pp^-T of the below tiles I other than the panel's, and one
+//    workgroup per pair of below tiles (I >= J) other than the panel's (c, c), forming L(I, p) and L(J, p)
+//    itself and subtracting L(I, p) L(J, p)^T from tile (I, J) (the second domain's separator block goes to
+//    the separate Schur block).
 // Every tile a workgroup writes is read by no other workgroup of the launch.
 struct Step {
   int panels, workers;
@@ -424,17 +375,84 @@ __device__ inline void lds_to_tile_step(const double* src, double* __restrict__ 
   for (int e = threadIdx.x; e < TILE * TILE; e += STEP_THREADS) dst[e] = src[(e >> 6) * LSTR + (e & 63)];
 }
 
+// This wavefront's two 16 x 16 blocks (q = wave, wave + 8; block (q >> 2, q & 3)) of A Linv^T, A and Linv
+// 64 x 64 in LDS.
+__device__ inline void times_inverse_t(const double* A, const double* Linv, v4d out[2]) {
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int q = wave + STEP_WAVES * h;
+    out[h] = mfma_xyt<TILE>(A + 16 * (q >> 2) * LSTR, Linv + 16 * (q & 3) * LSTR, 0);
+  }
+}
+
+__device__ inline void store_blocks(const v4d in[2], double* X) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int q = wave + STEP_WAVES * h;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) X[(16 * (q >> 2) + (lane >> 4) + 4 * v) * LSTR + 16 * (q & 3) + (lane & 15)] = in[h][v];
+  }
+}
+
+// X = L^-1 for the lower factor tile L (LDS; rd[i] = 1 / L_ii); X (LDS) must be zero on entry; Y: 16 x 17 x 3
+// scratch. The diagonal blocks by forward substitution (wavefront w < 4, lane j < 16 holding column j), then
+// the off-diagonal blocks by distance from the diagonal on MFMA, one wavefront per block.
+__device__ inline void invert_tile(const double* L, double* X, double* Y, const double* rd) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wave < 4 && lane < 16) {
+    const int o = 16 * wave;
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      double acc[4] = {lane == i ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m = 0; m < i; ++m) acc[m & 3] = fma(-L[(o + i) * LSTR + o + m], x[m], acc[m & 3]);
+      x[i] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * rd[o + i];
+      X[(o + i) * LSTR + o + lane] = x[i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int dist = 1; dist < 4; ++dist) {
+    if (wave < 4 - dist) {
+      const int b = wave, a = b + dist;
+      // Y = sum_{c=b}^{a-1} L_ac X_cb: A[i][k] = L[16 a + i][16 b + k'], B[k][j] = X[16 b + k'][16 b + j]
+      const double* lr = L + (16 * a + (lane & 15)) * LSTR + 16 * b + (lane >> 4);
+      const double* xc = X + (16 * b + (lane >> 4)) * LSTR + 16 * b + (lane & 15);
+      v4d y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 16 * dist; k += 4) y = __builtin_amdgcn_mfma_f64_16x16x4f64(lr[k], xc[k * LSTR], y, 0, 0, 0);
+      double* yb = Y + wave * 16 * 17;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) yb[((lane >> 4) + 4 * v) * 17 + (lane & 15)] = y[v];
+      wave_order();
+      // X_ab = -X_aa Y
+      const double* xa = X + (16 * a + (lane & 15)) * LSTR + 16 * a + (lane >> 4);
+      const double* yc = yb + (lane >> 4) * 17 + (lane & 15);
+      v4d z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 16; k += 4) z = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[k], yc[k * 17], z, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) X[(16 * a + (lane >> 4) + 4 * v) * LSTR + 16 * b + (lane & 15)] = -z[v];
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __restrict__ linv, Step st, int* info) {
   __shared__ double sa[TILE * LSTR];
   __shared__ double sb[TILE * LSTR];
   __shared__ double sc[TILE * LSTR];
+  __shared__ double sy[3 * 16 * 17];
   __shared__ double blk[16];
   __shared__ double colbuf[4 * TILE];
   __shared__ double rdiag[TILE];
   __shared__ int fail;
   if (*info != 0) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int slots = t.kb + t.ts, per = 1 + slots + slots * (slots + 1) / 2;
+  const int slots = t.kb + t.ts, per = slots + slots * (slots + 1) / 2;
   const int bid = blockIdx.x;
   if (bid < st.panels) {
     const int k = st.c[bid];
@@ -442,11 +460,12 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
     double* tile = band_tile(t, k, 0);
     double* Ls = sc;
     if (prev >= 0) {
-      tiles_to_lds_step(band_tile(t, prev, 0), sa, a_tile(t, k, prev), sb, tile, sc);
+      tiles_to_lds_step(linv + ((size_t)prev << 12), sa, a_tile(t, k, prev), sb, tile, sc);
       __syncthreads();
-      if (tid < TILE) rdiag[tid] = 1.0 / sa[tid * LSTR + tid];
+      v4d lk[2];
+      times_inverse_t(sb, sa, lk);  // L(k, prev)
       __syncthreads();
-      solve_lt(sa, rdiag, sb);
+      store_blocks(lk, sb);
       __syncthreads();
       // A(k, k) -= L(k, prev) L(k, prev)^T on the ten blocks on or below the diagonal
       for (int q = wave; q < 10; q += STEP_WAVES) {
@@ -459,6 +478,8 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
     } else {
       tile_to_lds_step<TILE>(tile, sc);
     }
+    // sa becomes L_kk^-1
+    for (int e = tid; e < TILE * TILE; e += STEP_THREADS) sa[(e >> 6) * LSTR + (e & 63)] = 0.0;
     if (tid == 0) fail = 0;
     __syncthreads();
     for (int b = 0; b < 4; ++b) {
@@ -564,11 +585,13 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
       }
       __syncthreads();
     }
-    // the factor back, zero above the diagonal, and L(k, prev) (sb is not written after the solve)
+    invert_tile(Ls, sa, sy, rdiag);
+    // the factor back (zero above the diagonal), its inverse, and L(k, prev)
     for (int e = tid; e < TILE * TILE; e += STEP_THREADS) {
       const int i = e >> 6, j = e & 63;
       tile[e] = j <= i ? Ls[i * LSTR + j] : 0.0;
     }
+    lds_to_tile_step(sa, linv + ((size_t)k << 12));
     if (prev >= 0) lds_to_tile_step(sb, l_tile(t, k, prev));
     return;
   }
@@ -578,86 +601,37 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
   int excl = -1;
   for (int e = 0; e < st.panels; ++e)
     if (st.c[e] == p + 1 && seg_of(t, p + 1) == seg_of(t, p)) excl = p + 1;
-  if (role == 0) {
-    // L_pp^-1 for the inverse sweep
-    double* Ls = sa;
-    double* Xs = sb;
-    double* Ys = sc;
-    tile_to_lds_step<TILE>(band_tile(t, p, 0), Ls);
-    for (int e = tid; e < TILE * TILE; e += STEP_THREADS) Xs[(e >> 6) * LSTR + (e & 63)] = 0.0;
-    __syncthreads();
-    if (tid < TILE) rdiag[tid] = 1.0 / Ls[tid * LSTR + tid];
-    __syncthreads();
-  // inverse: diagonal 16 x 16 blocks, wavefront w inverts block w (lane j < 16 holds column j)
-  {
-    const int o = 16 * wave;
-    if (wave < 4 && lane < 16) {
-      double x[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        double acc = lane == i ? 1.0 : 0.0;
-#pragma unroll
-        for (int m = 0; m < i; ++m) acc -= Ls[(o + i) * LSTR + o + m] * x[m];
-        x[i] = acc * rdiag[o + i];
-        Xs[(o + i) * LSTR + o + lane] = x[i];
-      }
-    }
-  }
-  __syncthreads();
-  // off-diagonal blocks (a, b = a - dist): Y = sum_{c=b}^{a-1} L_ac X_cb, then X_ab = -X_aa Y; the sums run
-  // over compile-time lengths in four independent partial sums so that the LDS loads pipeline
-#pragma unroll
-  for (int dist = 1; dist < 4; ++dist) {
-    const int nblk = 4 - dist;
-    for (int q = tid; q < nblk * 256; q += (int)blockDim.x) {
-      const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
-      const int a = dist + blk, b = blk;
-      const double* lrow = Ls + (16 * a + i) * LSTR + 16 * b;
-      const double* xcol = Xs + (16 * b) * LSTR + 16 * b + j;
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int m = 0; m < 16 * dist; ++m) acc[m & 3] += lrow[m] * xcol[m * LSTR];
-      Ys[blk * 16 * 17 + i * 17 + j] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    }
-    __syncthreads();
-    for (int q = tid; q < nblk * 256; q += (int)blockDim.x) {
-      const int blk = q >> 8, i = (q >> 4) & 15, j = q & 15;
-      const int a = dist + blk, b = blk;
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int m = 0; m < 16; ++m) acc[m & 3] += Xs[(16 * a + i) * LSTR + 16 * a + m] * Ys[blk * 16 * 17 + m * 17 + j];
-      Xs[(16 * a + i) * LSTR + 16 * b + j] = -((acc[0] + acc[1]) + (acc[2] + acc[3]));
-    }
-    __syncthreads();
-  }
-    lds_to_tile_step(Xs, linv + ((size_t)p << 12));
-    return;
-  }
-  if (role <= slots) {
-    const int I = below_tile(t, p, role - 1);
+  if (role < slots) {
+    const int I = below_tile(t, p, role);
     if (I < 0 || I == excl) return;
-    tiles_to_lds_step(band_tile(t, p, 0), sa, a_tile(t, I, p), sb, nullptr, nullptr);
+    tiles_to_lds_step(linv + ((size_t)p << 12), sa, a_tile(t, I, p), sb, nullptr, nullptr);
     __syncthreads();
-    if (tid < TILE) rdiag[tid] = 1.0 / sa[tid * LSTR + tid];
-    __syncthreads();
-    solve_lt(sa, rdiag, sb);
-    __syncthreads();
-    lds_to_tile_step(sb, l_tile(t, I, p));
+    v4d li[2];
+    times_inverse_t(sb, sa, li);
+    double* out = l_tile(t, I, p);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = wave + STEP_WAVES * h;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[((16 * (q >> 2) + (lane >> 4) + 4 * v) << 6) + 16 * (q & 3) + (lane & 15)] = li[h][v];
+    }
     return;
   }
-  int i = 0, idx = role - 1 - slots;
+  int i = 0, idx = role - slots;
   while (idx > i) {
     idx -= i + 1;
     ++i;
   }
   const int I = below_tile(t, p, i), J = below_tile(t, p, idx);
   if (I < 0 || J < 0 || (I == excl && J == excl)) return;
-  tiles_to_lds_step(band_tile(t, p, 0), sa, a_tile(t, J, p), sb, I != J ? a_tile(t, I, p) : nullptr, sc);
+  tiles_to_lds_step(linv + ((size_t)p << 12), sa, a_tile(t, J, p), sb, I != J ? a_tile(t, I, p) : nullptr, sc);
   __syncthreads();
-  if (tid < TILE) rdiag[tid] = 1.0 / sa[tid * LSTR + tid];
+  v4d lj[2], li[2];
+  times_inverse_t(sb, sa, lj);
+  if (I != J) times_inverse_t(sc, sa, li);
   __syncthreads();
-  solve_lt(sa, rdiag, sb);
-  if (I != J) solve_lt(sa, rdiag, sc);
+  store_blocks(lj, sb);
+  if (I != J) store_blocks(li, sc);
   __syncthreads();
   const double* LI = I != J ? sc : sb;
   double* out;
@@ -1712,10 +1686,10 @@ static int factorise_chain(Context& c, hipStream_t s, const float* w, double* li
   if (after_product) HIP_TRY(hipEventRecord(after_product, s));
   k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(tl, 1.0 + c.rules.coarse_shift);
   const int slots = t->kb + t->ts;
-  const int per = 1 + slots + slots * (slots + 1) / 2;  // worker workgroups per finished column
+  const int per = slots + slots * (slots + 1) / 2;  // worker workgroups per finished column
   auto inverse_columns = [&](int I) { return (TILE * (I + 1) - (I >= t->seg[1] && I < t->seg[2] ? TILE * t->seg[1] : 0)) / INV_COLS; };
-  // Tile rows of B = L^-1 go to the inverse stream once their column is finished (its L^-1 tile formed), a few
-  // steps at a time behind one event, beside the chain's later steps.
+  // Tile rows of B = L^-1 go to the inverse stream once their panel has run (its L^-1 tile formed), a few steps
+  // at a time behind one event, beside the chain's later steps.
   std::vector<int2> rows;
   int since = 0;
   auto flush_rows = [&]() -> int {
@@ -1744,7 +1718,7 @@ static int factorise_chain(Context& c, hipStream_t s, const float* w, double* li
       }
       if (st.panels + st.workers == 0) continue;
       k_chol_step<<<st.panels + st.workers * per, STEP_THREADS, 0, s>>>(tl, linv, st, t->d_info);
-      if (st.workers > 0) rows.push_back(make_int2(st.p[0], st.workers > 1 ? st.p[1] : -1));
+      if (st.panels > 0) rows.push_back(make_int2(st.c[0], st.panels > 1 ? st.c[1] : -1));
       if (++since == 4) TRY_RC(flush_rows());
     }
     return DRAPE_OK;
