@@ -108,8 +108,11 @@ LEG_NAMES = {"learned": "two-level, learned prolongation", "two_level": "two-lev
 # p^T A p <= 0 when it reduced the residual (two-level fallback = 2, block_jacobi_truncate = 1) and otherwise
 # reports the breakdown; "regularize" always reports it (fallback = 0, block_jacobi_truncate = 0), the
 # reference's rule. Either way the Newton loop raises mu on a reported breakdown.
-POLICIES = {"truncate": {"fallback": 2, "block_jacobi_truncate": 1},
-            "regularize": {"fallback": 0, "block_jacobi_truncate": 0}}
+# Negative-curvature policies, applied to every leg alike. "truncate": a solve that meets p^T A p <= 0 returns
+# its last iterate when that reduced the residual, and a step starts at the mu its predecessor's first update
+# needed (rules.mu_carry); "regularize": the reference's rule, every breakdown raises mu from 0 at each step.
+POLICIES = {"truncate": {"fallback": 2, "block_jacobi_truncate": 1, "mu_carry": 1},
+            "regularize": {"fallback": 0, "block_jacobi_truncate": 0, "mu_carry": 0}}
 
 
 def kernel_roofline(ctx, kernel: str, launches: int, note: str) -> dict:

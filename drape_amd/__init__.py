@@ -55,7 +55,8 @@ class Rules(C.Structure):
                 ("coarse_half", C.c_int32), ("refresh_rate", C.c_double), ("refresh_on_breakdown", C.c_int32),
                 ("self_contact", C.c_int32), ("coarse_async", C.c_int32), ("block_jacobi_truncate", C.c_int32),
                 ("adaptive_kappa", C.c_int32), ("kappa_distance", C.c_double), ("kappa_max_factor", C.c_double),
-                ("coarse_factor", C.c_int32), ("coarse_split", C.c_int32)]
+                ("coarse_factor", C.c_int32), ("coarse_split", C.c_int32),
+                ("mu_carry", C.c_int32)]
 
 
 class PcgStats(C.Structure):

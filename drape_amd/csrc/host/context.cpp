@@ -342,6 +342,7 @@ void drape_default_rules(drape_rules* r) {
   r->kappa_max_factor = 100.0;
   r->coarse_factor = 0;
   r->coarse_split = 1;
+  r->mu_carry = 0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }

@@ -29,6 +29,7 @@ struct Run {
   std::vector<drape_record> records;
   double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   int64_t pcg_iterations = 0;
+  double carried_mu = 0.0;  // rules.mu_carry: the mu the last step's first update was solved at
   explicit Run(Context& ctx) : c(ctx), rules(ctx.rules) {}
 };
 
@@ -260,6 +261,10 @@ int update(Run& run, StepState& state, int step, int iteration, float relative, 
 int solve_step_robust(Run& run, float scale, int step) {
   StepState state;
   state.eta = run.rules.eta_start;
+  // robust-newton.ts:232 starts every step at mu = 0; rules.mu_carry starts it where the last step's first
+  // update needed it, so that a step whose predecessor needed regularisation does not break down again on
+  // the way up
+  if (run.rules.mu_carry && run.rules.regularize) state.mu = run.carried_mu;
   for (int iteration = 0;; ++iteration) {
     int rc = write_mu(run, state.mu);
     if (rc != DRAPE_OK) return rc;
@@ -280,6 +285,7 @@ int solve_step_robust(Run& run, float scale, int step) {
     drape_record extra;
     rc = update(run, state, step, iteration, relative, res, &cont, &extra);
     if (rc != DRAPE_OK) return rc;
+    if (iteration == 0 && cont) run.carried_mu = extra.mu;
     if (!cont) return DRAPE_OK;
     const bool small =
         extra.update_inf <= std::fmax(run.rules.update_stop * extra.step_inf, run.rules.resolution_factor * extra.resolution);

@@ -135,6 +135,10 @@ typedef struct {
                               domains (the coarse grid rows above and below a middle separator as wide as the
                               operator's coupling) and the separator, so that the domains factor side by side
                               and the inverse factor is zero between them; 0 keeps row order (one band) */
+  int32_t mu_carry;        /* 1: a step of the Newton loop starts at the mu its predecessor's first update was
+                              solved at, instead of 0 (robust-newton.ts:232), so that a run whose steps keep
+                              needing regularisation does not meet the same breakdowns again at every step;
+                              0 (default) is the reference's rule. No reference counterpart */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0

@@ -32,6 +32,8 @@ FIX = os.path.join(HERE, "golden", "ptc_probes_n129.npz")
 STEPS = 25
 LEGS = {"learned": dict(preconditioner=1), "learned_fp64": dict(preconditioner=1, refine_fp64=1),
         "block_jacobi": dict(preconditioner=0)}
+# the bench's `truncate` policy: truncation at negative curvature and the carried step-start mu
+CARRY = dict(preconditioner=1, fallback=2, block_jacobi_truncate=1, mu_carry=1)
 
 
 @pytest.fixture(scope="module")
@@ -44,13 +46,13 @@ def scene():
 
 
 def context(arrays, leg):
-    ctx = D.Drape(arrays, D.default_rules(**LEGS[leg]))
+    ctx = D.Drape(arrays, D.default_rules(**(CARRY if leg == "learned_carry" else LEGS[leg])))
     if leg.startswith("learned"):
         ctx.set_prolongation_model(load_model())
     return ctx
 
 
-@pytest.mark.parametrize("leg", list(LEGS))
+@pytest.mark.parametrize("leg", list(LEGS) + ["learned_carry"])
 def test_ptc_reaches_the_representation_floor(scene, leg):
     arrays, od, fix = scene
     ctx = context(arrays, leg)
@@ -127,3 +129,30 @@ def test_ptc_probes_match_the_oracle_step(scene, leg):
             continue
         assert ok, s
     ctx.close()
+
+
+def test_mu_carry_starts_each_step_where_the_last_one_needed_it(scene):
+    """rules.mu_carry: every Newton solve of the PTC run starts at the mu the previous solve's first accepted
+    update was made at (its first record's mu is at least that), where the reference's rule restarts from 0
+    (robust-newton.ts:232) and meets the same breakdowns again on the way up: over the same 12 steps, the
+    first updates of the steps need fewer mu raises (retries) with the carried mu than without it."""
+    arrays, od, fix = scene
+    retries = {}
+    for carry in (0, 1):
+        ctx = D.Drape(arrays, D.default_rules(**dict(CARRY, mu_carry=carry)))
+        ctx.set_prolongation_model(load_model())
+        out = ctx.solve_equilibrium(start=arrays.array("rest_positions"), cap=12)
+        firsts = [r for r in out["records"] if r.iteration == 0]
+        carried, raised = 0.0, 0
+        for r in firsts:
+            if carry:
+                assert r.mu >= carried, (r.step, r.mu, carried)
+            if r.status == 0 and np.isfinite(r.pcg_iterations):
+                carried = r.mu
+                raised += int(r.retries)
+        retries[carry] = raised
+        res = out["result"]
+        print(f"[gpu] mu carry {carry}: {res.newton_iterations} Newton, {res.pcg_iterations} PCG, {res.breakdowns} "
+              f"breakdowns; mu raises at the steps' first updates {raised}", flush=True)
+        ctx.close()
+    assert retries[1] < retries[0]
