@@ -103,7 +103,10 @@ def cpu_baseline(columns: int) -> dict:
 
 
 LEG_NAMES = {"learned": "two-level, learned prolongation", "two_level": "two-level, bilinear prolongation",
-             "block_jacobi": "block-Jacobi"}
+             "rotations": "two-level, bilinear prolongation with rotation unknowns", "block_jacobi": "block-Jacobi"}
+# the coarse space with three rotation unknowns per node beside the translations, its Cholesky factorisation
+# with the pivot floor
+ROTATIONS = {"coarse_rotations": 1, "coarse_pivot_floor": 1e-10}
 # negative-curvature policies applied to every leg alike: "truncate" returns the last iterate before
 # p^T A p <= 0 when it reduced the residual (two-level fallback = 2, block_jacobi_truncate = 1) and otherwise
 # reports the breakdown; "regularize" always reports it (fallback = 0, block_jacobi_truncate = 0), the
@@ -159,8 +162,9 @@ def main() -> None:
     arrays = D.SceneArrays(scene)
 
     def context(pre, policy):
+        extra = ROTATIONS if pre == "rotations" else {}
         c = D.Drape(arrays, D.default_rules(preconditioner=0 if pre == "block_jacobi" else 1,
-                                            self_contact=1 if args.self_contact else 0, **POLICIES[policy]))
+                                            self_contact=1 if args.self_contact else 0, **POLICIES[policy], **extra))
         if pre == "learned":
             from drape_amd.learned import load_model
             c.set_prolongation_model(load_model())

@@ -56,7 +56,7 @@ class Rules(C.Structure):
                 ("self_contact", C.c_int32), ("coarse_async", C.c_int32), ("block_jacobi_truncate", C.c_int32),
                 ("adaptive_kappa", C.c_int32), ("kappa_distance", C.c_double), ("kappa_max_factor", C.c_double),
                 ("coarse_factor", C.c_int32), ("coarse_split", C.c_int32),
-                ("mu_carry", C.c_int32)]
+                ("mu_carry", C.c_int32), ("coarse_rotations", C.c_int32), ("coarse_pivot_floor", C.c_double)]
 
 
 class PcgStats(C.Structure):
@@ -165,6 +165,7 @@ def lib():
                                      i32p]
         L.drape_two_level_inverse_factor.argtypes = [vp, f32p]
         L.drape_two_level_ordering.argtypes = [vp, i32p]
+        L.drape_two_level_rotations.argtypes = [vp, i32p, f32p, C.POINTER(C.c_int64)]
         L.drape_ptc_probe.argtypes = [vp, f32p, C.c_double, C.c_int32, C.POINTER(Record), C.c_int32, i32p, f32p,
                                       f32p]
         L.drape_kernel_timing.argtypes = [vp, C.c_int32, C.c_int32, f64p, f64p]
@@ -476,10 +477,25 @@ class Drape:
         _check(lib().drape_two_level_layout(self.h, C.byref(nc), _ptr(parent, i32p), _ptr(w, f32p), cnt))
         return nc.value, parent.reshape(-1, 4), w.reshape(-1, 4), tuple(cnt)
 
+    def two_level_rotations(self):
+        """(unknowns per coarse node, rotation levers [R, 4, 3] of the current factorisation, pivots replaced by
+        the pivot floor so far)."""
+        dof = C.c_int32()
+        rep = C.c_int64()
+        lv = np.zeros(16 * self.R, np.float32)
+        _check(lib().drape_two_level_rotations(self.h, C.byref(dof), _ptr(lv, f32p), C.byref(rep)))
+        return dof.value, lv.reshape(self.R, 4, 4)[:, :, :3].copy(), rep.value
+
+    def coarse_unknowns(self):
+        """Unknowns of the coarse space (unknowns per node x coarse nodes)."""
+        dof = C.c_int32()
+        _check(lib().drape_two_level_rotations(self.h, C.byref(dof), None, None))
+        return dof.value * self.two_level_layout()[0]
+
     def two_level_coarse(self):
         """The Galerkin coarse operator P^T A P of the assembled system (dense, f64)."""
-        nc = self.two_level_layout()[0]
-        out = np.zeros((3 * nc, 3 * nc))
+        n = self.coarse_unknowns()
+        out = np.zeros((n, n))
         _check(lib().drape_two_level_coarse(self.h, _ptr(out, f64p)))
         return out
 
@@ -539,15 +555,14 @@ class Drape:
     def two_level_inverse_factor(self):
         """B = L^-1 of the current coarse factorisation (FP32, dense lower triangular) in the factor's ordering
         (two_level_ordering)."""
-        nc = self.two_level_layout()[0]
-        out = np.zeros((3 * nc, 3 * nc), np.float32)
+        n = self.coarse_unknowns()
+        out = np.zeros((n, n), np.float32)
         _check(lib().drape_two_level_inverse_factor(self.h, _ptr(out, f32p)))
         return out
 
     def two_level_ordering(self):
-        """The natural coarse unknown 3 I + q at each position of the factor's ordering [3Nc]."""
-        nc = self.two_level_layout()[0]
-        out = np.zeros(3 * nc, np.int32)
+        """The natural coarse unknown dof I + q at each position of the factor's ordering [dof Nc]."""
+        out = np.zeros(self.coarse_unknowns(), np.int32)
         _check(lib().drape_two_level_ordering(self.h, _ptr(out, i32p)))
         return out
 

@@ -343,6 +343,8 @@ void drape_default_rules(drape_rules* r) {
   r->coarse_factor = 0;
   r->coarse_split = 1;
   r->mu_carry = 0;
+  r->coarse_rotations = 0;
+  r->coarse_pivot_floor = 0.0;
 }
 
 const char* drape_last_error(void) { return g_error.c_str(); }
@@ -683,6 +685,13 @@ int drape_two_level_layout(drape_ctx* p, int32_t* coarse_rows, int32_t* parent, 
     counters[3] = (int64_t)(1000.0 * ms);
   }
   return DRAPE_OK;
+}
+
+int drape_two_level_rotations(drape_ctx* p, int32_t* dof, float* levers, int64_t* replaced) {
+  Context* c = ctx_of(p);
+  if (!c) { set_error("null context"); return DRAPE_E_INVALID; }
+  if (!c->two_level) { set_error("no two-level layout on this context"); return DRAPE_E_INVALID; }
+  return two_level_rotations(*c, dof, levers, replaced);
 }
 
 int drape_two_level_coarse(drape_ctx* p, double* coarse) {

@@ -246,9 +246,12 @@ int two_level_iterate(Context& c, const float* partial_pq, const float4* p, floa
 struct CoarseView {
   const int32_t* ppos;    // [4R] position of each parent slot's node in yc (the coarse factor's ordering), -1 if none
   const float* w;         // [4R] prolongation weights
-  const float* yc;        // [3Nc] the coarse solution
+  const float4* lever;    // [4R] rotation levers of the slots (null without rotation unknowns)
+  const float* yc;        // the coarse solution, 3 (or 6 with rotations) unknowns per coarse node
 };
 CoarseView two_level_view(const Context& c);
+int two_level_dof(const Context& c);  // unknowns per coarse node: 3, or 6 with rotations
+int two_level_rotations(Context& c, int32_t* dof, float* levers, int64_t* replaced);
 int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32 = false);  // fp32: FP32 B always
 void two_level_observe(Context& c, int iterations, double relative_residual);
 int two_level_galerkin(Context& c, double* out);       // A_c of the assembled system, row-major, to the host
@@ -277,9 +280,12 @@ struct GnnArgs {
   const float* theta;       // parameters
   float *scale, *features, *edge, *h0, *h1, *msg;  // scratch: [R], [13R], [B], [32R] x 3
   float* w;                 // [4R] output weights
+  float* sigma;             // [4R] output rotation scales (the rotation head; null: the network has none)
 };
 constexpr int GNN_FEATURES = 13, GNN_WIDTH = 32, GNN_LAYERS = 3;
-int gnn_param_count();
+// Parameters of the network: input layer, three layers and the weight head; the rotation head (Wr [4 x 32],
+// br [4] after them) adds 4 x 32 + 4.
+int gnn_param_count(bool rotation_head = false);
 int gnn_infer(const GnnArgs& a);
 int gnn_features(const GnnArgs& a);  // scale, features and normalised edge weights only
 // gnn_infer with every activation kept: hs [4][32R] (h0..h3), ms [3][32R] (the messages of the three

@@ -6,9 +6,12 @@
 //     coupling strength sum_j |A_ij|_F / (3 s_i) over the off-diagonal blocks (CSR order);
 //   edge weights c_ij = |A_ij|_F / (3 sqrt(s_i s_j)), normalised per row by (sum_j c_ij + 1e-6);
 //   h0 = relu(W0 f + b0); three layers h <- relu(Ws h_i + Wn m_i + b), m_i = sum_j c~_ij h_j;
-//   delta = Wo h + bo; w_ik = mass_i softmax_k(log w_bilinear_ik + delta_ik) over the non-empty slots.
+//   delta = Wo h + bo; w_ik = mass_i softmax_k(log w_bilinear_ik + delta_ik) over the non-empty slots;
+//   with the rotation head (coarse nodes with rotation unknowns), the rotation scale of each slot
+//   sigma_ik = exp(clamp(Wr h + br, -3, 3))_k, which multiplies the slot's geometric lever, so a zero rotation
+//   head keeps the levers of the current shape.
 // Parameters: one flat array, W0 [32 x 13], b0 [32], then per layer Ws [32 x 32], Wn [32 x 32], b [32], then
-// Wo [4 x 32], bo [4], row-major.
+// Wo [4 x 32], bo [4], and optionally Wr [4 x 32], br [4], row-major.
 #include <cmath>
 
 #include "../internal.h"
@@ -124,9 +127,11 @@ __global__ void __launch_bounds__(256) k_gnn_dense(const float* __restrict__ lay
   out[t] = fmaxf(a + b + layer[2 * H * H + o], 0.0f);
 }
 
-// delta = Wo h + bo, then the softmax over the non-empty slots scaled by the bilinear mass
+// delta = Wo h + bo, then the softmax over the non-empty slots scaled by the bilinear mass; with `sigma`, the
+// rotation head's scales from Wr, br (after bo)
 __global__ void __launch_bounds__(256) k_gnn_head(const float* __restrict__ head, const float* __restrict__ h,
-                                                  const float* __restrict__ wbil, int R, float* __restrict__ w) {
+                                                  const float* __restrict__ wbil, int R, float* __restrict__ w,
+                                                  float* __restrict__ sigma) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= R) return;
   const float* hi = h + (size_t)H * i;
@@ -148,11 +153,22 @@ __global__ void __launch_bounds__(256) k_gnn_head(const float* __restrict__ head
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) w[4 * i + k] = mass * (e[k] / sum);
+  if (sigma) {
+    const float* rot = head + 4 * H + 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float o = rot[4 * H + k];
+      for (int j = 0; j < H; ++j) o = o + rot[H * k + j] * hi[j];
+      sigma[4 * i + k] = expf(fminf(fmaxf(o, -3.0f), 3.0f));
+    }
+  }
 }
 
 }  // namespace
 
-int gnn_param_count() { return H * F_IN + H + 3 * (2 * H * H + H) + 4 * H + 4; }
+int gnn_param_count(bool rotation_head) {
+  return H * F_IN + H + 3 * (2 * H * H + H) + 4 * H + 4 + (rotation_head ? 4 * H + 4 : 0);
+}
 
 int gnn_features(const GnnArgs& a) {
   hipStream_t s = a.stream;
@@ -180,7 +196,7 @@ int gnn_infer(const GnnArgs& a) {
     h = out;
     out = tmp;
   }
-  k_gnn_head<<<blocks_for(R), 256, 0, s>>>(layer, h, a.wbil, R, a.w);
+  k_gnn_head<<<blocks_for(R), 256, 0, s>>>(layer, h, a.wbil, R, a.w, a.sigma);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }
@@ -197,7 +213,7 @@ int gnn_forward_keep(const GnnArgs& a, float* hs, float* ms) {
     k_gnn_dense<<<blocks_for(R * H), 256, 0, s>>>(layer, hs + l * plane, ms + l * plane, R, hs + (l + 1) * plane);
     layer += 2 * H * H + H;
   }
-  k_gnn_head<<<blocks_for(R), 256, 0, s>>>(layer, hs + GNN_LAYERS * plane, a.wbil, R, a.w);
+  k_gnn_head<<<blocks_for(R), 256, 0, s>>>(layer, hs + GNN_LAYERS * plane, a.wbil, R, a.w, a.sigma);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }

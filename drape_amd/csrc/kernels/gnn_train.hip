@@ -135,10 +135,11 @@ __global__ void __launch_bounds__(256) k_spmv_dot(SellView A, const float4* __re
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// eps = e - P c (ppos: the position of each parent slot's node in c, -1 when empty)
+// eps = e - P c (ppos: the position of each parent slot's node in c, -1 when empty; lever: the slots' rotation
+// levers, null without rotation unknowns)
 __global__ void __launch_bounds__(256) k_eps(const int32_t* __restrict__ ppos, const float* __restrict__ w,
-                                             const float* __restrict__ yc, const float4* __restrict__ e,
-                                             float4* __restrict__ eps, int R) {
+                                             const float4* __restrict__ lever, const float* __restrict__ yc,
+                                             const float4* __restrict__ e, float4* __restrict__ eps, int R) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= R) return;
   float4 v = e[i];
@@ -150,6 +151,13 @@ __global__ void __launch_bounds__(256) k_eps(const int32_t* __restrict__ ppos, c
       v.x = v.x - ww * yc[P];
       v.y = v.y - ww * yc[P + 1];
       v.z = v.z - ww * yc[P + 2];
+      if (lever) {
+        const float4 lv = lever[4 * i + k];
+        const float a = yc[P + 3], b = yc[P + 4], c = yc[P + 5];
+        v.x = v.x - (b * lv.z - c * lv.y);
+        v.y = v.y - (c * lv.x - a * lv.z);
+        v.z = v.z - (a * lv.y - b * lv.x);
+      }
     }
   }
   eps[i] = v;
@@ -399,6 +407,10 @@ int check_context(Context& c) {
     set_error("prolongation training needs a mesh context with a two-level layout");
     return DRAPE_E_INVALID;
   }
+  if (two_level_dof(c) != 3) {
+    set_error("prolongation training: coarse rotation unknowns are not supported by the trainer yet");
+    return DRAPE_E_INVALID;
+  }
   return ensure_state(c);
 }
 
@@ -489,7 +501,7 @@ int gnn_train_gradient(Context& c, const float* params, int count, double* loss,
     const float4* E = g->d_E + (size_t)e * R;
     k_spmv_dot<<<c.G, 256, 0, s>>>(A, E, g->d_q, g->d_partial);
     TRY_RC(two_level_restrict_solve(c, g->d_q, 2, true));
-    k_eps<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.w, cv.yc, E, g->d_eps, R);
+    k_eps<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.w, cv.lever, cv.yc, E, g->d_eps, R);
     k_spmv_dot<<<c.G, 256, 0, s>>>(A, g->d_eps, g->d_q, g->d_partial + c.G);
     k_sum_pair<<<1, 256, 0, s>>>(g->d_partial, c.G, g->d_sums + 2 * e);
     k_weight_grad<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.yc, g->d_q, g->d_sums + 2 * e, n, e == 0, g->d_gw, R);

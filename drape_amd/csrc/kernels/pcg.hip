@@ -12,6 +12,7 @@
 // consumers: every workgroup re-reduces the <= ceil(R/256) partials in the same fixed order, so one
 // iteration is three launches (spmv+dot, update+dots, direction).
 #include "../internal.h"
+#include "coarse.hpp"
 #include "reduce.hpp"
 #include "sell.hpp"
 
@@ -374,19 +375,10 @@ __global__ void __launch_bounds__(256) k_vc_residual1(SellView A, const float4* 
   r1[row] = make_float4(rv.x - a.x, rv.y - a.y, rv.z - a.z, 0.0f);
 }
 
-// z2 of a row: z1 + sum over its parent slots of w y_parent
+// z2 of a row: z1 + (P y)_row over its parent slots
 __device__ inline float4 vc_z2(const float4* __restrict__ z1, const CoarseView& cv, uint32_t row) {
   float4 v = z1[row];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int P = cv.ppos[4 * row + k];
-    if (P >= 0) {
-      const float ww = cv.w[4 * row + k];
-      v.x = v.x + ww * cv.yc[P];
-      v.y = v.y + ww * cv.yc[P + 1];
-      v.z = v.z + ww * cv.yc[P + 2];
-    }
-  }
+  coarse_prolong_add(cv.ppos, cv.w, cv.lever, cv.yc, (int)row, v.x, v.y, v.z);
   return v;
 }
 

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../internal.h"
+#include "coarse.hpp"
 #include "reduce.hpp"
 
 namespace drape {
@@ -38,7 +39,8 @@ namespace drape {
 struct TwoLevel {
   int nc = 0;        // coarse grid columns
   int Nc = 0;        // coarse rows (free coarse nodes)
-  int N3 = 0;        // 3 Nc
+  int dof = 3;       // unknowns per coarse node: 3 translations, or 6 with rotations
+  int N3 = 0;        // dof Nc coarse unknowns
   int N3p = 0;       // padded row stride of the FP32 inverse (multiple of 64 floats)
   int npairs = 0;    // coarse block pairs
   uint64_t factored = ~0ull;  // system version of the current factorisation (~0: none)
@@ -54,7 +56,14 @@ struct TwoLevel {
   float* d_w = nullptr;          // [4R] current weights
   float* d_wbil = nullptr;       // [4R] bilinear weights
   float2* d_fxy = nullptr;       // [R] position of each row inside its coarse cell
+  float4* d_lever = nullptr;     // [4R] rotation lever of each slot at the current factorisation (rotations only)
+  float* d_sigma = nullptr;      // [4R] the network's rotation scales of the slots (rotation head only)
+  int32_t* d_cvert = nullptr;    // [Nc] vertex of each coarse node
+  float inv_spacing = 1.0f;      // 1 / H, H the coarse grid spacing of the rest shape
+  double* d_adiag = nullptr;     // [N3p] diagonal of A_c before the factorisation (the pivot floor's reference)
   float* d_theta = nullptr;      // learned-prolongation parameters (null: fixed weights)
+  int theta_count = 0;           // their number
+  bool rotation_head = false;    // the network sets the rotation scales (rotation unknowns only)
   float* d_gnn = nullptr;        // its scratch: scale [R], features [13R], edge [B], activations [32R] x 3
   uint32_t* d_child_ptr = nullptr;  // [Nc+1]
   uint32_t* d_child = nullptr;      // 4 i + k of each child slot, in fine-row order
@@ -81,6 +90,7 @@ struct TwoLevel {
   hipStream_t inv_stream = nullptr; // the inverse sweep, behind each panel of the factorisation chain
   hipEvent_t ev_panel = nullptr, ev_inverse = nullptr;
   double* d_Ac = nullptr;           // [N3 * N3] dense A_c for the check entry point (allocated on demand)
+  float4* d_Ac_lever = nullptr;     // [4R] the check entry point's levers (rotations only, allocated on demand)
   float* d_B = nullptr;             // [N3p * N3p] row-major B = L^-1 in the factor's ordering (zero above the
                                     // diagonal, between the domains and at padding)
   float* d_BT = nullptr;            // [N3p * N3p] row-major B^T
@@ -97,6 +107,8 @@ struct TwoLevel {
     double* linv = nullptr;
     float *B = nullptr, *BT = nullptr;
     _Float16 *Bh = nullptr, *BTh = nullptr;
+    float* w = nullptr;             // the prolongation the saved factorisation was formed with
+    float4* lever = nullptr;
   } saved;
   int64_t restores = 0;             // factorisations avoided by returning to the saved one
   // coarse_async: a factorisation running on `side` into these buffers while the solves keep the current one
@@ -107,10 +119,11 @@ struct TwoLevel {
     uint64_t factored = 0, factored_shift = 0;
     double factored_h = 0.0;
     float* w = nullptr;
+    float4* lever = nullptr;
     double* linv = nullptr;
     float *B = nullptr, *BT = nullptr;
     _Float16 *Bh = nullptr, *BTh = nullptr;
-    int* h_info = nullptr;          // pinned
+    int* h_info = nullptr;          // pinned [2]: failure, replaced pivots
     hipStream_t side = nullptr;
     hipEvent_t ready = nullptr, read = nullptr, done = nullptr;
     hipEvent_t began = nullptr, ended = nullptr;  // timing events around the chain on the side stream
@@ -118,9 +131,10 @@ struct TwoLevel {
   float* d_t = nullptr;             // [N3p] B r_c
   float* d_rc = nullptr;            // [N3p]
   float* d_yc = nullptr;            // [N3p]
-  int* d_info = nullptr;
-  int* h_info = nullptr;
+  int* d_info = nullptr;             // [2]: 64 tile + column + 1 of a failed pivot (0: none), replaced pivots
+  int* h_info = nullptr;             // pinned [2]
   int64_t factorisations = 0, failures = 0;
+  int64_t replaced = 0;              // pivots replaced by the pivot floor, over all factorisations
   double factor_ms = 0.0;     // time of all factorisations (Galerkin product, Cholesky, inverse): wall time of the
                               // synchronous ones, device time (events on the side stream) of the side-stream ones
 };
@@ -134,12 +148,15 @@ void two_level_destroy(TwoLevel* t) {
     if (e) (void)hipEventDestroy(e);
   if (t->inv_stream) (void)hipStreamDestroy(t->inv_stream);
   if (t->d_xring) (void)hipFree(t->d_xring);
-  for (void* p : {(void*)q.w, (void*)q.linv, (void*)q.B, (void*)q.BT, (void*)q.Bh, (void*)q.BTh})
+  for (void* p : {(void*)q.w, (void*)q.lever, (void*)q.linv, (void*)q.B, (void*)q.BT, (void*)q.Bh, (void*)q.BTh})
     if (p) (void)hipFree(p);
   if (q.h_info) (void)hipHostFree(q.h_info);
   for (hipEvent_t e : {q.ready, q.read, q.done, q.began, q.ended})
     if (e) (void)hipEventDestroy(e);
   if (q.side) (void)hipStreamDestroy(q.side);
+  for (void* p : {(void*)t->d_lever, (void*)t->d_sigma, (void*)t->d_cvert, (void*)t->d_adiag, (void*)t->saved.w,
+                  (void*)t->saved.lever, (void*)t->d_Ac_lever})
+    if (p) (void)hipFree(p);
   for (void* p : {(void*)t->d_parent, (void*)t->d_w, (void*)t->d_wbil, (void*)t->d_fxy, (void*)t->d_theta,
                   (void*)t->d_gnn, (void*)t->d_child_ptr, (void*)t->d_child, (void*)t->d_pair_ptr,
                   (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel,
@@ -175,6 +192,8 @@ struct Tiles {
   int seg[4];       // first tile of each segment (first domain, second domain, separator), seg[3] = nt
   int bstart[2];    // first border tile of each domain (= its segment's end without a separator)
   int real[3];      // unknowns of each segment; the rest of its tiles is padding with a unit diagonal
+  double* adiag;    // [64 nt] the diagonal of A_c as factored (padding: 1)
+  double floor;     // pivot floor: a pivot <= floor * adiag is replaced by adiag (0: such a pivot fails)
 };
 
 __device__ inline int seg_of(const Tiles& t, int T) { return T < t.seg[1] ? 0 : (T < t.seg[2] ? 1 : 2); }
@@ -226,54 +245,116 @@ __device__ inline void tile_to_lds(const double* __restrict__ src, double* dst) 
   }
 }
 
+// One Galerkin term P_ik^T A_b P_jl of the coarse block pair: the fine block A_b (SELL planes, stride 64) between
+// the slots ik and jl, added to the DOF x DOF accumulator in FP64. With rotations the slot blocks are
+// [w I | S], S = -[v]x (coarse.hpp), so the pair block is [[wi wj A, wi A Sj], [wj Si^T A, Si^T A Sj]].
+template <int DOF>
+__device__ inline void galerkin_term(double* acc, const float* __restrict__ a, const float* __restrict__ w,
+                                     const float4* __restrict__ lever, uint32_t si, uint32_t sj) {
+  if (DOF == 3) {
+    const double ww = (double)w[si] * (double)w[sj];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] += ww * (double)a[64 * q];
+  } else {
+    double A[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) A[q] = (double)a[64 * q];
+    const double wi = w[si], wj = w[sj];
+    const float4 fi = lever[si], fj = lever[sj];
+    const double vi[3] = {fi.x, fi.y, fi.z}, vj[3] = {fj.x, fj.y, fj.z};
+    // Sj = -[vj]x and Si^T = [vi]x, row-major
+    const double Sj[9] = {0.0, vj[2], -vj[1], -vj[2], 0.0, vj[0], vj[1], -vj[0], 0.0};
+    const double Ti[9] = {0.0, -vi[2], vi[1], vi[2], 0.0, -vi[0], -vi[1], vi[0], 0.0};
+    double X[9], Y[9];  // A Sj and Si^T A
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        X[3 * r + c] = A[3 * r] * Sj[c] + A[3 * r + 1] * Sj[3 + c] + A[3 * r + 2] * Sj[6 + c];
+        Y[3 * r + c] = Ti[3 * r] * A[c] + Ti[3 * r + 1] * A[3 + c] + Ti[3 * r + 2] * A[6 + c];
+      }
+    const double ww = wi * wj;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        acc[6 * r + c] += ww * A[3 * r + c];
+        acc[6 * r + 3 + c] += wi * X[3 * r + c];
+        acc[6 * (3 + r) + c] += wj * Y[3 * r + c];
+        acc[6 * (3 + r) + 3 + c] += Ti[3 * r] * X[c] + Ti[3 * r + 1] * X[3 + c] + Ti[3 * r + 2] * X[6 + c];
+      }
+  }
+}
+
 // The coarse block pair (I, J) into the tiles, entries on or below the diagonal of the factor's ordering
 // (pos: the position of each coarse node's first unknown): one wavefront per pair, lane l summing the pair's
-// terms l, l + 64, ... for all nine entries, then a fixed-order wavefront tree, so the sum is reproducible.
-// (One thread per entry walking a pair's few hundred terms in sequence was latency-bound: 248 us per product
-// at n129.)
+// terms l, l + 64, ... for all DOF x DOF entries, then a fixed-order wavefront tree, so the sum is
+// reproducible. (One thread per entry walking a pair's few hundred terms in sequence was latency-bound: 248 us
+// per product at n129.)
+template <int DOF>
 __global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restrict__ pair_ptr,
                                                        const uint32_t* __restrict__ pair_key,
                                                        const uint4* __restrict__ terms, const float* __restrict__ w,
+                                                       const float4* __restrict__ lever,
                                                        const float* __restrict__ sell_val,
                                                        const uint32_t* __restrict__ block_dst, int npairs, int Nc,
                                                        const int32_t* __restrict__ pos, Tiles tl) {
+  constexpr int NE = DOF * DOF;
   const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (p >= npairs) return;  // uniform per wavefront
   const uint32_t key = pair_key[p];
   const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
-  double acc[9];
+  double acc[NE];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  for (int q = 0; q < NE; ++q) acc[q] = 0.0;
   const uint32_t t1 = pair_ptr[p + 1];
   for (uint32_t t = pair_ptr[p] + lane; t < t1; t += 64) {
     const uint4 e = terms[t];
-    const double ww = (double)w[e.y] * (double)w[e.z];
-    const float* a = sell_val + block_dst[e.x];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] += ww * (double)a[64 * q];
+    galerkin_term<DOF>(acc, sell_val + block_dst[e.x], w, lever, e.y, e.z);
   }
 #pragma unroll
-  for (int q = 0; q < 9; ++q)
+  for (int q = 0; q < NE; ++q)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_down(acc[q], o, 64);
   if (lane == 0) {
     const int pI = pos[I], pJ = pos[J];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      const int R = pI + q / 3, C = pJ + q % 3;
+    for (int q = 0; q < NE; ++q) {
+      const int R = pI + q / DOF, C = pJ + q % DOF;
       if (R < C) continue;
       a_tile(tl, R >> 6, C >> 6)[((R & 63) << 6) + (C & 63)] = acc[q];
     }
   }
 }
 
-// Diagonal of A_c scaled by `factor`; the padding rows of each segment get a unit diagonal.
+// The rotation lever of every parent slot from the current positions: v = w sigma (x_i - X_I) / H
+// (sigma: the network's rotation scale of the slot, 1 when null); empty slots get zero.
+__global__ void __launch_bounds__(256) k_lever(const int32_t* __restrict__ parent, const float* __restrict__ w,
+                                               const float* __restrict__ sigma, const uint32_t* __restrict__ rvert,
+                                               const int32_t* __restrict__ cvert, const float4* __restrict__ pos,
+                                               float inv_h, int slots, float4* __restrict__ lever) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= slots) return;
+  const int I = parent[s];
+  float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (I >= 0) {
+    const float4 xf = pos[rvert[s >> 2]], xc = pos[cvert[I]];
+    const float g = w[s] * (sigma ? sigma[s] : 1.0f) * inv_h;
+    v = make_float4(g * (xf.x - xc.x), g * (xf.y - xc.y), g * (xf.z - xc.z), 0.0f);
+  }
+  lever[s] = v;
+}
+
+// Diagonal of A_c scaled by `factor`; the padding rows of each segment get a unit diagonal. The result is kept
+// in adiag for the pivot floor.
 __global__ void k_band_diagonal(Tiles tl, double factor) {
   const int R = blockIdx.x * blockDim.x + threadIdx.x;
   if (R >= tl.nt * TILE) return;
   double* a = band_tile(tl, R >> 6, 0) + ((R & 63) << 6) + (R & 63);
-  *a = real_row(tl, R) ? *a * factor : 1.0;
+  const double d = real_row(tl, R) ? *a * factor : 1.0;
+  *a = d;
+  tl.adiag[R] = d;
 }
 
 // The second domain's Schur update into the separator block (tiles on or below its diagonal).
@@ -482,6 +563,7 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
     for (int e = tid; e < TILE * TILE; e += STEP_THREADS) sa[(e >> 6) * LSTR + (e & 63)] = 0.0;
     if (tid == 0) fail = 0;
     __syncthreads();
+    int nrep = 0;  // pivots replaced by the floor (wavefront 0, the same on every lane)
     for (int b = 0; b < 4; ++b) {
       const int c0 = 16 * b;
       if (wave == 0) {
@@ -514,6 +596,14 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
             double piv = d[v][v];
 #pragma unroll
             for (int m = 0; m < v; ++m) piv = fma(-l[v][m], l[v][m], piv);
+            if (t.floor > 0.0) {
+              // modified Cholesky: a collapsed or negative pivot becomes the column's diagonal entry of A_c
+              const double a = t.adiag[TILE * k + c0 + q + v];
+              if (!(piv > t.floor * a) && piv > -INFINITY && a > 0.0 && a < INFINITY) {
+                piv = a;
+                ++nrep;
+              }
+            }
             if (!(piv > 0.0) || !(piv < INFINITY)) {
               if (bad == 0) bad = c0 + q + v + 1;
               piv = 1.0;
@@ -585,6 +675,7 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
       }
       __syncthreads();
     }
+    if (wave == 0 && lane == 0 && nrep > 0) atomicAdd(info + 1, nrep);
     invert_tile(Ls, sa, sy, rdiag);
     // the factor back (zero above the diagonal), its inverse, and L(k, prev)
     for (int e = tid; e < TILE * TILE; e += STEP_THREADS) {
@@ -729,36 +820,53 @@ __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __re
   }
 }
 
-// dense A_c (both triangles, row-major) for the check entry point
+// dense A_c (both triangles, row-major, DOF unknowns per coarse node) for the check entry point
+template <int DOF>
 __global__ void __launch_bounds__(256) k_galerkin_dense(const uint32_t* __restrict__ pair_ptr,
                                                         const uint32_t* __restrict__ pair_key,
                                                         const uint4* __restrict__ terms, const float* __restrict__ w,
+                                                        const float4* __restrict__ lever,
                                                         const float* __restrict__ sell_val,
                                                         const uint32_t* __restrict__ block_dst, int npairs, int Nc,
                                                         double* __restrict__ Ac) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double acc[DOF * DOF];
+#pragma unroll
+  for (int q = 0; q < DOF * DOF; ++q) acc[q] = 0.0;
   for (uint32_t t = pair_ptr[p]; t < pair_ptr[p + 1]; ++t) {
     const uint4 e = terms[t];
-    const double ww = (double)w[e.y] * (double)w[e.z];
-    const float* a = sell_val + block_dst[e.x];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] += ww * (double)a[64 * q];
+    galerkin_term<DOF>(acc, sell_val + block_dst[e.x], w, lever, e.y, e.z);
   }
   const uint32_t key = pair_key[p];
   const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
-  const size_t N3 = 3 * (size_t)Nc;
+  const size_t N = (size_t)DOF * Nc;
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
+  for (int r = 0; r < DOF; ++r)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) Ac[(3 * (size_t)I + r) * N3 + 3 * (size_t)J + c] = acc[3 * r + c];
+    for (int c = 0; c < DOF; ++c) Ac[(DOF * (size_t)I + r) * N + DOF * (size_t)J + c] = acc[DOF * r + c];
 }
 
-// r_c[I] = sum over the children (i, k) of I of w_ik r_i at the node's position in the factor's ordering
-// (pos): one wavefront per coarse node, lane j takes child j (a node has at most 49), then a fixed shuffle tree.
+// The restriction's fixed shuffle tree and store: r_c at the node's position P, three translation sums and,
+// with rotations, three rotation sums.
+template <bool ROT>
+__device__ inline void restrict_store(float* a, int lane, int P, float* __restrict__ rc) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < (ROT ? 6 : 3); ++q) a[q] = a[q] + __shfl_down(a[q], o, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < (ROT ? 6 : 3); ++q) rc[P + q] = a[q];
+}
+
+// r_c[I] = sum over the children (i, k) of I of P_ik^T r_i at the node's position in the factor's ordering
+// (pos): w_ik r_i, and with rotations v_ik x r_i; one wavefront per coarse node, lane j takes child j (a node
+// has at most 49), then a fixed shuffle tree.
+template <bool ROT>
 __global__ void __launch_bounds__(256) k_restrict(const uint32_t* __restrict__ child_ptr,
                                                   const uint32_t* __restrict__ child, const float* __restrict__ w,
+                                                  const float4* __restrict__ lever,
                                                   const int32_t* __restrict__ pos, const float4* __restrict__ r,
                                                   int Nc, float* __restrict__ rc, const PcgState* st, int guard) {
   if (guard == 0 && st->status != 0) return;
@@ -767,27 +875,22 @@ __global__ void __launch_bounds__(256) k_restrict(const uint32_t* __restrict__ c
   const int lane = threadIdx.x & 63;
   if (I >= Nc) return;
   const uint32_t t0 = child_ptr[I], n = child_ptr[I + 1] - t0;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   for (uint32_t j = lane; j < n; j += 64) {
     const uint32_t s = child[t0 + j];
     const float ww = w[s];
     const float4 v = r[s >> 2];
-    a0 = a0 + ww * v.x;
-    a1 = a1 + ww * v.y;
-    a2 = a2 + ww * v.z;
+    a[0] = a[0] + ww * v.x;
+    a[1] = a[1] + ww * v.y;
+    a[2] = a[2] + ww * v.z;
+    if (ROT) {
+      const float3 c = lever_cross(lever[s], v.x, v.y, v.z);
+      a[3] = a[3] + c.x;
+      a[4] = a[4] + c.y;
+      a[5] = a[5] + c.z;
+    }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    a0 = a0 + __shfl_down(a0, o, 64);
-    a1 = a1 + __shfl_down(a1, o, 64);
-    a2 = a2 + __shfl_down(a2, o, 64);
-  }
-  if (lane == 0) {
-    const int P = pos[I];
-    rc[P] = a0;
-    rc[P + 1] = a1;
-    rc[P + 2] = a2;
-  }
+  restrict_store<ROT>(a, lane, pos[I], rc);
 }
 
 // Row ranges of B = L^-1 in the factor's ordering: d2 and sep are the first positions of the second domain
@@ -911,6 +1014,7 @@ __global__ void __launch_bounds__(256) k_tri_gemv_h(const _Float16* __restrict__
 // z = D^-1 r + sum_k w_ik y_c[parent(i, k)], partial (r.z, r.r); mode 2 (setup) also sets p = z. ppos holds the
 // position of each parent slot's node in the factor's ordering (-1: empty slot).
 __global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ ppos, const float* __restrict__ w,
+                                                 const float4* __restrict__ lever,
                                                  const float* __restrict__ yc, const float* __restrict__ dinv,
                                                  const float4* __restrict__ r, float4* __restrict__ z,
                                                  float4* __restrict__ p, float* __restrict__ partials, int R,
@@ -926,16 +1030,7 @@ __global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ ppo
     float z0 = d[0] * rv.x + d[1] * rv.y + d[2] * rv.z;
     float z1 = d[3] * rv.x + d[4] * rv.y + d[5] * rv.z;
     float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int P = ppos[4 * row + k];
-      if (P >= 0) {
-        const float ww = w[4 * row + k];
-        z0 = z0 + ww * yc[P];
-        z1 = z1 + ww * yc[P + 1];
-        z2 = z2 + ww * yc[P + 2];
-      }
-    }
+    coarse_prolong_add(ppos, w, lever, yc, row, z0, z1, z2);
     const float4 zz = make_float4(z0, z1, z2, 0.0f);
     z[row] = zz;
     if (guard == 2) p[row] = zz;
@@ -954,9 +1049,12 @@ __global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ ppo
 // gathers the children's updated residual r - alpha q on the fly, and the prolongation writes
 // x += alpha p and r -= alpha q before z = D^-1 r + P y. The arithmetic is that of the separate update,
 // so the iterates are bit-identical to it.
+template <bool ROT>
 __global__ void __launch_bounds__(256) k_restrict_update(const uint32_t* __restrict__ child_ptr,
                                                          const uint32_t* __restrict__ child,
-                                                         const float* __restrict__ w, const int32_t* __restrict__ pos,
+                                                         const float* __restrict__ w,
+                                                         const float4* __restrict__ lever,
+                                                         const int32_t* __restrict__ pos,
                                                          const float4* __restrict__ r,
                                                          const float4* __restrict__ q, int Nc, float* __restrict__ rc,
                                                          const float* __restrict__ partial_pq, int G, PcgState* st,
@@ -977,31 +1075,27 @@ __global__ void __launch_bounds__(256) k_restrict_update(const uint32_t* __restr
   const int lane = threadIdx.x & 63;
   if (I >= Nc) return;
   const uint32_t t0 = child_ptr[I], n = child_ptr[I + 1] - t0;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   for (uint32_t j = lane; j < n; j += 64) {
     const uint32_t s = child[t0 + j];
     const float ww = w[s];
     const float4 v = r[s >> 2], qq = q[s >> 2];
     const float rx = v.x - alpha * qq.x, ry = v.y - alpha * qq.y, rz = v.z - alpha * qq.z;
-    a0 = a0 + ww * rx;
-    a1 = a1 + ww * ry;
-    a2 = a2 + ww * rz;
+    a[0] = a[0] + ww * rx;
+    a[1] = a[1] + ww * ry;
+    a[2] = a[2] + ww * rz;
+    if (ROT) {
+      const float3 c = lever_cross(lever[s], rx, ry, rz);
+      a[3] = a[3] + c.x;
+      a[4] = a[4] + c.y;
+      a[5] = a[5] + c.z;
+    }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    a0 = a0 + __shfl_down(a0, o, 64);
-    a1 = a1 + __shfl_down(a1, o, 64);
-    a2 = a2 + __shfl_down(a2, o, 64);
-  }
-  if (lane == 0) {
-    const int P = pos[I];
-    rc[P] = a0;
-    rc[P + 1] = a1;
-    rc[P + 2] = a2;
-  }
+  restrict_store<ROT>(a, lane, pos[I], rc);
 }
 
 __global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restrict__ ppos, const float* __restrict__ w,
+                                                        const float4* __restrict__ lever,
                                                         const float* __restrict__ yc, const float* __restrict__ dinv,
                                                         const float4* __restrict__ p, const float4* __restrict__ q,
                                                         float4* __restrict__ x, float4* __restrict__ r,
@@ -1033,16 +1127,7 @@ __global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restric
     float z0 = d[0] * rv.x + d[1] * rv.y + d[2] * rv.z;
     float z1 = d[3] * rv.x + d[4] * rv.y + d[5] * rv.z;
     float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int P = ppos[4 * row + k];
-      if (P >= 0) {
-        const float ww = w[4 * row + k];
-        z0 = z0 + ww * yc[P];
-        z1 = z1 + ww * yc[P + 1];
-        z2 = z2 + ww * yc[P + 2];
-      }
-    }
+    coarse_prolong_add(ppos, w, lever, yc, row, z0, z1, z2);
     z[row] = make_float4(z0, z1, z2, 0.0f);
     rzs = rv.x * z0 + rv.y * z1 + rv.z * z2;
     rrs = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
@@ -1085,7 +1170,20 @@ int two_level_create(Context& c, int columns) {
     for (int a = 0; a < nc; ++a)
       if (row_of[(f * b) * n + f * a] >= 0) coarse_of[b * nc + a] = Nc++;
   t->Nc = Nc;
-  t->N3 = 3 * Nc;
+  t->dof = c.rules.coarse_rotations ? 6 : 3;
+  const int dof = t->dof;
+  t->N3 = dof * Nc;
+  std::vector<int32_t> cvert(Nc);
+  for (int b = 0; b < nc; ++b)
+    for (int a = 0; a < nc; ++a)
+      if (coarse_of[b * nc + a] >= 0) cvert[coarse_of[b * nc + a]] = (f * b) * n + f * a;
+  {
+    // the coarse grid spacing of the rest shape: vertex 0 to vertex f of the first grid row
+    const float* r0 = c.rest_host.data();
+    const double dx = (double)r0[3 * f] - r0[0], dy = (double)r0[3 * f + 1] - r0[1], dz = (double)r0[3 * f + 2] - r0[2];
+    const double h = std::sqrt(dx * dx + dy * dy + dz * dz);
+    t->inv_spacing = h > 0.0 ? (float)(1.0 / h) : 1.0f;
+  }
   std::vector<int> node_b(Nc);
   for (int b = 0; b < nc; ++b)
     for (int a = 0; a < nc; ++a)
@@ -1172,7 +1270,7 @@ int two_level_create(Context& c, int columns) {
       }
     int tiles[3];
     for (int g = 0; g < 3; ++g) {
-      t->real[g] = 3 * count[g];
+      t->real[g] = dof * count[g];
       tiles[g] = (t->real[g] + TILE - 1) / TILE;
     }
     t->seg[0] = 0;
@@ -1180,15 +1278,15 @@ int two_level_create(Context& c, int columns) {
     t->seg[2] = tiles[0] + tiles[1];
     t->seg[3] = t->nt = tiles[0] + tiles[1] + tiles[2];
     t->ts = tiles[2];
-    for (int I = 0; I < Nc; ++I) pos[I] = TILE * t->seg[seg_of_row(node_b[I])] + 3 * idx[I];
+    for (int I = 0; I < Nc; ++I) pos[I] = TILE * t->seg[seg_of_row(node_b[I])] + dof * idx[I];
     auto seg_of_tile = [&](int T) { return T < t->seg[1] ? 0 : (T < t->seg[2] ? 1 : 2); };
     t->kb = 0;
     t->bstart[0] = t->seg[1];
     t->bstart[1] = t->seg[2];
     for (uint32_t key : pair_key) {
       const int I = (int)(key / (uint32_t)Nc), J = (int)(key % (uint32_t)Nc);
-      for (int q = 0; q < 9; ++q) {
-        const int R = pos[I] + q / 3, C = pos[J] + q % 3;
+      for (int q = 0; q < dof * dof; ++q) {
+        const int R = pos[I] + q / dof, C = pos[J] + q % dof;
         if (R < C) continue;
         const int sR = seg_of_tile(R / TILE), sC = seg_of_tile(C / TILE);
         if (sR == sC) t->kb = std::max(t->kb, R / TILE - C / TILE);
@@ -1218,7 +1316,7 @@ int two_level_create(Context& c, int columns) {
   {
     std::vector<int32_t> at(t->N3p, -1);
     for (int I = 0; I < Nc; ++I)
-      for (int q = 0; q < 3; ++q) at[pos[I] + q] = 3 * I + q;
+      for (int q = 0; q < dof; ++q) at[pos[I] + q] = dof * I + q;
     int n = 0;
     for (int P = 0; P < t->N3p; ++P)
       if (at[P] >= 0) t->order[n++] = at[P];
@@ -1243,6 +1341,9 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_w, t->bilinear.size());
   TL_ALLOC(t->d_wbil, t->bilinear.size());
   TL_ALLOC(t->d_fxy, fxy.size());
+  TL_ALLOC(t->d_cvert, cvert.size());
+  TL_ALLOC(t->d_adiag, (size_t)t->nt * TILE);
+  if (dof == 6) TL_ALLOC(t->d_lever, t->parent.size());
   TL_ALLOC(t->d_child_ptr, child_ptr.size());
   TL_ALLOC(t->d_child, child.size());
   TL_ALLOC(t->d_pair_ptr, pair_ptr.size());
@@ -1271,9 +1372,9 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_t, (size_t)t->N3p);
   TL_ALLOC(t->d_rc, (size_t)t->N3p);
   TL_ALLOC(t->d_yc, (size_t)t->N3p);
-  TL_ALLOC(t->d_info, 1);
+  TL_ALLOC(t->d_info, 2);
 #undef TL_ALLOC
-  if ((e = hipHostMalloc((void**)&t->h_info, sizeof(int))) != hipSuccess) return fail(e);
+  if ((e = hipHostMalloc((void**)&t->h_info, 2 * sizeof(int))) != hipSuccess) return fail(e);
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
   };
@@ -1283,6 +1384,8 @@ int two_level_create(Context& c, int columns) {
       (e = up(t->d_w, t->bilinear.data(), t->bilinear.size() * 4)) != hipSuccess ||
       (e = up(t->d_wbil, t->bilinear.data(), t->bilinear.size() * 4)) != hipSuccess ||
       (e = up(t->d_fxy, fxy.data(), fxy.size() * sizeof(float2))) != hipSuccess ||
+      (e = up(t->d_cvert, cvert.data(), cvert.size() * 4)) != hipSuccess ||
+      (e = hipMemsetAsync(t->d_info, 0, 2 * sizeof(int), s)) != hipSuccess ||
       (e = up(t->d_child_ptr, child_ptr.data(), child_ptr.size() * 4)) != hipSuccess ||
       (e = up(t->d_child, child.data(), child.size() * 4)) != hipSuccess ||
       (e = up(t->d_pair_ptr, pair_ptr.data(), pair_ptr.size() * 4)) != hipSuccess ||
@@ -1333,11 +1436,19 @@ int two_level_set_model(Context& c, const float* theta, int count) {
     return DRAPE_E_INVALID;
   }
   if (!theta) return two_level_set_weights(c, nullptr);
-  if (count != gnn_param_count()) {
-    set_error("prolongation model: expected " + std::to_string(gnn_param_count()) + " parameters");
+  if (count != gnn_param_count(false) && count != gnn_param_count(true)) {
+    set_error("prolongation model: expected " + std::to_string(gnn_param_count(false)) + " parameters (" +
+              std::to_string(gnn_param_count(true)) + " with the rotation head)");
     return DRAPE_E_INVALID;
   }
+  if (t->d_theta && t->theta_count != count) {
+    HIP_TRY(hipFree(t->d_theta));
+    t->d_theta = nullptr;
+  }
   if (!t->d_theta) HIP_TRY(hipMalloc((void**)&t->d_theta, (size_t)count * sizeof(float)));
+  t->theta_count = count;
+  t->rotation_head = count == gnn_param_count(true) && t->dof == 6;
+  if (t->rotation_head && !t->d_sigma) HIP_TRY(hipMalloc((void**)&t->d_sigma, 4 * (size_t)c.R * sizeof(float)));
   if (!t->d_gnn) {
     const size_t n = (size_t)c.R * (1 + 13 + 3 * 32) + (size_t)c.B;
     HIP_TRY(hipMalloc((void**)&t->d_gnn, n * sizeof(float)));
@@ -1376,6 +1487,7 @@ GnnArgs two_level_gnn_args(Context& c) {
   a.h1 = a.h0 + 32 * (size_t)c.R;
   a.msg = a.h1 + 32 * (size_t)c.R;
   a.w = t->d_w;
+  a.sigma = t->rotation_head ? t->d_sigma : nullptr;
   return a;
 }
 
@@ -1394,6 +1506,8 @@ static Tiles tiles_of(const TwoLevel* t) {
   for (int i = 0; i < 4; ++i) v.seg[i] = t->seg[i];
   for (int i = 0; i < 2; ++i) v.bstart[i] = t->bstart[i];
   for (int i = 0; i < 3; ++i) v.real[i] = t->real[i];
+  v.adiag = t->d_adiag;
+  v.floor = 0.0;
   return v;
 }
 
@@ -1433,7 +1547,17 @@ static int swap_saved(Context& c, bool keep) {
     HIP_TRY(hipMemsetAsync(v.BT, 0, nb * sizeof(float), c.stream));
     HIP_TRY(hipMemsetAsync(v.Bh, 0, nb * sizeof(_Float16), c.stream));
     HIP_TRY(hipMemsetAsync(v.BTh, 0, nb * sizeof(_Float16), c.stream));
+    // the prolongation travels with its factorisation; a fresh copy starts as the current one
+    HIP_TRY(hipMalloc((void**)&v.w, 4 * (size_t)c.R * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(v.w, t->d_w, 4 * (size_t)c.R * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    if (t->dof == 6) {
+      HIP_TRY(hipMalloc((void**)&v.lever, 4 * (size_t)c.R * sizeof(float4)));
+      HIP_TRY(hipMemcpyAsync(v.lever, t->d_lever, 4 * (size_t)c.R * sizeof(float4), hipMemcpyDeviceToDevice,
+                             c.stream));
+    }
   }
+  std::swap(v.w, t->d_w);
+  std::swap(v.lever, t->d_lever);
   std::swap(v.linv, t->d_linv);
   std::swap(v.B, t->d_B);
   std::swap(v.BT, t->d_BT);
@@ -1480,6 +1604,7 @@ static int adopt_pending(Context& c) {
   HIP_TRY(hipEventElapsedTime(&ms, q.began, q.ended));
   t->factor_ms += ms;
   std::swap(q.w, t->d_w);
+  std::swap(q.lever, t->d_lever);
   std::swap(q.linv, t->d_linv);
   std::swap(q.B, t->d_B);
   std::swap(q.BT, t->d_BT);
@@ -1492,6 +1617,7 @@ static int adopt_pending(Context& c) {
   t->refresh = false;
   t->solves = 0;
   t->factorisations++;
+  t->replaced += q.h_info[1];
   t->ok = *q.h_info == 0;
   if (!t->ok) t->failures++;
   q.active = false;
@@ -1526,8 +1652,9 @@ int two_level_before_assembly(Context& c) {
   return DRAPE_OK;
 }
 
-static int factorise_chain(Context& c, hipStream_t s, const float* w, double* linv, float* B, float* BT,
-                           _Float16* Bh, _Float16* BTh, bool half, hipEvent_t after_product);
+static int factorise_chain(Context& c, hipStream_t s, const float* w, const float4* lever, double* linv, float* B,
+                           float* BT, _Float16* Bh, _Float16* BTh, bool half, hipEvent_t after_product);
+static int compute_levers(Context& c, hipStream_t s, const float* w, float4* lever);
 
 // The factorisation of the assembled system on the side stream, into the pending buffers; the current one
 // keeps serving until the next system adopts it.
@@ -1540,12 +1667,13 @@ static int factorise_async(Context& c) {
     for (hipEvent_t* e : {&q.ready, &q.read, &q.done}) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     for (hipEvent_t* e : {&q.began, &q.ended}) HIP_TRY(hipEventCreate(e));
     HIP_TRY(hipMalloc((void**)&q.w, 4 * (size_t)c.R * sizeof(float)));
+    if (t->dof == 6) HIP_TRY(hipMalloc((void**)&q.lever, 4 * (size_t)c.R * sizeof(float4)));
     HIP_TRY(hipMalloc((void**)&q.linv, (size_t)t->nt * TILE * TILE * sizeof(double)));
     HIP_TRY(hipMalloc((void**)&q.B, nb * sizeof(float)));
     HIP_TRY(hipMalloc((void**)&q.BT, nb * sizeof(float)));
     HIP_TRY(hipMalloc((void**)&q.Bh, nb * sizeof(_Float16)));
     HIP_TRY(hipMalloc((void**)&q.BTh, nb * sizeof(_Float16)));
-    HIP_TRY(hipHostMalloc((void**)&q.h_info, sizeof(int)));
+    HIP_TRY(hipHostMalloc((void**)&q.h_info, 2 * sizeof(int)));
     // the copies are written below/above the diagonal only: the rest stays zero
     HIP_TRY(hipMemsetAsync(q.B, 0, nb * sizeof(float), c.stream));
     HIP_TRY(hipMemsetAsync(q.BT, 0, nb * sizeof(float), c.stream));
@@ -1591,9 +1719,10 @@ static int launch_chain(Context& c) {
     HIP_TRY(hipMemcpyAsync(q.w, t->d_w, 4 * (size_t)c.R * sizeof(float), hipMemcpyDeviceToDevice, q.side));
   }
   q.half_valid = use_half(c, t);
-  TRY_RC(factorise_chain(c, q.side, q.w, q.linv, q.B, q.BT, q.Bh, q.BTh, q.half_valid, q.read));
+  TRY_RC(compute_levers(c, q.side, q.w, q.lever));
+  TRY_RC(factorise_chain(c, q.side, q.w, q.lever, q.linv, q.B, q.BT, q.Bh, q.BTh, q.half_valid, q.read));
   HIP_TRY(hipEventRecord(q.ended, q.side));
-  HIP_TRY(hipMemcpyAsync(q.h_info, t->d_info, sizeof(int), hipMemcpyDeviceToHost, q.side));
+  HIP_TRY(hipMemcpyAsync(q.h_info, t->d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, q.side));
   HIP_TRY(hipEventRecord(q.done, q.side));
   q.galerkin = true;
   return DRAPE_OK;
@@ -1658,31 +1787,54 @@ static int factorise(Context& c, bool model, bool* ok) {
   } clock{t, started};
   if (model && t->d_theta) TRY_RC(run_model(c));
   t->half_valid = use_half(c, t);
-  TRY_RC(factorise_chain(c, s, t->d_w, t->d_linv, t->d_B, t->d_BT, t->d_Bh, t->d_BTh, t->half_valid, nullptr));
-  TRY_RC(read_back(c, t->h_info, t->d_info, sizeof(int)));
+  TRY_RC(compute_levers(c, s, t->d_w, t->d_lever));
+  TRY_RC(factorise_chain(c, s, t->d_w, t->d_lever, t->d_linv, t->d_B, t->d_BT, t->d_Bh, t->d_BTh, t->half_valid,
+                         nullptr));
+  TRY_RC(read_back(c, t->h_info, t->d_info, 2 * sizeof(int)));
   t->factorisations++;
+  t->replaced += t->h_info[1];
   t->ok = *t->h_info == 0;
   if (!t->ok) t->failures++;
   *ok = t->ok;
   return DRAPE_OK;
 }
 
-// Galerkin product with weights w, banded Cholesky and the inverse factor into (linv, B, B^T and, when
-// `half`, the FP16 copies), on stream s; `after_product` (if not null) is recorded once the product has read
-// the assembled matrix.
-static int factorise_chain(Context& c, hipStream_t s, const float* w, double* linv, float* B, float* BT,
-                           _Float16* Bh, _Float16* BTh, bool half, hipEvent_t after_product) {
+// The rotation levers of the slots for the weights w (and the network's rotation scales when it has a
+// rotation head) at the current positions, on stream s; nothing without rotation unknowns.
+static int compute_levers(Context& c, hipStream_t s, const float* w, float4* lever) {
   TwoLevel* t = c.two_level;
-  const Tiles tl = tiles_of(t);
+  if (t->dof != 6) return DRAPE_OK;
+  const int slots = 4 * c.R;
+  k_lever<<<blocks_for(slots), 256, 0, s>>>(t->d_parent, w, t->d_theta && t->rotation_head ? t->d_sigma : nullptr,
+                                            c.rvert, t->d_cvert,
+                                            c.pos, t->inv_spacing, slots, lever);
+  HIP_TRY(hipGetLastError());
+  return DRAPE_OK;
+}
+
+// Galerkin product with weights w (and rotation levers), banded Cholesky and the inverse factor into (linv, B,
+// B^T and, when `half`, the FP16 copies), on stream s; `after_product` (if not null) is recorded once the
+// product has read the assembled matrix.
+static int factorise_chain(Context& c, hipStream_t s, const float* w, const float4* lever, double* linv, float* B,
+                           float* BT, _Float16* Bh, _Float16* BTh, bool half, hipEvent_t after_product) {
+  TwoLevel* t = c.two_level;
+  Tiles tl = tiles_of(t);
+  tl.floor = c.rules.coarse_pivot_floor;
   const size_t tile_bytes = (size_t)TILE * TILE * sizeof(double);
   HIP_TRY(hipMemsetAsync(t->d_band, 0, (size_t)t->nt * (t->kb + 1) * tile_bytes, s));
   if (t->ts > 0) {
     HIP_TRY(hipMemsetAsync(t->d_border, 0, (size_t)2 * (t->kb + 1) * t->ts * tile_bytes, s));
     HIP_TRY(hipMemsetAsync(t->d_sacc, 0, (size_t)t->ts * t->ts * tile_bytes, s));
   }
-  HIP_TRY(hipMemsetAsync(t->d_info, 0, sizeof(int), s));
-  k_galerkin_band<<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, w,
-                                                        c.sell_val, c.block_dst, t->npairs, t->Nc, t->d_pos, tl);
+  HIP_TRY(hipMemsetAsync(t->d_info, 0, 2 * sizeof(int), s));
+  if (t->dof == 6)
+    k_galerkin_band<6><<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, w,
+                                                                 lever, c.sell_val, c.block_dst, t->npairs, t->Nc,
+                                                                 t->d_pos, tl);
+  else
+    k_galerkin_band<3><<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, w,
+                                                                 nullptr, c.sell_val, c.block_dst, t->npairs, t->Nc,
+                                                                 t->d_pos, tl);
   if (after_product) HIP_TRY(hipEventRecord(after_product, s));
   k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(tl, 1.0 + c.rules.coarse_shift);
   const int slots = t->kb + t->ts;
@@ -1745,8 +1897,18 @@ int two_level_galerkin(Context& c, double* out) {
   const size_t n2 = (size_t)t->N3 * t->N3;
   if (!t->d_Ac) HIP_TRY(hipMalloc((void**)&t->d_Ac, n2 * sizeof(double)));
   HIP_TRY(hipMemsetAsync(t->d_Ac, 0, n2 * sizeof(double), s));
-  k_galerkin_dense<<<blocks_for(t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
-                                                         c.sell_val, c.block_dst, t->npairs, t->Nc, t->d_Ac);
+  if (t->dof == 6) {
+    // the levers of the current weights at the current positions (the factorisation's are left as they are)
+    if (!t->d_Ac_lever) HIP_TRY(hipMalloc((void**)&t->d_Ac_lever, 4 * (size_t)c.R * sizeof(float4)));
+    TRY_RC(compute_levers(c, s, t->d_w, t->d_Ac_lever));
+    k_galerkin_dense<6><<<blocks_for(t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
+                                                              t->d_Ac_lever, c.sell_val, c.block_dst, t->npairs, t->Nc,
+                                                              t->d_Ac);
+  } else {
+    k_galerkin_dense<3><<<blocks_for(t->npairs), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, t->d_w,
+                                                              nullptr, c.sell_val, c.block_dst, t->npairs, t->Nc,
+                                                              t->d_Ac);
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, t->d_Ac, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -1770,7 +1932,26 @@ int two_level_inverse_factor(Context& c, float* out) {
   return DRAPE_OK;
 }
 
-// The factor's ordering: order[i] = 3 I + q, the natural coarse unknown at its i-th real position.
+// Unknowns per coarse node, the rotation levers of the current factorisation [4R] (x, y, z, 0 per slot; zero
+// without rotations) and the pivots replaced by the pivot floor over all factorisations. Any output may be null.
+int two_level_rotations(Context& c, int32_t* dof, float* levers, int64_t* replaced) {
+  TRY_RC(adopt_pending(c));
+  TwoLevel* t = c.two_level;
+  if (dof) *dof = t->dof;
+  if (replaced) *replaced = t->replaced;
+  if (levers) {
+    const size_t n = 4 * (size_t)c.R;
+    if (t->dof == 6) {
+      HIP_TRY(hipMemcpyAsync(levers, t->d_lever, n * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
+      HIP_TRY(hipStreamSynchronize(c.stream));
+    } else {
+      std::fill(levers, levers + 4 * n, 0.0f);
+    }
+  }
+  return DRAPE_OK;
+}
+
+// The factor's ordering: order[i] = dof I + q, the natural coarse unknown at its i-th real position.
 int two_level_ordering(Context& c, int32_t* order) {
   const TwoLevel* t = c.two_level;
   std::copy(t->order.begin(), t->order.end(), order);
@@ -1839,13 +2020,25 @@ int two_level_gemv_timing(Context& c, bool lower, int launches, double* avg_ms, 
 }
 
 // z = D^-1 r + P B^T B P^T r for the current r (guard: 0 running, 1 replacement gate, 2 setup with p = z).
+static void launch_restrict(Context& c, const TwoLevel* t, const float4* r, int guard) {
+  hipStream_t s = c.stream;
+  if (t->dof == 6)
+    k_restrict<true><<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_lever, t->d_pos, r, t->Nc,
+                                                     t->d_rc, c.state, guard);
+  else
+    k_restrict<false><<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, nullptr, t->d_pos, r, t->Nc,
+                                                      t->d_rc, c.state, guard);
+}
+
+static const float4* lever_of(const TwoLevel* t) { return t->dof == 6 ? t->d_lever : nullptr; }
+
 int two_level_apply(Context& c, float* partials, int guard) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
-  k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_pos, c.r, t->Nc, t->d_rc, c.state,
-                                             guard);
+  launch_restrict(c, t, c.r, guard);
   coarse_solve(c, t, guard, false);
-  k_prolong<<<c.G, 256, 0, s>>>(t->d_ppos, t->d_w, t->d_yc, c.dinv, c.r, c.z, c.p, partials, c.R, c.state, guard);
+  k_prolong<<<c.G, 256, 0, s>>>(t->d_ppos, t->d_w, lever_of(t), t->d_yc, c.dinv, c.r, c.z, c.p, partials, c.R, c.state,
+                                guard);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }
@@ -1853,25 +2046,30 @@ int two_level_apply(Context& c, float* partials, int guard) {
 int two_level_iterate(Context& c, const float* partial_pq, const float4* p, float* partial_rz, int it) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
-  k_restrict_update<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_pos, c.r, c.q, t->Nc,
-                                                    t->d_rc, partial_pq, c.G, c.state, it);
+  if (t->dof == 6)
+    k_restrict_update<true><<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_lever, t->d_pos,
+                                                             c.r, c.q, t->Nc, t->d_rc, partial_pq, c.G, c.state, it);
+  else
+    k_restrict_update<false><<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, nullptr, t->d_pos,
+                                                              c.r, c.q, t->Nc, t->d_rc, partial_pq, c.G, c.state, it);
   coarse_solve(c, t, 0, false);
-  k_prolong_update<<<c.G, 256, 0, s>>>(t->d_ppos, t->d_w, t->d_yc, c.dinv, p, c.q, c.x, c.r, c.z, partial_rz, c.R,
-                                       partial_pq, c.G, c.state, it);
+  k_prolong_update<<<c.G, 256, 0, s>>>(t->d_ppos, t->d_w, lever_of(t), t->d_yc, c.dinv, p, c.q, c.x, c.r, c.z,
+                                       partial_rz, c.R, partial_pq, c.G, c.state, it);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;
 }
 
+int two_level_dof(const Context& c) { return c.two_level ? c.two_level->dof : 3; }
+
 CoarseView two_level_view(const Context& c) {
   const TwoLevel* t = c.two_level;
-  return {t->d_ppos, t->d_w, t->d_yc};
+  return {t->d_ppos, t->d_w, lever_of(t), t->d_yc};
 }
 
 int two_level_restrict_solve(Context& c, const float4* r, int guard, bool fp32) {
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
-  k_restrict<<<(t->Nc + 3) / 4, 256, 0, s>>>(t->d_child_ptr, t->d_child, t->d_w, t->d_pos, r, t->Nc, t->d_rc, c.state,
-                                             guard);
+  launch_restrict(c, t, r, guard);
   coarse_solve(c, t, guard, fp32);
   HIP_TRY(hipGetLastError());
   return DRAPE_OK;

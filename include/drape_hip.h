@@ -139,6 +139,19 @@ typedef struct {
                               solved at, instead of 0 (robust-newton.ts:232), so that a run whose steps keep
                               needing regularisation does not meet the same breakdowns again at every step;
                               0 (default) is the reference's rule. No reference counterpart */
+  int32_t coarse_rotations; /* two-level: 1 gives every coarse node three rotation unknowns beside its three
+                              translations: slot k of fine row i adds the columns w_ik (e_a x v_ik), a = x, y, z,
+                              with the lever v_ik = sigma_ik (x_i - X_I) / H from the positions at the
+                              factorisation (X_I the coarse node's vertex, H the coarse grid spacing, sigma 1 or
+                              the learned prolongation's rotation scale), so P reproduces every rigid motion of
+                              the current shape; 6 unknowns per coarse node. 0 (default): translations only.
+                              No reference counterpart */
+  double coarse_pivot_floor; /* two-level: a pivot of the coarse Cholesky factorisation at most coarse_pivot_floor
+                              times its diagonal entry of A_c (near-dependent or negative-curvature coarse
+                              directions) is replaced by that entry (a modified Cholesky factorisation of
+                              A_c + E, E >= 0 diagonal), so the coarse operator stays positive definite; 0 keeps
+                              the rule that such a pivot fails the factorisation (block-Jacobi fallback).
+                              Default 0; 1e-10 with coarse_rotations */
 } drape_rules;
 
 #define DRAPE_PRECOND_BLOCK_JACOBI 0
@@ -393,17 +406,25 @@ int drape_prolongation_gradient(drape_ctx* ctx, const float* params, int32_t cou
  * their FP64 distances. Any output may be NULL (a first call with NULL arrays gives the counts). */
 int drape_self_contact_pairs(drape_ctx* ctx, int32_t* n_pt, int32_t* n_ee, int32_t* vertices, double* distances);
 
-/* The Galerkin coarse operator A_c = P^T A P of the assembled system, row-major [3Nc x 3Nc] doubles (Nc from
- * drape_two_level_layout), for checking; the preconditioner's factorisation is left as it is. */
+/* The rotation unknowns of the coarse space (rules.coarse_rotations): *dof = unknowns per coarse node (3, or 6
+ * with rotations), `levers` [4R x 4] the rotation lever (x, y, z, 0) of every parent slot of the current
+ * factorisation (zero without rotations), *replaced the pivots the pivot floor (rules.coarse_pivot_floor) has
+ * replaced over all factorisations. Any output may be NULL. No reference counterpart (the two-level
+ * preconditioner is new). */
+int drape_two_level_rotations(drape_ctx* ctx, int32_t* dof, float* levers, int64_t* replaced);
+
+/* The Galerkin coarse operator A_c = P^T A P of the assembled system, row-major [dof Nc x dof Nc] doubles (Nc
+ * from drape_two_level_layout, dof from drape_two_level_rotations; with rotations, P's levers are formed from
+ * the current weights and positions), for checking; the preconditioner's factorisation is left as it is. */
 int drape_two_level_coarse(drape_ctx* ctx, double* coarse);
 
-/* B = L^-1 (A_c = L L^T) of the current coarse factorisation as FP32, row-major [3Nc x 3Nc] (zero above the
+/* B = L^-1 (A_c = L L^T) of the current coarse factorisation as FP32, row-major [dof Nc x dof Nc] (zero above the
  * diagonal), in the factor's ordering of the coarse unknowns (drape_two_level_ordering), for checking; the
  * coarse solve applies B^T B. No reference counterpart (the two-level preconditioner is new). */
 int drape_two_level_inverse_factor(drape_ctx* ctx, float* factor);
 
-/* The factor's ordering of the coarse unknowns: order[i] = 3 I + q (coarse node I, component q) at position i
- * of drape_two_level_inverse_factor ([3Nc]), so that B^T B = (Q A_c Q^T)^-1 with Q the permutation that picks
+/* The factor's ordering of the coarse unknowns: order[i] = dof I + q (coarse node I, component q) at position i
+ * of drape_two_level_inverse_factor ([dof Nc]), so that B^T B = (Q A_c Q^T)^-1 with Q the permutation that picks
  * the rows order[0], order[1], ... No reference counterpart. */
 int drape_two_level_ordering(drape_ctx* ctx, int32_t* order);
 

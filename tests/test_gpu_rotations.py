@@ -1,0 +1,171 @@
+"""GPU tests of the coarse space with rotation unknowns (rules.coarse_rotations: three translations and three
+rotations per coarse node, the rotation levers from the positions at the factorisation) and of the pivot floor of
+the coarse Cholesky factorisation (rules.coarse_pivot_floor), through the C ABI, against oracle/two_level.py on
+the same FP32 inputs. No reference counterpart (the reference's PCG is block-Jacobi only). The bars:
+
+- the levers within 1e-6 (relative, FP32 arithmetic) of the oracle's from the same positions and weights;
+- A_c within 1e-12 of max|A_c| of the f64 triple product P^T A P with the device's levers;
+- B = L^-1 within 1e-6 of max|B| of the inverse of the oracle's modified Cholesky factor, with the same pivots
+  replaced, on a draped state (none replaced) and on the flat sheet (near-dependent rotations replaced);
+- PCG within 15% (+3) of the f64 two-level count with the same coarse space, with a true residual within 1.5x
+  of the tolerance or of the system's FP32 floor (the residual of its f64 solution rounded to FP32), whichever
+  is larger (a sheet in contact with the sphere: the barrier's stiffness puts that floor near 3e-4);
+- on the n129 PTC drape (the oracle's states at its probe steps 15 and 17), fewer PCG iterations than the
+  translations-only coarse space where the tangent is positive definite along the coarse space, and, where it
+  is not (A_c indefinite: the exact tangent has negative curvature along the coarse rotations), the breakdown
+  reported within a few iterations instead of after tens.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import drape_amd as D
+from oracle import two_level as TL
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROT = dict(preconditioner=1, coarse_rotations=1, coarse_pivot_floor=1e-10)
+G = 9.81
+
+
+def c2_state(k=-1):
+    """An n65 sheet draped over the sphere (the oracle's C2 trajectory at its probe step k) and a system at
+    h = 1/60 around it."""
+    fix = np.load(os.path.join(HERE, "golden", "c2_probes.npz"))
+    arrays = D.SceneArrays(D.drape_config_scene(65, clearance=0.05))
+    return arrays, fix["probe_start"][k].astype(np.float32), 1.0 / 60.0
+
+
+def system_at(arrays, x, h, rules, mu=0.0):
+    ctx = D.Drape(arrays, D.default_rules(**rules))
+    free = arrays.array("row_vertex").astype(np.int64)
+    x_hat = x.copy()
+    x_hat[3 * free + 2] -= np.float32(h * h * G)
+    ctx.set_state(x, x_hat, h, mu)
+    ctx.residual()
+    return ctx
+
+
+def coarse_spacing(arrays, f):
+    rest = arrays.array("rest_positions").astype(np.float32)
+    d = rest[3 * f:3 * f + 3].astype(np.float64) - rest[0:3].astype(np.float64)
+    return float(np.float32(np.sqrt(d @ d)))
+
+
+def oracle_prolongation(ctx, arrays, x, levers=None):
+    nc, parent, _, _ = ctx.two_level_layout()
+    w = ctx.two_level_weights().astype(np.float64)
+    rows = arrays.array("row_vertex").astype(np.int64)
+    f = (int(np.sqrt(arrays.V)) - 1) // (int(np.sqrt(nc)) - 1)
+    if levers is None:
+        cv = TL.coarse_vertices(int(np.sqrt(arrays.V)), rows, f)
+        levers = TL.lever_arms(parent, w, rows, cv, x.astype(np.float64), coarse_spacing(arrays, f))
+    return nc, TL.prolongation(parent, w, nc, levers), levers
+
+
+def test_levers_and_galerkin_operator_match_the_oracle():
+    arrays, x, h = c2_state()
+    ctx = system_at(arrays, x, h, ROT)
+    st, _ = ctx.solve_once(1e-2, 500)
+    dof, lev, replaced = ctx.two_level_rotations()
+    nc, _, want_lev = oracle_prolongation(ctx, arrays, x)
+    lerr = np.abs(lev - want_lev).max() / np.abs(want_lev).max()
+    Ac = ctx.two_level_coarse()
+    rhs, vals = ctx.system()
+    A = TL.bsr_matrix(arrays.array("row_ptr"), arrays.array("col_idx"), vals.astype(np.float64))
+    _, P, _ = oracle_prolongation(ctx, arrays, x, lev.astype(np.float64))
+    want = TL.galerkin(A, P)
+    err = np.abs(Ac - want).max() / np.abs(want).max()
+    print(f"[gpu] rotations n65 draped: dof {dof}, {6 * nc} coarse unknowns, lever error {lerr:.2e}, Galerkin error "
+          f"{err:.2e}, replaced pivots {replaced}, PCG {st.iterations}", flush=True)
+    assert dof == 6 and Ac.shape == (6 * nc, 6 * nc)
+    assert lerr <= 1e-6
+    assert err <= 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("where", ["draped", "flat"])
+def test_inverse_factor_matches_modified_cholesky(where):
+    """On the draped n65 state no pivot is replaced. On the flat sheet at rest the bilinear weights reproduce
+    linear fields, so within each coarse grid row the columns of the rotations about one in-plane axis sum to zero
+    (and within each coarse grid column those about the other), and the in-plane rotations once over the sheet:
+    2 x 17 + 1 = 35 dependent directions at n65, which the device replaces as the oracle's modified Cholesky does
+    (pivots below 1e-10 of their diagonal entry)."""
+    arrays, x, h = c2_state()
+    if where == "flat":
+        x = arrays.array("rest_positions").astype(np.float32)
+    ctx = system_at(arrays, x, h, ROT)
+    ctx.solve_once(1e-2, 500)
+    _, _, replaced = ctx.two_level_rotations()
+    Ac = ctx.two_level_coarse()
+    order = ctx.two_level_ordering()
+    B = ctx.two_level_inverse_factor().astype(np.float64)
+    Aq = Ac[np.ix_(order, order)]
+    L, rep = TL.modified_cholesky(0.5 * (Aq + Aq.T), 1e-10)
+    want = np.linalg.inv(L)
+    err = np.abs(B - want).max() / np.abs(want).max()
+    print(f"[gpu] rotations inverse factor n65 {where}: {len(order)} unknowns, replaced pivots GPU {replaced} oracle "
+          f"{len(rep)}, max error / max|B| {err:.2e}", flush=True)
+    assert replaced == len(rep)
+    if where == "draped":
+        assert replaced == 0
+    assert err <= 1e-6
+    assert not np.any(np.triu(B, 1))
+    ctx.close()
+
+
+@pytest.mark.parametrize("tol", [1e-2, 1e-4])
+def test_rotation_pcg_matches_f64_two_level(tol):
+    arrays, x, h = c2_state()
+    ctx = system_at(arrays, x, h, ROT)
+    rhs, vals = ctx.system()
+    A = TL.bsr_matrix(arrays.array("row_ptr"), arrays.array("col_idx"), vals.astype(np.float64))
+    b = rhs.astype(np.float64)
+    st, xs = ctx.solve_once(tol, 5000)
+    dof, lev, _ = ctx.two_level_rotations()
+    _, P, _ = oracle_prolongation(ctx, arrays, x, lev.astype(np.float64))
+    _, oit, ost, _ = TL.two_level_pcg(A, b, P, tol, 5000, pivot_floor=1e-10)
+    xf, _, _, _ = TL.two_level_pcg(A, b, P, 1e-12, 5000, pivot_floor=1e-10)
+    floor = np.linalg.norm(b - A @ xf.astype(np.float32).astype(np.float64)) / np.linalg.norm(b)
+    true_rel = np.linalg.norm(b - A @ xs.astype(np.float64)) / np.linalg.norm(b)
+    print(f"[gpu] rotations PCG n65 draped tol {tol}: {st.iterations} iterations (f64 {oit}), true relative "
+          f"residual {true_rel:.2e} (FP32 floor of the solution {floor:.2e})", flush=True)
+    assert D.PCG_STATUS[st.status] == "converged" and ost == 1
+    assert abs(st.iterations - oit) <= max(3, 0.15 * oit)
+    assert true_rel <= 1.5 * max(tol, floor)
+    ctx.close()
+
+
+def test_rotations_beat_translations_on_the_n129_drape():
+    """The oracle's n129 PTC states at its probe steps 15 (the sheet draped, the step size 1/15) and 17 (1/3.75,
+    an indefinite tangent), mu = 0: two-level PCG to 1e-4 and to the forcing tolerance 1e-2. At step 17 the
+    Galerkin operator of the rotation space has two negative eigenvalues (f64: -1.0e-3 against 2.2e3), so the
+    tangent is indefinite: the factorisation's pivot floor keeps the preconditioner positive definite, and PCG
+    meets p^T A p <= 0 at once, which the Newton loop answers by truncating or raising mu."""
+    fix = np.load(os.path.join(HERE, "golden", "ptc_probes_n129.npz"))
+    arrays = D.SceneArrays(D.drape_config_scene(129))
+    out = {}
+    for k in (2, 3):
+        step, x, h = int(fix["probe_steps"][k]), fix["probe_start"][k].astype(np.float32), float(fix["probe_h"][k])
+        for name, rules in (("translations", dict(preconditioner=1, fallback=0)), ("rotations", dict(ROT, fallback=0))):
+            for tol in (1e-2, 1e-4):
+                ctx = system_at(arrays, x, h, rules)
+                st, _ = ctx.solve_once(tol, 5000)
+                out[(step, name, tol)] = (st.iterations, D.PCG_STATUS[st.status])
+                ctx.close()
+        print(f"[gpu] n129 step {step} (h {h:.4f}): " + ", ".join(
+            f"{n} tol {t:.0e} {out[(step, n, t)][0]} {out[(step, n, t)][1]}" for n in ("translations", "rotations")
+            for t in (1e-2, 1e-4)), flush=True)
+    s15 = int(fix["probe_steps"][2])
+    s17 = int(fix["probe_steps"][3])
+    for tol in (1e-2, 1e-4):
+        # the translations-only solve breaks down at negative curvature or converges; either way it spends more
+        it_t, _ = out[(s15, "translations", tol)]
+        it_r, st_r = out[(s15, "rotations", tol)]
+        assert st_r == "converged"
+        assert it_r < it_t
+    it_r, st_r = out[(s17, "rotations", 1e-2)]
+    it_t, st_t = out[(s17, "translations", 1e-2)]
+    assert st_r == "breakdown" and it_r <= 5
+    assert st_t != "breakdown" or it_r < it_t

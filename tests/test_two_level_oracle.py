@@ -147,3 +147,70 @@ def test_vcycle_pcg_solves_with_fewer_iterations():
     assert st1 == 1 and st2 == 1
     assert np.abs(x2 - xd).max() <= 1e-7 * np.abs(xd).max()
     assert it2 < it1
+
+
+@pytest.mark.parametrize("n,pinned", [(17, False), (33, True)])
+def test_rotation_prolongation_reproduces_rigid_motions_of_the_shape(n, pinned):
+    """With levers from a curved shape x, P reproduces t + theta x (x_i - o) exactly on every row whose parents
+    are all free, from the coarse values (t + theta x (X_I - o), H theta): a lever of the wrong sign, a column
+    e_a x v written as v x e_a, or a lever without its weight fails it."""
+    rows = grid_rows(n, pinned)
+    nc, parent, w = TL.coarse_layout(n, rows)
+    cv = TL.coarse_vertices(n, rows)
+    assert len(cv) == nc
+    rng = np.random.default_rng(n)
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n)), -1).reshape(-1, 2) / (n - 1)
+    # a draped-like shape: a bump, a twist and noise
+    x = np.stack([g[:, 0] + 0.05 * np.sin(3 * g[:, 1]), g[:, 1], 0.3 * np.exp(-8 * ((g - 0.5) ** 2).sum(1))], 1)
+    x += 1e-3 * rng.standard_normal(x.shape)
+    H = 4.0 / (n - 1)
+    lev = TL.lever_arms(parent, w, rows, cv, x.ravel(), H)
+    P = TL.prolongation(parent, w, nc, lev)
+    assert P.shape == (3 * len(rows), 6 * nc)
+    t, theta, o = rng.standard_normal(3), rng.standard_normal(3), rng.standard_normal(3)
+    X = x[cv]
+    c = np.zeros((nc, 6))
+    c[:, :3] = t + np.cross(theta, X - o)
+    c[:, 3:] = H * theta
+    got = (P @ c.ravel()).reshape(-1, 3)
+    want = t + np.cross(theta, x[rows] - o)
+    full = np.abs(w.sum(1) - 1.0) < 1e-12
+    assert full.sum() > len(rows) // 2
+    np.testing.assert_allclose(got[full], want[full], rtol=0, atol=1e-12 * np.abs(want).max())
+    # the translation columns alone are the 3-unknown prolongation
+    P3 = TL.prolongation(parent, w, nc)
+    sel = np.concatenate([6 * np.arange(nc)[:, None] + np.arange(3)], 1).ravel()
+    np.testing.assert_array_equal(P[:, sel].toarray(), P3.toarray())
+
+
+def test_modified_cholesky_floor():
+    """No pivot below the floor: numpy's Cholesky factor. A Gram matrix with dependent columns and an indefinite
+    matrix: L L^T - A is diagonal, non-negative and nonzero exactly at the replaced pivots, and a rank
+    deficiency of k replaces k pivots."""
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((40, 40))
+    A = M @ M.T + 40 * np.eye(40)
+    L, rep = TL.modified_cholesky(A, 1e-10)
+    assert rep == []
+    np.testing.assert_allclose(L, np.linalg.cholesky(A), rtol=0, atol=1e-12 * np.abs(L).max())
+    V = rng.standard_normal((60, 30))
+    V[:, 7] = V[:, 2] + 2 * V[:, 5]
+    V[:, 20] = V[:, 11]
+    G = V.T @ V
+    L, rep = TL.modified_cholesky(G, 1e-10)
+    E = L @ L.T - G
+    off = E - np.diag(np.diag(E))
+    assert np.abs(off).max() <= 1e-10 * np.abs(G).max()
+    assert sorted(rep) == [7, 20]
+    d = np.diag(E)
+    assert np.all(d[rep] > 0.5 * np.diag(G)[rep])
+    assert np.abs(np.delete(d, rep)).max() <= 1e-10 * np.abs(G).max()
+    # indefinite with a positive diagonal: the second pivot 1 - 2^2 = -3 becomes 1
+    S = np.array([[1.0, 2.0, 0.5], [2.0, 1.0, 0.0], [0.5, 0.0, 3.0]])
+    L, rep = TL.modified_cholesky(S, 1e-10)
+    assert rep == [1]
+    E = L @ L.T - S
+    np.testing.assert_allclose(E, np.diag([0.0, 4.0, 0.0]), atol=1e-14)
+    # a non-positive diagonal entry is not replaced: the factorisation fails
+    with pytest.raises(np.linalg.LinAlgError):
+        TL.modified_cholesky(np.diag([1.0, -1.0]), 1e-10)
