@@ -162,7 +162,11 @@ def main() -> None:
     arrays = D.SceneArrays(scene)
 
     def context(pre, policy):
-        extra = ROTATIONS if pre == "rotations" else {}
+        if pre == "learned":
+            from drape_amd.learned import model_rules
+            extra = model_rules()
+        else:
+            extra = ROTATIONS if pre == "rotations" else {}
         c = D.Drape(arrays, D.default_rules(preconditioner=0 if pre == "block_jacobi" else 1,
                                             self_contact=1 if args.self_contact else 0, **POLICIES[policy], **extra))
         if pre == "learned":

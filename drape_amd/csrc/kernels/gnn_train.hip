@@ -8,6 +8,9 @@
 //   L = (1/n) sum_e |e - P c_e|_A^2 / |e|_A^2,   c_e = A_c^-1 P^T A e,   A_c = P^T A P.
 // Galerkin orthogonality, P^T A (e - P c_e) = 0, removes the derivative through c_e:
 //   dL/dw_ik = -(2/n) sum_e (A eps_e)_i . c_e[parent(i, k)] / |e|_A^2,   eps_e = e - P c_e.
+// With rotation unknowns, slot block [w I | -[v]x] and lever v = w sigma d: the lever's gradient is
+//   g_v = -(2/n) sum_e (A eps_e)_i x c_r / |e|_A^2   (c_r the parent's rotation unknowns),
+// which adds g_v . v / w to dL/dw (through v) and gives dL/dsigma = g_v . v / sigma for the rotation head.
 // A_c^-1 is the preconditioner's own factorisation (FP64 banded Cholesky, applied as B^T B with the FP32
 // inverse factor B = L^-1, also where the preconditioner applies the FP16 copy); the products with A are the
 // solver's SELL-64 rows.
@@ -32,6 +35,7 @@ constexpr int OUTER_ROWS = 128;                 // rows per chunk of the outer-p
 constexpr int LAYER_STRIDE = 2 * H * H + H;     // Ws, Wn, b of one layer
 constexpr int OFF_LAYERS = H * F_IN + H;        // after W0, b0
 constexpr int OFF_HEAD = OFF_LAYERS + GNN_LAYERS * LAYER_STRIDE;  // Wo [4 x H], bo [4]
+constexpr int OFF_ROT = OFF_HEAD + 4 * H + 4;                       // the rotation head: Wr [4 x H], br [4]
 
 }  // namespace
 
@@ -47,6 +51,8 @@ struct GnnTrain {
   float* d_hs = nullptr;       // [4][32R] h0..h3
   float* d_ms = nullptr;       // [3][32R] messages of the three layers
   float* d_gw = nullptr;       // [4R] dL/dw
+  float* d_gs = nullptr;       // [4R] dL/dsigma (rotation scales)
+  float* d_go = nullptr;       // [4R] dL/do (rotation head outputs)
   float* d_gz = nullptr;       // [4R] dL/dz (softmax inputs)
   float* d_gh = nullptr;       // [32R] dL/dh
   float* d_gu = nullptr;       // [32R] dL/du (pre-activation)
@@ -60,7 +66,8 @@ struct GnnTrain {
 
 void gnn_train_destroy(GnnTrain* g) {
   for (void* p : {(void*)g->d_E, (void*)g->d_q, (void*)g->d_eps, (void*)g->d_tmp, (void*)g->d_dinv, (void*)g->d_hs,
-                  (void*)g->d_ms, (void*)g->d_gw, (void*)g->d_gz, (void*)g->d_gh, (void*)g->d_gu, (void*)g->d_t,
+                  (void*)g->d_ms, (void*)g->d_gw, (void*)g->d_gs, (void*)g->d_go, (void*)g->d_gz, (void*)g->d_gh,
+                  (void*)g->d_gu, (void*)g->d_t,
                   (void*)g->d_tpos, (void*)g->d_partial, (void*)g->d_opart, (void*)g->d_grad, (void*)g->d_sums})
     if (p) (void)hipFree(p);
   delete g;
@@ -189,10 +196,14 @@ __global__ void __launch_bounds__(256) k_sum_pair(const float* __restrict__ part
   }
 }
 
-// gw[i, k] (+)= -2 / (n |e|_A^2) (A eps)_i . c[parent(i, k)]  (the first vector writes, the others add)
+// gw[i, k] (+)= -2 / (n |e|_A^2) (A eps)_i . c[parent(i, k)]  (the first vector writes, the others add); with
+// levers also the lever terms: g_v = -2 / (n |e|_A^2) (A eps)_i x c_r, gw += g_v . v / w, gs (+)= g_v . v / sigma
+// (sigma: the rotation scales, 1 when null)
 __global__ void __launch_bounds__(256) k_weight_grad(const int32_t* __restrict__ ppos, const float* __restrict__ yc,
                                                      const float4* __restrict__ aeps, const double* __restrict__ sums,
-                                                     int n, int first, float* __restrict__ gw, int R) {
+                                                     int n, int first, float* __restrict__ gw, int R,
+                                                     const float* __restrict__ w, const float4* __restrict__ lever,
+                                                     const float* __restrict__ sigma, float* __restrict__ gs) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= R) return;
   const float scale = (float)(-2.0 / ((double)n * sums[0]));
@@ -200,9 +211,37 @@ __global__ void __launch_bounds__(256) k_weight_grad(const int32_t* __restrict__
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int P = ppos[4 * i + k];
-    float v = 0.0f;
-    if (P >= 0) v = scale * (a.x * yc[P] + a.y * yc[P + 1] + a.z * yc[P + 2]);
+    float v = 0.0f, vs = 0.0f;
+    if (P >= 0) {
+      v = scale * (a.x * yc[P] + a.y * yc[P + 1] + a.z * yc[P + 2]);
+      if (lever) {
+        const float4 lv = lever[4 * i + k];
+        const float cx = yc[P + 3], cy = yc[P + 4], cz = yc[P + 5];
+        const float gx = scale * (a.y * cz - a.z * cy), gy = scale * (a.z * cx - a.x * cz),
+                    gz = scale * (a.x * cy - a.y * cx);
+        const float gv = gx * lv.x + gy * lv.y + gz * lv.z;
+        v = v + gv / w[4 * i + k];
+        vs = gv / (sigma ? sigma[4 * i + k] : 1.0f);
+      }
+    }
     gw[4 * i + k] = first ? v : gw[4 * i + k] + v;
+    if (gs) gs[4 * i + k] = first ? vs : gs[4 * i + k] + vs;
+  }
+}
+
+// The rotation head's backward: o = br + Wr h (recomputed as the forward head does), sigma = exp(clamp(o, -3, 3)),
+// go = gs sigma inside the clamp, 0 outside.
+__global__ void __launch_bounds__(256) k_rotation_back(const float* __restrict__ rot, const float* __restrict__ h,
+                                                       const float* __restrict__ gs, int R, float* __restrict__ go) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= R) return;
+  const float* hi = h + (size_t)H * i;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float o = rot[4 * H + k];
+    for (int j = 0; j < H; ++j) o = o + rot[H * k + j] * hi[j];
+    const float sg = expf(fminf(fmaxf(o, -3.0f), 3.0f));
+    go[4 * i + k] = (o > -3.0f && o < 3.0f) ? gs[4 * i + k] * sg : 0.0f;
   }
 }
 
@@ -287,8 +326,9 @@ __global__ void __launch_bounds__(256) k_outer_sum(const float* __restrict__ par
     bias[o] = (float)acc;
 }
 
-// gh[i][j] = sum_k gz[i][k] Wo[k][j]
-__global__ void __launch_bounds__(256) k_head_back(const float* __restrict__ Wo, const float* __restrict__ gz, int R,
+// gh[i][j] = sum_k gz[i][k] Wo[k][j] (+ sum_k go[i][k] Wr[k][j] with the rotation head)
+__global__ void __launch_bounds__(256) k_head_back(const float* __restrict__ Wo, const float* __restrict__ gz,
+                                                   const float* __restrict__ Wr, const float* __restrict__ go, int R,
                                                    float* __restrict__ gh) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= R * H) return;
@@ -296,6 +336,10 @@ __global__ void __launch_bounds__(256) k_head_back(const float* __restrict__ Wo,
   float acc = 0.0f;
 #pragma unroll
   for (int k = 0; k < 4; ++k) acc = acc + gz[4 * i + k] * Wo[H * k + j];
+  if (go) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc = acc + go[4 * i + k] * Wr[H * k + j];
+  }
   gh[t] = acc;
 }
 
@@ -358,13 +402,15 @@ int ensure_state(Context& c) {
   TRY_RC(alloc_n(&g->d_hs, (GNN_LAYERS + 1) * plane));
   TRY_RC(alloc_n(&g->d_ms, GNN_LAYERS * plane));
   TRY_RC(alloc_n(&g->d_gw, 4 * R));
+  TRY_RC(alloc_n(&g->d_gs, 4 * R));
+  TRY_RC(alloc_n(&g->d_go, 4 * R));
   TRY_RC(alloc_n(&g->d_gz, 4 * R));
   TRY_RC(alloc_n(&g->d_gh, plane));
   TRY_RC(alloc_n(&g->d_gu, plane));
   TRY_RC(alloc_n(&g->d_t, plane));
   TRY_RC(alloc_n(&g->d_partial, 2 * (size_t)c.G));
   TRY_RC(alloc_n(&g->d_opart, (size_t)g->chunks * H * (H + 1)));
-  TRY_RC(alloc_n(&g->d_grad, (size_t)gnn_param_count()));
+  TRY_RC(alloc_n(&g->d_grad, (size_t)gnn_param_count(true)));
   // the pattern is symmetric: block (i, j) at b has its transpose (j, i) in row j, columns sorted
   const std::vector<uint32_t>& rp = c.row_ptr_host;
   const std::vector<uint32_t>& ci = c.col_idx_host;
@@ -405,10 +451,6 @@ int ensure_vectors(Context& c, int count) {
 int check_context(Context& c) {
   if (c.linear || !c.two_level) {
     set_error("prolongation training needs a mesh context with a two-level layout");
-    return DRAPE_E_INVALID;
-  }
-  if (two_level_dof(c) != 3) {
-    set_error("prolongation training: coarse rotation unknowns are not supported by the trainer yet");
     return DRAPE_E_INVALID;
   }
   return ensure_state(c);
@@ -504,7 +546,8 @@ int gnn_train_gradient(Context& c, const float* params, int count, double* loss,
     k_eps<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.w, cv.lever, cv.yc, E, g->d_eps, R);
     k_spmv_dot<<<c.G, 256, 0, s>>>(A, g->d_eps, g->d_q, g->d_partial + c.G);
     k_sum_pair<<<1, 256, 0, s>>>(g->d_partial, c.G, g->d_sums + 2 * e);
-    k_weight_grad<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.yc, g->d_q, g->d_sums + 2 * e, n, e == 0, g->d_gw, R);
+    k_weight_grad<<<blocks_for(R), 256, 0, s>>>(cv.ppos, cv.yc, g->d_q, g->d_sums + 2 * e, n, e == 0, g->d_gw, R,
+                                                cv.w, cv.lever, a.sigma, cv.lever ? g->d_gs : nullptr);
   }
   // backward
   const float* theta = a.theta;
@@ -514,7 +557,17 @@ int gnn_train_gradient(Context& c, const float* params, int count, double* loss,
   k_softmax_back<<<blocks_for(R), 256, 0, s>>>(theta + OFF_HEAD, h_top, a.wbil, g->d_gw, R, g->d_gz);
   k_outer<4, H><<<g->chunks, 256, 0, s>>>(g->d_gz, h_top, R, g->d_opart);
   k_outer_sum<<<sum_blocks, 256, 0, s>>>(g->d_opart, g->chunks, 4, H, dg + OFF_HEAD, dg + OFF_HEAD + 4 * H);
-  k_head_back<<<blocks_for(R * H), 256, 0, s>>>(theta + OFF_HEAD, g->d_gz, R, g->d_gh);
+  const bool rot_head = a.sigma != nullptr;  // the network has a rotation head and the coarse space rotations
+  if (rot_head) {
+    const float* rot = theta + OFF_ROT;
+    k_rotation_back<<<blocks_for(R), 256, 0, s>>>(rot, h_top, g->d_gs, R, g->d_go);
+    k_outer<4, H><<<g->chunks, 256, 0, s>>>(g->d_go, h_top, R, g->d_opart);
+    k_outer_sum<<<sum_blocks, 256, 0, s>>>(g->d_opart, g->chunks, 4, H, dg + OFF_ROT, dg + OFF_ROT + 4 * H);
+  } else if (count == gnn_param_count(true)) {
+    HIP_TRY(hipMemsetAsync(dg + OFF_ROT, 0, (4 * H + 4) * sizeof(float), s));  // the head is not used
+  }
+  k_head_back<<<blocks_for(R * H), 256, 0, s>>>(theta + OFF_HEAD, g->d_gz, rot_head ? theta + OFF_ROT : nullptr,
+                                                rot_head ? g->d_go : nullptr, R, g->d_gh);
   for (int l = GNN_LAYERS - 1; l >= 0; --l) {
     const int base = OFF_LAYERS + l * LAYER_STRIDE;
     const float* h_in = g->d_hs + l * plane;

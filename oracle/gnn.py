@@ -19,9 +19,14 @@ Model (shared with the HIP inference kernels and the training script):
   edge weights c_ij = |A_ij|_F / (3 sqrt(s_i s_j)) over the off-diagonal blocks;
   h0 = relu(W0 f + b0); three layers h <- relu(Ws h_i + Wn m_i + b), m_i = sum_j c_ij h_j / (sum_j c_ij + 1e-6);
   delta = Wo h + bo (4); weights = mass_i softmax(log w_bilinear + delta) over the parent slots with a
-  non-zero bilinear weight, mass_i their bilinear sum.
+  non-zero bilinear weight, mass_i their bilinear sum; with the rotation head (a coarse space with rotation
+  unknowns, two_level.prolongation with levers), sigma = exp(clip(Wr h + br, -3, 3)) scales each slot's lever
+  v = w sigma (x_i - X_I) / H, so a zero head keeps the levers of the shape.
 Parameters are one flat float array: W0 [H x 13], b0 [H], then per layer Ws [H x H], Wn [H x H], b [H], then
-Wo [4 x H], bo [4] (row-major), H = 32.
+Wo [4 x H], bo [4], and optionally Wr [4 x H], br [4] (row-major), H = 32.
+With rotations the gradient also runs through the levers: dL/dv = -2 (A eps)_i x c_r / |e|_A^2 per slot, which
+adds dL/dv . v / w to dL/dw and gives dL/dsigma = dL/dv . v / sigma (its finite-difference pin is in
+tests/test_gnn_oracle.py).
 """
 from __future__ import annotations
 
@@ -35,8 +40,9 @@ HIDDEN = 32
 LAYERS = 3
 
 
-def param_count(hidden: int = HIDDEN) -> int:
-    return hidden * F_IN + hidden + LAYERS * (2 * hidden * hidden + hidden) + 4 * hidden + 4
+def param_count(hidden: int = HIDDEN, rotation_head: bool = False) -> int:
+    base = hidden * F_IN + hidden + LAYERS * (2 * hidden * hidden + hidden) + 4 * hidden + 4
+    return base + (4 * hidden + 4 if rotation_head else 0)
 
 
 def unpack(theta: np.ndarray, hidden: int = HIDDEN) -> dict:
@@ -57,6 +63,9 @@ def unpack(theta: np.ndarray, hidden: int = HIDDEN) -> dict:
         take(f"bl{l}", (hidden,))
     take("Wo", (4, hidden))
     take("bo", (4,))
+    if theta.size > o:
+        take("Wr", (4, hidden))
+        take("br", (4,))
     assert o == theta.size, (o, theta.size)
     return out
 
@@ -66,11 +75,14 @@ def pack(p: dict, hidden: int = HIDDEN) -> np.ndarray:
     for l in range(LAYERS):
         parts += [p[f"Ws{l}"], p[f"Wn{l}"], p[f"bl{l}"]]
     parts += [p["Wo"], p["bo"]]
+    if "Wr" in p:
+        parts += [p["Wr"], p["br"]]
     return np.concatenate([np.ravel(x) for x in parts])
 
 
-def init_params(seed: int = 0, hidden: int = HIDDEN, scale: float = 1.0) -> np.ndarray:
-    """He-style random hidden layers, zero output layer (the network starts at the bilinear weights)."""
+def init_params(seed: int = 0, hidden: int = HIDDEN, scale: float = 1.0, rotation_head: bool = False) -> np.ndarray:
+    """He-style random hidden layers, zero output layers (the network starts at the bilinear weights and, with
+    the rotation head, at the levers of the shape)."""
     rng = np.random.default_rng(seed)
     p = {"W0": rng.standard_normal((hidden, F_IN)) * scale * np.sqrt(2.0 / F_IN), "b0": np.zeros(hidden)}
     for l in range(LAYERS):
@@ -79,13 +91,17 @@ def init_params(seed: int = 0, hidden: int = HIDDEN, scale: float = 1.0) -> np.n
         p[f"bl{l}"] = np.zeros(hidden)
     p["Wo"] = np.zeros((4, hidden))
     p["bo"] = np.zeros(4)
+    if rotation_head:
+        p["Wr"] = np.zeros((4, hidden))
+        p["br"] = np.zeros(4)
     return pack(p, hidden)
 
 
 class Graph:
     """Features and edge weights of one assembled system (row_ptr, col_idx, vals [9B]) on a grid sheet."""
 
-    def __init__(self, columns, row_vertex, row_ptr, col_idx, vals):
+    def __init__(self, columns, row_vertex, row_ptr, col_idx, vals, positions=None, spacing=None):
+        """positions [3V] and the coarse spacing H give the coarse space rotation unknowns (levers from them)."""
         row_vertex = np.asarray(row_vertex, np.int64)
         row_ptr = np.asarray(row_ptr, np.int64)
         col_idx = np.asarray(col_idx, np.int64)
@@ -122,6 +138,10 @@ class Graph:
         self.A = TL.bsr_matrix(row_ptr, col_idx, blocks.reshape(-1))
         self.valid = self.w_bil > 0
         self.mass = self.w_bil.sum(axis=1)
+        self.d = None  # unweighted lever directions (x_i - X_I) / H [R, 4, 3] of a coarse space with rotations
+        if positions is not None:
+            cv = TL.coarse_vertices(columns, row_vertex)
+            self.d = TL.lever_arms(self.parent, np.where(self.valid, 1.0, 0.0), row_vertex, cv, positions, spacing)
 
 
 def forward(theta, g: Graph, hidden: int = HIDDEN, keep: bool = False):
@@ -143,10 +163,16 @@ def forward(theta, g: Graph, hidden: int = HIDDEN, keep: bool = False):
     ez = np.where(g.valid, np.exp(z - zmax), 0.0)
     soft = ez / ez.sum(axis=1, keepdims=True)
     w = g.mass[:, None] * soft
+    sigma = None
+    if "Wr" in p and g.d is not None:
+        o = h @ p["Wr"].T + p["br"]
+        sigma = np.exp(np.clip(o, -3.0, 3.0))
+        cache["o"] = o
     if keep:
         cache["soft"] = soft
+        cache["sigma"] = sigma
         return w, cache
-    return w
+    return w if sigma is None else (w, sigma)
 
 
 def smooth_vectors(g: Graph, count: int, sweeps: int = 5, omega: float = 0.6, seed: int = 0) -> np.ndarray:
@@ -159,12 +185,25 @@ def smooth_vectors(g: Graph, count: int, sweeps: int = 5, omega: float = 0.6, se
     return E
 
 
-def loss_and_weight_gradient(w: np.ndarray, g: Graph, E: np.ndarray):
+def levers(w: np.ndarray, g: Graph, sigma: np.ndarray | None = None) -> np.ndarray | None:
+    """The slot levers v = w sigma d [R, 4, 3] of a graph with rotations (None without)."""
+    if g.d is None:
+        return None
+    s = np.ones_like(w) if sigma is None else sigma
+    return (w * s)[:, :, None] * g.d
+
+
+def loss_and_weight_gradient(w: np.ndarray, g: Graph, E: np.ndarray, sigma: np.ndarray | None = None,
+                             with_sigma: bool = False):
     """L = mean_e |e - P c|_A^2 / |e|_A^2 with c = A_c^-1 P^T A e, and dL/dw [R, 4]. By Galerkin
-    orthogonality (P^T A (e - P c) = 0) the derivative through c vanishes: dL/dP = -2 (A eps) c^T / |e|_A^2."""
-    P = TL.prolongation(g.parent, w, g.coarse_rows)
+    orthogonality (P^T A (e - P c) = 0) the derivative through c vanishes: dL/dP = -2 (A eps) c^T / |e|_A^2.
+    With rotations (g.d) the levers v = w sigma d carry the gradient too; with_sigma also returns dL/dsigma."""
+    lev = levers(w, g, sigma)
+    P = TL.prolongation(g.parent, w, g.coarse_rows, lev)
+    dof = 3 if lev is None else 6
     Ac = (P.T @ (g.A @ P)).toarray()
-    L = np.linalg.cholesky(Ac)
+    # the device's factorisation: with rotations, the modified Cholesky of its pivot floor
+    L = np.linalg.cholesky(Ac) if lev is None else TL.modified_cholesky(0.5 * (Ac + Ac.T), 1e-10)[0]
     AE = g.A @ E
     rhs = P.T @ AE
     Cc = np.linalg.solve(L.T, np.linalg.solve(L, rhs))
@@ -174,26 +213,40 @@ def loss_and_weight_gradient(w: np.ndarray, g: Graph, E: np.ndarray):
     den = np.einsum("ij,ij->j", E, AE)
     loss = float(np.mean(num / den))
     grad = np.zeros_like(w)
+    gsig = np.zeros_like(w)
     R = g.R
     Aeps3 = Aeps.reshape(R, 3, -1)
+    C3 = Cc.reshape(-1, dof, Cc.shape[1])
     for k in range(4):
         I = g.parent[:, k]
         ok = I >= 0
-        cc = Cc.reshape(-1, 3, Cc.shape[1])[np.where(ok, I, 0)]
-        contrib = -2.0 * np.einsum("idn,idn->in", Aeps3, cc) / den[None, :]
-        grad[:, k] = np.where(ok, contrib.mean(axis=1), 0.0)
+        cc = C3[np.where(ok, I, 0)]
+        contrib = -2.0 * np.einsum("idn,idn->in", Aeps3, cc[:, :3]) / den[None, :]
+        gk = contrib.mean(axis=1)
+        if lev is not None:
+            gv = (-2.0 * np.cross(Aeps3, cc[:, 3:], axis=1) / den[None, None, :]).mean(axis=2)
+            gvv = np.einsum("id,id->i", gv, lev[:, k])
+            gk = gk + gvv / np.where(ok, w[:, k], 1.0)
+            s = np.ones(R) if sigma is None else sigma[:, k]
+            gsig[:, k] = np.where(ok, gvv / s, 0.0)
+        grad[:, k] = np.where(ok, gk, 0.0)
+    if with_sigma:
+        return loss, grad, gsig
     return loss, grad
 
 
 def loss(theta, g: Graph, E: np.ndarray, hidden: int = HIDDEN) -> float:
-    return loss_and_weight_gradient(forward(theta, g, hidden), g, E)[0]
+    out = forward(theta, g, hidden)
+    w, sigma = out if isinstance(out, tuple) else (out, None)
+    return loss_and_weight_gradient(w, g, E, sigma)[0]
 
 
 def backward(theta, g: Graph, E: np.ndarray, hidden: int = HIDDEN):
     """(loss, dL/dtheta) by hand: softmax, output layer, three message-passing layers, input layer."""
     p = unpack(theta, hidden)
     w, cache = forward(theta, g, hidden, keep=True)
-    L, gw = loss_and_weight_gradient(w, g, E)
+    sigma = cache["sigma"]
+    L, gw, gs = loss_and_weight_gradient(w, g, E, sigma, with_sigma=True)
     soft = cache["soft"]
     gw = np.where(g.valid, gw, 0.0)
     # w = mass softmax(z): dz_j = mass p_j (gw_j - sum_k gw_k p_k)
@@ -201,6 +254,14 @@ def backward(theta, g: Graph, E: np.ndarray, hidden: int = HIDDEN):
     gz = np.where(g.valid, gz, 0.0)
     grads = {"Wo": gz.T @ cache["h"][-1], "bo": gz.sum(axis=0)}
     gh = gz @ p["Wo"]
+    if "Wr" in p:
+        if sigma is None:  # a rotation head without a rotation coarse space: unused
+            grads["Wr"], grads["br"] = np.zeros_like(p["Wr"]), np.zeros_like(p["br"])
+        else:
+            o = cache["o"]
+            go = np.where((o > -3.0) & (o < 3.0), gs * sigma, 0.0)
+            grads["Wr"], grads["br"] = go.T @ cache["h"][-1], go.sum(axis=0)
+            gh = gh + go @ p["Wr"]
     for l in reversed(range(LAYERS)):
         gu = gh * (cache["u"][l + 1] > 0)
         h_prev, m = cache["h"][l], cache["m"][l]

@@ -92,3 +92,80 @@ def test_parameter_gradient_matches_finite_differences(graph17):
         tm[j] -= h
         fd = (G.loss(tp, g, E) - G.loss(tm, g, E)) / (2 * h)
         assert fd == pytest.approx(grad[j], rel=2e-4, abs=1e-9), j
+
+
+@pytest.fixture(scope="module")
+def graph17rot():
+    """The pinned n17 sheet bulged into a curved shape (a coarse space with rotation unknowns needs one: on the
+    flat sheet its rotation columns are dependent), assembled there at h = 0.1."""
+    od = O.Drape(O.reference_scene(17, sphere=False, placed=True))
+    a = od.problem_arrays()
+    rest = od.rest_positions()
+    x = rest.copy().reshape(-1, 3)
+    u = (x[:, 0] - x[:, 0].min()) / np.ptp(x[:, 0])
+    v = (x[:, 1] - x[:, 1].min()) / np.ptp(x[:, 1])
+    x[:, 2] += 0.04 * np.sin(np.pi * u) * np.sin(np.pi * v) + 0.01 * u * v
+    x = x.ravel()
+    x_hat = x.copy()
+    x_hat[2::3] -= 1e-3
+    od.set_time_step(0.1)
+    rhs, vals = od.assemble(x, x_hat, 0.0, True)
+    H = float(np.linalg.norm(rest[12:15] - rest[0:3]))   # vertex 0 to vertex 4 of the first grid row
+    return G.Graph(17, a["row_vertex"], a["row_ptr"], a["col_idx"], vals, positions=x, spacing=H)
+
+
+def test_zero_rotation_head_keeps_the_shape_levers(graph17rot):
+    g = graph17rot
+    theta = G.init_params(seed=3, rotation_head=True)
+    assert theta.size == G.param_count(rotation_head=True) == G.param_count() + 4 * 32 + 4
+    w, sigma = G.forward(theta, g)
+    assert np.allclose(w, g.w_bil, rtol=0, atol=1e-15)
+    assert np.array_equal(sigma, np.ones_like(sigma))
+    lev = G.levers(w, g, sigma)
+    np.testing.assert_allclose(lev, g.w_bil[:, :, None] * g.d, rtol=0, atol=1e-15)
+
+
+def test_weight_and_rotation_scale_gradients_match_finite_differences(graph17rot):
+    """dL/dw through both the translation block and the lever, and dL/dsigma through the lever."""
+    g = graph17rot
+    E = G.smooth_vectors(g, 3, seed=2)
+    w = g.w_bil.copy()
+    rng = np.random.default_rng(4)
+    sigma = np.exp(0.2 * rng.standard_normal(w.shape))
+    L0, gw, gs = G.loss_and_weight_gradient(w, g, E, sigma, with_sigma=True)
+    checked = 0
+    for _ in range(12):
+        i, k = rng.integers(g.R), rng.integers(4)
+        if not g.valid[i, k]:
+            continue
+        h = 1e-6
+        for arr, grad in ((w, gw), (sigma, gs)):
+            p_, m_ = arr.copy(), arr.copy()
+            p_[i, k] += h
+            m_[i, k] -= h
+            args_p = (p_, sigma) if arr is w else (w, p_)
+            args_m = (m_, sigma) if arr is w else (w, m_)
+            fd = (G.loss_and_weight_gradient(args_p[0], g, E, args_p[1])[0] -
+                  G.loss_and_weight_gradient(args_m[0], g, E, args_m[1])[0]) / (2 * h)
+            assert fd == pytest.approx(grad[i, k], rel=1e-5, abs=1e-10)
+        checked += 1
+    assert checked >= 4
+
+
+def test_parameter_gradient_with_rotation_head_matches_finite_differences(graph17rot):
+    g = graph17rot
+    E = G.smooth_vectors(g, 2, seed=5)
+    theta = G.init_params(seed=7, rotation_head=True)
+    rng = np.random.default_rng(8)
+    theta = theta + 0.05 * rng.standard_normal(theta.size)
+    L, grad = G.backward(theta, g, E)
+    n0 = G.param_count()
+    picks = np.concatenate([rng.choice(n0, 8, replace=False), n0 + rng.choice(4 * 32 + 4, 8, replace=False),
+                            [theta.size - 1, n0 - 1, 0]])
+    for j in picks:
+        h = 1e-6
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd = (G.loss(tp, g, E) - G.loss(tm, g, E)) / (2 * h)
+        assert fd == pytest.approx(grad[j], rel=2e-4, abs=1e-9), j

@@ -21,7 +21,7 @@ import numpy as np
 import pytest
 
 import drape_amd as D
-from drape_amd.learned import load_model
+from drape_amd.learned import load_model, model_rules
 
 pytestmark = pytest.mark.gpu
 FIX = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_probes.npz"))
@@ -95,7 +95,8 @@ def test_c2_probes_match_the_oracle_step(preconditioner, oracle):
     relative to the step's scale; newton_ok's FLOOR_FACTOR). Both are printed per probe."""
     od, free, h, g = oracle
     arrays = scene_arrays()
-    ctx = D.Drape(arrays, D.default_rules(preconditioner=0 if preconditioner == "block_jacobi" else 1))
+    ctx = D.Drape(arrays, D.default_rules(**(dict(preconditioner=0) if preconditioner == "block_jacobi" else
+                                             dict(preconditioner=1, **model_rules()))))
     if preconditioner == "learned":
         ctx.set_prolongation_model(load_model())
     results = []

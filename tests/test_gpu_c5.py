@@ -25,7 +25,7 @@ import numpy as np
 import pytest
 
 import drape_amd as D
-from drape_amd.learned import load_model
+from drape_amd.learned import load_model, model_rules
 import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
@@ -48,7 +48,7 @@ def c5():
 
 def test_c5_drape_with_the_learned_preconditioner_zero_shot(c5):
     arrays, od = c5
-    ctx = D.Drape(arrays, D.default_rules(preconditioner=1))
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=1, **model_rules()))
     ctx.set_prolongation_model(load_model())
     print("[gpu] C5: n257 irregular sphere drape, learned two-level zero-shot, 25 PTC steps", flush=True)
     out = ctx.solve_equilibrium(start=arrays.array("rest_positions"), cap=25, keep_states=True)
@@ -73,7 +73,7 @@ def test_c5_drape_with_the_learned_preconditioner_zero_shot(c5):
     bj = D.Drape(arrays, D.default_rules(preconditioner=0))
     out_bj = bj.solve_equilibrium(start=arrays.array("rest_positions"), cap=6)
     rb = out_bj["result"]
-    lt = D.Drape(arrays, D.default_rules(preconditioner=1))
+    lt = D.Drape(arrays, D.default_rules(preconditioner=1, **model_rules()))
     lt.set_prolongation_model(load_model())
     out_lt = lt.solve_equilibrium(start=arrays.array("rest_positions"), cap=6)
     rl = out_lt["result"]
@@ -105,7 +105,8 @@ def test_c5_probes_match_the_oracle_step(c5, leg):
     starts = probe_starts(od.rest_positions())
     free = od.problem_arrays()["row_vertex"].astype(np.int64)
     g = od.scene.gravity
-    ctx = D.Drape(arrays, D.default_rules(preconditioner=1 if leg == "learned" else 0))
+    ctx = D.Drape(arrays, D.default_rules(**(dict(preconditioner=1, **model_rules()) if leg == "learned" else
+                                             dict(preconditioner=0))))
     if leg == "learned":
         ctx.set_prolongation_model(load_model())
     strict = 0

@@ -66,6 +66,51 @@ def test_device_loss_and_gradient_match_f64_backward(columns):
     ctx.close()
 
 
+def test_device_gradient_with_rotations_matches_f64_backward():
+    """The coarse space with rotation unknowns (rules.coarse_rotations) and a network with the rotation head, on
+    the n65 sheet draped over the sphere (the oracle's C2 state at its last probe step, h = 1/60): the loss and the
+    gradient through the weights, the levers and the rotation head against the f64 backward with the same levers'
+    geometry, at the bars above."""
+    import os
+    fix = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_probes.npz"))
+    arrays = D.SceneArrays(D.drape_config_scene(65, clearance=0.05))
+    ctx = D.Drape(arrays, D.default_rules(preconditioner=1, coarse_rotations=1, coarse_pivot_floor=1e-10))
+    x = fix["probe_start"][-1].astype(np.float32)
+    free = arrays.array("row_vertex").astype(np.int64)
+    x_hat = x.copy()
+    x_hat[3 * free + 2] -= np.float32(9.81 / 3600.0)
+    ctx.set_state(x, x_hat, 1.0 / 60.0, 0.0)
+    ctx.residual()
+    _, vals = ctx.system()
+    rest = arrays.array("rest_positions").astype(np.float64)
+    H = float(np.float32(np.linalg.norm(rest[12:15] - rest[0:3])))
+    g = G.Graph(65, free, arrays.array("row_ptr"), arrays.array("col_idx"), vals.astype(np.float64),
+                positions=x.astype(np.float64), spacing=H)
+    E = G.smooth_vectors(g, 6, seed=3).T.astype(np.float32)
+    ctx.training_vectors(E)
+    theta = G.init_params(seed=5, rotation_head=True)
+    rng = np.random.default_rng(6)
+    theta = (theta + 0.1 * rng.standard_normal(theta.size)).astype(np.float32)
+    loss, grad = ctx.prolongation_gradient(theta)
+    _, _, replaced = ctx.two_level_rotations()
+    L_ref, g_ref = G.backward(theta.astype(np.float64), g, E.T.astype(np.float64))
+    rel_loss = abs(loss - L_ref) / L_ref
+    rel = np.linalg.norm(grad - g_ref) / np.linalg.norm(g_ref)
+    gd, gr = G.unpack(grad), G.unpack(g_ref)
+    per = {k: np.linalg.norm(gd[k] - gr[k]) / max(np.linalg.norm(gr[k]), 1e-30) for k in GROUPS + ["Wr", "br"]}
+    worst = max(per, key=per.get)
+    print(f"[gpu] prolongation gradient with rotations n65 draped: loss {loss:.6f} (f64 {L_ref:.6f}, rel "
+          f"{rel_loss:.1e}); gradient rel L2 {rel:.1e}, worst group {worst} {per[worst]:.1e}; rotation head "
+          f"{per['Wr']:.1e} / {per['br']:.1e}; replaced pivots {replaced}", flush=True)
+    assert replaced == 0
+    assert rel_loss <= 1e-6
+    assert rel <= 1e-5
+    assert per[worst] <= 1e-4
+    loss2, grad2 = ctx.prolongation_gradient(theta)
+    assert loss2 == loss and np.array_equal(grad2.view(np.uint32), grad.view(np.uint32))
+    ctx.close()
+
+
 def test_device_vectors_and_sweep():
     ctx, g = assembled(33)
     E0 = ctx.training_vectors(count=4, sweeps=0, seed=7)

@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 import drape_amd as D
-from drape_amd.learned import load_model
+from drape_amd.learned import load_model, model_rules
 import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
@@ -46,7 +46,10 @@ def scene():
 
 
 def context(arrays, leg):
-    ctx = D.Drape(arrays, D.default_rules(**(CARRY if leg == "learned_carry" else LEGS[leg])))
+    rules = dict(CARRY if leg == "learned_carry" else LEGS[leg])
+    if leg.startswith("learned"):
+        rules.update(model_rules())
+    ctx = D.Drape(arrays, D.default_rules(**rules))
     if leg.startswith("learned"):
         ctx.set_prolongation_model(load_model())
     return ctx
@@ -139,7 +142,7 @@ def test_mu_carry_starts_each_step_where_the_last_one_needed_it(scene):
     arrays, od, fix = scene
     retries = {}
     for carry in (0, 1):
-        ctx = D.Drape(arrays, D.default_rules(**dict(CARRY, mu_carry=carry)))
+        ctx = D.Drape(arrays, D.default_rules(**dict(CARRY, mu_carry=carry, **model_rules())))
         ctx.set_prolongation_model(load_model())
         out = ctx.solve_equilibrium(start=arrays.array("rest_positions"), cap=12)
         firsts = [r for r in out["records"] if r.iteration == 0]

