@@ -39,10 +39,13 @@ import drape_amd as D  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by the same counters of a
-# known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part), keyed by (kernel, grid columns); the
-# summaries are under profiles/. (At n129 the bytes are L2 misses served mostly by the Infinity Cache.)
-TRAFFIC = {("k_pcg_spmv", 1025): 523_831_470, ("k_pcg_spmv", 129): 8_638_625, ("k_pcg_dir_spmv", 129): 9_762_528,
-           ("k_tri_gemv_lower", 129): 12_372_435, ("k_tri_gemv_upper", 129): 13_493_029}
+# known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part), keyed by (kernel, grid columns, coarse
+# unknowns; 0 for the SpMVs); the summaries are under profiles/. (At n129 the bytes are L2 misses served mostly
+# by the Infinity Cache.)
+TRAFFIC = {("k_pcg_spmv", 1025, 0): 523_831_470, ("k_pcg_spmv", 129, 0): 8_638_625,
+           ("k_pcg_dir_spmv", 129, 0): 9_762_528,
+           ("k_tri_gemv_lower", 129, 3267): 12_372_435, ("k_tri_gemv_upper", 129, 3267): 13_493_029,
+           ("k_tri_gemv_lower", 129, 6534): 48_591_794, ("k_tri_gemv_upper", 129, 6534): 50_698_324}
 
 
 def spmv_bytes(rows: int, blocks: int) -> int:
@@ -63,7 +66,7 @@ def large_grid_spmv(columns: int, launches: int = 50) -> dict:
     nbytes = spmv_bytes(arrays.R, arrays.B)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(("k_pcg_spmv", columns)), "kernel": "k_pcg_spmv",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(("k_pcg_spmv", columns, 0)), "kernel": "k_pcg_spmv",
            "avg_kernel_us": round(1000.0 * spmv_ms, 3), "algorithmic_bytes": nbytes,
            "workload": f"n{columns} ({arrays.R} rows, {arrays.B} blocks)"}
     ctx.close()
@@ -121,8 +124,9 @@ POLICIES = {"truncate": {"fallback": 2, "block_jacobi_truncate": 1, "mu_carry": 
 def kernel_roofline(ctx, kernel: str, launches: int, note: str) -> dict:
     ms, nbytes = ctx.kernel_timing(kernel, launches)
     achieved = nbytes / (ms * 1e-3) / 1e9
+    unknowns = ctx.coarse_unknowns() if kernel.startswith("k_tri_gemv") else 0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get((kernel, ctx.columns)),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get((kernel, ctx.columns, unknowns)),
             "kernel": kernel, "avg_kernel_us": round(1000.0 * ms, 3), "algorithmic_bytes": int(nbytes),
             "launches": launches, "note": note}
 
@@ -246,7 +250,13 @@ def main() -> None:
 
     # rooflines of the PCG iteration's kernels on the last assembled system of the measured leg, timed with
     # HIP events on the context's stream
-    cache = "n129 working set (matrix 7.7 MB, coarse inverse 2 x 12.0 MB in the split ordering) is resident in the 256 MB Infinity Cache"
+    if args.preconditioner != "block_jacobi":
+        n = ctx.coarse_unknowns()
+        gemv_mb = ctx.kernel_timing("k_tri_gemv_lower", 1)[1] / 1e6   # the stored triangle of B, one GEMV's bytes
+        cache = (f"n{args.columns} working set (matrix 7.7 MB, coarse inverse factor B and B^T over {n} coarse "
+                 f"unknowns, {gemv_mb:.1f} MB each in the split ordering) is resident in the 256 MB Infinity Cache")
+    else:
+        cache = f"n{args.columns} working set (matrix 7.7 MB) is resident in the 256 MB Infinity Cache"
     kernels = [kernel_roofline(ctx, "k_pcg_dir_spmv", 400, "direction update fused into the SpMV, every "
                                "iteration after a chunk's first; " + cache),
                kernel_roofline(ctx, "k_pcg_spmv", 400, "plain SpMV, the first iteration of a chunk; " + cache)]
