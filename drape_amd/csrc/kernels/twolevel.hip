@@ -530,6 +530,7 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
   __shared__ double blk[16];
   __shared__ double colbuf[4 * TILE];
   __shared__ double rdiag[TILE];
+  __shared__ double adg[TILE];  // the panel's diagonal of A_c for the pivot floor (read in the pivot chain)
   __shared__ int fail;
   if (*info != 0) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -540,6 +541,7 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
     const int prev = k - 1 >= t.seg[seg_of(t, k)] ? k - 1 : -1;
     double* tile = band_tile(t, k, 0);
     double* Ls = sc;
+    if (t.floor > 0.0 && tid < TILE) adg[tid] = t.adiag[TILE * k + tid];  // visible after the next barrier
     if (prev >= 0) {
       tiles_to_lds_step(linv + ((size_t)prev << 12), sa, a_tile(t, k, prev), sb, tile, sc);
       __syncthreads();
@@ -598,7 +600,7 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
             for (int m = 0; m < v; ++m) piv = fma(-l[v][m], l[v][m], piv);
             if (t.floor > 0.0) {
               // modified Cholesky: a collapsed or negative pivot becomes the column's diagonal entry of A_c
-              const double a = t.adiag[TILE * k + c0 + q + v];
+              const double a = adg[c0 + q + v];
               if (!(piv > t.floor * a) && piv > -INFINITY && a > 0.0 && a < INFINITY) {
                 piv = a;
                 ++nrep;
