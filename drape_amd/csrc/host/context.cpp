@@ -49,7 +49,7 @@ static void free_context(Context* c) {
   c->two_level = nullptr;
   void* ptrs[] = {c->pos, c->xhat, c->dir, c->tri, c->geom0, c->geom1, c->hin, c->hstiff, c->rvert, c->rmass,
                   c->colliders, c->mu, c->unit_mass_dev, c->org, c->material, c->lgrad, c->lhess, c->bc_ptr,
-                  c->bc_idx, c->fc_ptr, c->fc_idx, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->perm, c->diag_slot,
+                  c->bc_idx, c->fc_ptr, c->fc_idx, c->rec_pos, c->kind_ptr, c->slice_ptr, c->sell_col, c->sell_val, c->row_cnt, c->perm, c->diag_slot,
                   c->block_dst, c->csr_row_ptr, c->csr_col, c->b, c->x, c->r, c->z, c->p, c->p2, c->q, c->dinv,
                   c->partials, c->state, c->refine, c->b_saved, c->z1, c->r1, c->vel, c->dpart, c->dout, c->bounds, c->dists, c->fpart, c->ipart};
   for (void* p : ptrs)
@@ -621,7 +621,7 @@ int drape_eval_elements(drape_ctx* p, const float* positions, float* grad, float
   Context* c = ctx_of(p);
   if (!c || c->linear) { set_error("not a mesh context"); return DRAPE_E_INVALID; }
   if (positions) TRY(upload_positions(*c, c->pos, positions));
-  TRY(launch_evaluate(*c, c->rules.exact_tangent != 0));
+  TRY(launch_evaluate(*c, c->rules.exact_tangent != 0, true));  // the reference's element-local layout
   const size_t ng = 9 * (size_t)c->T + 12 * (size_t)c->H + 3 * (size_t)c->R;
   const size_t nh = 81 * (size_t)c->T + 144 * (size_t)c->H + 9 * (size_t)c->R;
   if (grad) HIP_TRY(hipMemcpyAsync(grad, c->lgrad, ng * sizeof(float), hipMemcpyDeviceToHost, c->stream));

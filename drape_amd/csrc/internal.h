@@ -14,6 +14,7 @@ namespace drape {
 constexpr int WG = 256;          // rows per workgroup and per reduction group (pcg.rs:216 WORKGROUP)
 constexpr int SLICE = 64;        // rows per sliced-BSR slice: one wavefront
 constexpr int CHUNK_PLAIN = 32;  // PCG iterations between readbacks without replacement (solver.ts:145-149)
+constexpr uint32_t NO_SLOT = 0xffffffffu;
 
 // Scalar CG state on the device (state.wgsl:4-9), rz double-buffered by iteration parity so that every
 // workgroup of a kernel reads the previous value while workgroup 0 writes the next one.
@@ -73,6 +74,17 @@ struct Context {
   uint32_t* bc_idx = nullptr;
   uint32_t* fc_ptr = nullptr;
   uint32_t* fc_idx = nullptr;
+  // The block-sorted layout of the element-local Hessian blocks for the Newton path: three regions, the
+  // membrane blocks, then the hinge blocks (from hinge0), then the row blocks (from row0), each in bc_idx order
+  // (by BSR block, then by list position). rec_pos: the slot of every block (record id as in bc_idx: 9e + 3k + l,
+  // 9T + 16h + 4k + l, 9T + 16H + row), NO_SLOT for a block in no BSR block (a pinned vertex's); kind_ptr
+  // [3 (B + 1)]: per BSR block the first slot of its row, membrane and hinge blocks within their regions. The
+  // element kernels write each block to its slot, so each kernel fills whole lines of its own region and the
+  // assembly reads each BSR block's three runs contiguously, in list order (row, membranes, hinges).
+  uint32_t* rec_pos = nullptr;
+  uint32_t* kind_ptr = nullptr;
+  uint32_t hinge0 = 0, row0 = 0;
+  bool hess_sorted = false;       // lhess holds the block-sorted records (else the reference's element layout)
 
   // sliced BSR (SELL-64): per slice pointer (in block slots of 64 lanes) and width
   uint32_t* slice_ptr = nullptr;  // [S+1]
@@ -189,7 +201,9 @@ int build_maps_device(Context& c, const std::vector<uint32_t>& row_ptr, const st
                       const std::vector<int>& row_of);
 
 // ---- kernel launchers (elements.hip) --------------------------------------------------------
-int launch_evaluate(Context& c, bool exact);        // element + row kernels into lgrad/lhess
+// element + row kernels into lgrad/lhess: lhess in the block-sorted order of the assembly, or with `reference`
+// in the reference's element-local layout (81T + 144H + 9R, for the kernel checks)
+int launch_evaluate(Context& c, bool exact, bool reference = false);
 int launch_assemble(Context& c);                    // scatter maps -> sell_val and b
 int launch_rhs_norm(Context& c, double* norm2);     // FP64 |b|^2 (synchronises)
 int launch_slope(Context& c, double* slope);        // FP64 -sum b.x (synchronises)

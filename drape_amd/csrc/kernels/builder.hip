@@ -329,6 +329,45 @@ __device__ inline uint32_t block_of(const uint32_t* __restrict__ row_ptr, const 
   return (b < b1 && col_idx[b] == (uint32_t)c) ? b : NO_ROW;
 }
 
+// Kind flags of the list entries (membrane record < m_end <= hinge record < h_end <= row record), with a zero
+// entry at n so that the exclusive sums end with the totals.
+__global__ void __launch_bounds__(256) k_kind_flags(const uint32_t* __restrict__ bc_idx, uint32_t n, uint32_t m_end,
+                                                    uint32_t h_end, uint32_t* __restrict__ fm,
+                                                    uint32_t* __restrict__ fh, uint32_t* __restrict__ fr) {
+  const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (j > n) return;
+  const uint32_t rec = j < n ? bc_idx[j] : 0;
+  fm[j] = j < n && rec < m_end;
+  fh[j] = j < n && rec >= m_end && rec < h_end;
+  fr[j] = j < n && rec >= h_end;
+}
+
+// rec_pos[bc_idx[j]] = the slot of entry j in its kind's region (membranes from 0, hinges from hinge0, rows
+// from row0): its rank among the entries of its kind before it
+__global__ void __launch_bounds__(256) k_record_positions(const uint32_t* __restrict__ bc_idx, uint32_t n,
+                                                          uint32_t m_end, uint32_t h_end,
+                                                          const uint32_t* __restrict__ sm,
+                                                          const uint32_t* __restrict__ sh,
+                                                          const uint32_t* __restrict__ sr, uint32_t hinge0,
+                                                          uint32_t row0, uint32_t* __restrict__ rec_pos) {
+  const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t rec = bc_idx[j];
+  rec_pos[rec] = rec < m_end ? sm[j] : (rec < h_end ? hinge0 + sh[j] : row0 + sr[j]);
+}
+
+// kind_ptr[3 b + (0, 1, 2)] = the first row, membrane and hinge slot of block b in their regions (b = 0 .. B)
+__global__ void __launch_bounds__(256) k_kind_ptrs(const uint32_t* __restrict__ bc_ptr, int B,
+                                                   const uint32_t* __restrict__ sm, const uint32_t* __restrict__ sh,
+                                                   const uint32_t* __restrict__ sr, uint32_t* __restrict__ kind_ptr) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b > B) return;
+  const uint32_t p = bc_ptr[b];
+  kind_ptr[3 * (size_t)b] = sr[p];
+  kind_ptr[3 * (size_t)b + 1] = sm[p];
+  kind_ptr[3 * (size_t)b + 2] = sh[p];
+}
+
 __global__ void __launch_bounds__(256) k_map_pairs(const int4* __restrict__ tri, const int4* __restrict__ hin,
                                                    const int32_t* __restrict__ row_of, const uint32_t* __restrict__ row_ptr,
                                                    const uint32_t* __restrict__ col_idx, int T, int H, int R,
@@ -640,7 +679,32 @@ int build_maps_device(Context& c, const std::vector<uint32_t>& row_ptr_h, const 
   HIP_TRY(hipMalloc((void**)&c.fc_idx, std::max<size_t>(nfc, 1) * sizeof(uint32_t)));
   HIP_TRY(hipMemcpyAsync(c.bc_idx, bci, (size_t)nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(c.fc_idx, fci, (size_t)nfc * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  // the block-sorted layout: every record's slot in its kind's region (each record is in at most one list) and
+  // each block's first slot per kind
+  {
+    const uint32_t m_end = 9 * (uint32_t)T, h_end = m_end + 16 * (uint32_t)H;
+    uint32_t *fm = nullptr, *fh = nullptr, *fr = nullptr, *sm = nullptr, *sh = nullptr, *sr = nullptr;
+    for (uint32_t** q : {&fm, &fh, &fr, &sm, &sh, &sr}) HIP_TRY(pool.get(q, (size_t)nb + 1));
+    k_kind_flags<<<grid_of((size_t)nb + 1), 256, 0, s>>>(c.bc_idx, nb, m_end, h_end, fm, fh, fr);
+    TRY_RC(exclusive_sum(pool, fm, sm, (int)nb + 1, s));
+    TRY_RC(exclusive_sum(pool, fh, sh, (int)nb + 1, s));
+    TRY_RC(exclusive_sum(pool, fr, sr, (int)nb + 1, s));
+    uint32_t totals[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&totals[0], sm + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&totals[1], sh + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c.hinge0 = totals[0];
+    c.row0 = totals[0] + totals[1];
+    HIP_TRY(hipMalloc((void**)&c.rec_pos, np * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc((void**)&c.kind_ptr, 3 * ((size_t)c.B + 1) * sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(c.rec_pos, 0xff, np * sizeof(uint32_t), s));
+    if (nb > 0)
+      k_record_positions<<<grid_of(nb), 256, 0, s>>>(c.bc_idx, nb, m_end, h_end, sm, sh, sr, c.hinge0, c.row0,
+                                                     c.rec_pos);
+    k_kind_ptrs<<<grid_of((size_t)c.B + 1), 256, 0, s>>>(c.bc_ptr, c.B, sm, sh, sr, c.kind_ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+  }
   c.block_contribs = nb;
   c.force_contribs = nfc;
   return DRAPE_OK;

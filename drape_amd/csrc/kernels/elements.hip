@@ -44,11 +44,32 @@ __device__ inline void clamp_negative_stress(float& a, float& c, float& b) {
   b = high * v0 * v1;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: workgroup w runs on XCD w % 8.
+// This bijection of [0, groups) gives each XCD one contiguous run of logical workgroups, so rows that share
+// elements (a vertex and its neighbours one grid row away) gather their local blocks through the same L2, and
+// the elements of one run of grid rows write their blocks through one L2.
+constexpr int XCDS = 8;
+__device__ inline int xcd_group(int groups) {
+  const int w = blockIdx.x, x = w % XCDS, k = w / XCDS;
+  const int q = groups / XCDS, r = groups % XCDS;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// The 9 floats of element-local Hessian block `record` (9e + 3k + l, 9T + 16h + 4k + l or 9T + 16H + row):
+// at hess + 9 record in the reference's layout (rec_pos null), else at its slot in the block-sorted order
+// (rec_pos[record]; nothing for a block that belongs to no BSR block).
+__device__ inline float* block_out(float* hess, const uint32_t* __restrict__ rec_pos, size_t record) {
+  if (!rec_pos) return hess + 9 * record;
+  const uint32_t slot = rec_pos[record];
+  return slot == NO_SLOT ? nullptr : hess + 9 * (size_t)slot;
+}
+
 __global__ void __launch_bounds__(256) k_membrane(const int4* __restrict__ tri, const float4* __restrict__ geom0,
                                                   const float4* __restrict__ geom1, const float4* __restrict__ pos,
                                                   const float* __restrict__ mat, int T, int exact,
-                                                  float* __restrict__ grad, float* __restrict__ hess) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+                                                  float* __restrict__ grad, float* __restrict__ hess,
+                                                  const uint32_t* __restrict__ rec_pos) {
+  const int e = xcd_group(gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= T) return;
   const int4 v = tri[e];
   const float4 ga = geom0[e], gb = geom1[e];
@@ -77,13 +98,13 @@ __global__ void __launch_bounds__(256) k_membrane(const int4* __restrict__ tri, 
     cc[k] = fc * gy[k];
     sv[k] = (fw * gy[k] + fc * gx[k]) * 0.5f;
   }
-  float* h = hess + 81 * (size_t)e;
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       const float geo = gx[k] * (a * gx[l] + b * gy[l]) + gy[k] * (b * gx[l] + c * gy[l]);
-      float* blk = h + 9 * (3 * k + l);
+      float* blk = block_out(hess, rec_pos, 9 * (size_t)e + 3 * k + l);
+      if (!blk) continue;
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -163,10 +184,11 @@ constexpr int HINGE_GSTRIDE = 13;   // 12 floats + 1
 
 __global__ void __launch_bounds__(HINGE_WG) k_hinge(const int4* __restrict__ hin, const float* __restrict__ stiff,
                                                     const float4* __restrict__ pos, int H, int exact,
-                                                    float* __restrict__ grad, float* __restrict__ hess) {
+                                                    float* __restrict__ grad, float* __restrict__ hess,
+                                                    const uint32_t* __restrict__ rec_pos, size_t first_record) {
   __shared__ float stage_h[HINGE_WG * HINGE_HSTRIDE];
   __shared__ float stage_g[HINGE_WG * HINGE_GSTRIDE];
-  const int h0 = blockIdx.x * HINGE_WG;
+  const int h0 = xcd_group(gridDim.x) * HINGE_WG;
   const int h = h0 + threadIdx.x;
   if (h < H) {
     const int4 v = hin[h];
@@ -214,8 +236,21 @@ __global__ void __launch_bounds__(HINGE_WG) k_hinge(const int4* __restrict__ hin
   }
   __syncthreads();
   const int count = H - h0 < HINGE_WG ? H - h0 : HINGE_WG;
-  float* hout = hess + 144 * (size_t)h0;
-  for (int i = threadIdx.x; i < 144 * count; i += HINGE_WG) hout[i] = stage_h[(i / 144) * HINGE_HSTRIDE + i % 144];
+  if (rec_pos) {
+    // block-sorted: lane j writes the 9 floats of records j, j + 64, ... (16 per hinge) at their slots (writing
+    // float t from lane t % 64, seven blocks per store instruction, was slower: 37 against 26 us at n129)
+    for (int i = threadIdx.x; i < 16 * count; i += HINGE_WG) {
+      const uint32_t slot = rec_pos[first_record + 16 * (size_t)h0 + i];
+      if (slot == NO_SLOT) continue;
+      const float* src = stage_h + (i / 16) * HINGE_HSTRIDE + 9 * (i % 16);
+      float* dst = hess + 9 * (size_t)slot;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) dst[e] = src[e];
+    }
+  } else {
+    float* hout = hess + 144 * (size_t)h0;
+    for (int i = threadIdx.x; i < 144 * count; i += HINGE_WG) hout[i] = stage_h[(i / 144) * HINGE_HSTRIDE + i % 144];
+  }
   float* gout = grad + 12 * (size_t)h0;
   for (int i = threadIdx.x; i < 12 * count; i += HINGE_WG) gout[i] = stage_g[(i / 12) * HINGE_GSTRIDE + i % 12];
 }
@@ -247,8 +282,9 @@ __device__ inline Proximity proximity(const float* col, int k, const double x[3]
 __global__ void __launch_bounds__(256) k_rows(const float4* __restrict__ pos, const float4* __restrict__ xhat,
                                               const uint32_t* __restrict__ rvert, const float* __restrict__ rmass,
                                               const float* __restrict__ mu, const float* __restrict__ colliders,
-                                              int R, float* __restrict__ grad, float* __restrict__ hess) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+                                              int R, float* __restrict__ grad, float* __restrict__ hess,
+                                              const uint32_t* __restrict__ rec_pos, size_t first_record) {
+  const int row = xcd_group(gridDim.x) * blockDim.x + threadIdx.x;
   if (row >= R) return;
   const uint32_t v = rvert[row];
   const float m = rmass[row];
@@ -281,20 +317,12 @@ __global__ void __launch_bounds__(256) k_rows(const float4* __restrict__ pos, co
   }
   float* go = grad + 3 * (size_t)row;
   go[0] = g[0]; go[1] = g[1]; go[2] = g[2];
-  float* ho = hess + 9 * (size_t)row;
+  float* ho = rec_pos ? block_out(hess, rec_pos, first_record + row) : hess + 9 * (size_t)row;
+  if (!ho) return;
 #pragma unroll
   for (int i = 0; i < 9; ++i) ho[i] = h[i];
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: workgroup w runs on XCD w % 8.
-// This bijection of [0, groups) gives each XCD one contiguous run of logical workgroups, so rows that share
-// elements (a vertex and its neighbours one grid row away) gather their local blocks through the same L2.
-constexpr int XCDS = 8;
-__device__ inline int xcd_group(int groups) {
-  const int w = blockIdx.x, x = w % XCDS, k = w / XCDS;
-  const int q = groups / XCDS, r = groups % XCDS;
-  return x * q + (x < r ? x : r) + k;
-}
 
 // ---- assembly (assemble_matrix.wgsl / assemble_force.wgsl, one thread per block or row) -----
 // vals of CSR block b = sum of its local blocks in list order (mass/contact, membranes, hinges),
@@ -313,6 +341,33 @@ __global__ void __launch_bounds__(256) k_assemble_blocks(const uint32_t* __restr
 #pragma unroll
     for (int e = 0; e < 9; ++e) acc[e] = acc[e] + s[e];
   }
+  float* d = sell_val + block_dst[b];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) d[64 * e] = acc[e];
+}
+
+// The same sums from the block-sorted records: block b's row, membrane and hinge blocks are three contiguous runs
+// of their regions (kind_ptr), summed in that order, which is the list order, so the sum and its order are those
+// of k_assemble_blocks.
+__device__ inline void add_run(float* acc, const float* __restrict__ sorted, uint32_t first, uint32_t last) {
+  for (uint32_t j = first; j < last; ++j) {
+    const float* s = sorted + 9 * (size_t)j;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[e] = acc[e] + s[e];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_assemble_sorted(const uint32_t* __restrict__ kind_ptr,
+                                                         const float* __restrict__ sorted, uint32_t hinge0,
+                                                         uint32_t row0, const uint32_t* __restrict__ block_dst, int B,
+                                                         float* __restrict__ sell_val) {
+  const int b = xcd_group(gridDim.x) * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t* k0 = kind_ptr + 3 * (size_t)b;
+  add_run(acc, sorted, row0 + k0[0], row0 + k0[3]);
+  add_run(acc, sorted, k0[1], k0[4]);
+  add_run(acc, sorted, hinge0 + k0[2], hinge0 + k0[5]);
   float* d = sell_val + block_dst[b];
 #pragma unroll
   for (int e = 0; e < 9; ++e) d[64 * e] = acc[e];
@@ -689,16 +744,24 @@ int launch_set_float(Context& c, float* dst, float v) {
   return DRAPE_OK;
 }
 
-int launch_evaluate(Context& c, bool exact) {
+int launch_evaluate(Context& c, bool exact, bool reference) {
   hipStream_t s = c.stream;
-  if (c.T > 0) k_membrane<<<blocks_for(c.T), 256, 0, s>>>(c.tri, c.geom0, c.geom1, c.pos, c.material, c.T, exact, c.lgrad, c.lhess);
+  const bool sorted = !reference && c.rec_pos;
+  const uint32_t* rp = sorted ? c.rec_pos : nullptr;
+  // the reference layout places each kernel's blocks after the previous kernel's; the sorted one has slots
+  float* hh = sorted ? c.lhess : c.lhess + 81 * (size_t)c.T;
+  float* hr = sorted ? c.lhess : c.lhess + 81 * (size_t)c.T + 144 * (size_t)c.H;
+  if (c.T > 0)
+    k_membrane<<<blocks_for(c.T), 256, 0, s>>>(c.tri, c.geom0, c.geom1, c.pos, c.material, c.T, exact, c.lgrad, c.lhess,
+                                               rp);
   if (c.H > 0)
     k_hinge<<<blocks_for(c.H, HINGE_WG), HINGE_WG, 0, s>>>(c.hin, c.hstiff, c.pos, c.H, exact, c.lgrad + 9 * (size_t)c.T,
-                                                  c.lhess + 81 * (size_t)c.T);
+                                                           hh, rp, 9 * (size_t)c.T);
   k_rows<<<blocks_for(c.R), 256, 0, s>>>(c.pos, c.xhat, c.rvert, c.rmass, c.mu, c.K > 0 ? c.colliders : nullptr, c.R,
-                                         c.lgrad + 9 * (size_t)c.T + 12 * (size_t)c.H,
-                                         c.lhess + 81 * (size_t)c.T + 144 * (size_t)c.H);
+                                         c.lgrad + 9 * (size_t)c.T + 12 * (size_t)c.H, hr, rp,
+                                         9 * (size_t)c.T + 16 * (size_t)c.H);
   HIP_TRY(hipGetLastError());
+  c.hess_sorted = sorted;
   return self_contact_evaluate(c);
 }
 
@@ -706,7 +769,11 @@ int launch_assemble(Context& c) {
   hipStream_t s = c.stream;
   ++c.system_version;
   TRY_RC(two_level_before_assembly(c));
-  k_assemble_blocks<<<blocks_for(c.B), 256, 0, s>>>(c.bc_ptr, c.bc_idx, c.lhess, c.block_dst, c.B, c.sell_val);
+  if (c.hess_sorted)
+    k_assemble_sorted<<<blocks_for(c.B), 256, 0, s>>>(c.kind_ptr, c.lhess, c.hinge0, c.row0, c.block_dst, c.B,
+                                                      c.sell_val);
+  else
+    k_assemble_blocks<<<blocks_for(c.B), 256, 0, s>>>(c.bc_ptr, c.bc_idx, c.lhess, c.block_dst, c.B, c.sell_val);
   k_assemble_rhs<<<blocks_for(c.R), 256, 0, s>>>(c.fc_ptr, c.fc_idx, c.lgrad, c.R, c.b);
   HIP_TRY(hipGetLastError());
   return self_contact_assemble(c);

@@ -226,6 +226,28 @@ def test_assembled_system_matches_f64(bent_n33):
     assert norm2 == pytest.approx(float(np.sum(rhs.astype(np.float64) ** 2)), rel=1e-12)
 
 
+def test_sorted_assembly_is_the_list_order_sum_of_the_element_blocks(bent_n33):
+    """The Newton path's element kernels write every local Hessian block to its slot in the block lists' order
+    and the assembly reads each BSR block's slots contiguously; the result is, bit for bit, the FP32 sum in list
+    order of the blocks in the reference's element-local layout (drape_eval_elements), block contributions of
+    the pinned corners left out as the lists leave them."""
+    ps, od, ctx, state, x_hat = bent_n33
+    ctx.set_state(state, x_hat, 1.0 / 240.0, 0.37)
+    ctx.residual()
+    rhs, vals = ctx.system()
+    _, hess = ctx.eval_elements(state)
+    maps = ctx.contributions()
+    ptr, idx = maps["block_contrib_ptr"].astype(np.int64), maps["block_contrib_idx"].astype(np.int64)
+    blocks = hess.reshape(-1, 9)
+    want = np.zeros((len(ptr) - 1, 9), np.float32)
+    for j in range(int(np.diff(ptr).max())):
+        has = np.diff(ptr) > j
+        want[has] = want[has] + blocks[idx[ptr[:-1][has] + j]]
+    print(f"[gpu] sorted assembly n33: {len(idx)} contributions, {int((~np.isin(np.arange(len(blocks)), idx)).sum())} "
+          f"element blocks in no list", flush=True)
+    assert np.array_equal(vals.reshape(-1, 9).view(np.uint32), want.view(np.uint32))
+
+
 def test_pcg_on_assembled_system_bit_identical(bent_n33):
     """PCG on the GPU-assembled system equals the oracle's f32 emulation on the same f32 matrix, bit for bit."""
     ps, od, ctx, state, x_hat = bent_n33
