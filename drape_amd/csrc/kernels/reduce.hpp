@@ -118,6 +118,77 @@ __device__ inline float wave_reduce_partials(const float* __restrict__ part, int
   return __shfl(s, 0, 64);
 }
 
+// wave_reduce_partials / wave_reduce_partials2 in two phases, so that a kernel can issue these loads first and
+// its own before the sums need them (loads complete in issue order): start loads the first entry of each of
+// the lane's four strides, finish forms the same sums in the same order (the rest of each stride loaded there)
+// and the same tree.
+__device__ inline void partials_start(const float* __restrict__ part, int count, float raw[4]) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    const int j = l + 64 * qd;
+    raw[qd] = j < count ? part[j] : 0.0f;
+  }
+}
+
+__device__ inline float partials_finish(const float* __restrict__ part, int count, const float raw[4]) {
+  const int l = threadIdx.x & 63;
+  float v[4];
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    float a = 0.0f;
+    int j = l + 64 * qd;
+    if (j < count) {
+      a = a + raw[qd];
+      for (j += 256; j < count; j += 256) a = a + part[j];
+    }
+    v[qd] = a;
+  }
+  float s = (v[0] + v[2]) + (v[1] + v[3]);
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1) s = s + __shfl_down(s, st, 64);
+  return __shfl(s, 0, 64);
+}
+
+__device__ inline void partials2_start(const float* __restrict__ part, int count, float ra[4], float rb[4]) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    const int j = l + 64 * qd;
+    ra[qd] = j < count ? part[2 * j] : 0.0f;
+    rb[qd] = j < count ? part[2 * j + 1] : 0.0f;
+  }
+}
+
+__device__ inline float2 partials2_finish(const float* __restrict__ part, int count, const float ra[4],
+                                          const float rb[4]) {
+  const int l = threadIdx.x & 63;
+  float va[4], vb[4];
+#pragma unroll
+  for (int qd = 0; qd < 4; ++qd) {
+    float a = 0.0f, b = 0.0f;
+    int j = l + 64 * qd;
+    if (j < count) {
+      a = a + ra[qd];
+      b = b + rb[qd];
+      for (j += 256; j < count; j += 256) {
+        a = a + part[2 * j];
+        b = b + part[2 * j + 1];
+      }
+    }
+    va[qd] = a;
+    vb[qd] = b;
+  }
+  float sa = (va[0] + va[2]) + (va[1] + va[3]);
+  float sb = (vb[0] + vb[2]) + (vb[1] + vb[3]);
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1) {
+    sa = sa + __shfl_down(sa, st, 64);
+    sb = sb + __shfl_down(sb, st, 64);
+  }
+  return make_float2(__shfl(sa, 0, 64), __shfl(sb, 0, 64));
+}
+
 __device__ inline float2 wave_reduce_partials2(const float* __restrict__ part, int count) {
   const int l = threadIdx.x & 63;
   float va[4], vb[4];

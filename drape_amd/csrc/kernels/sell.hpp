@@ -33,9 +33,66 @@ struct SellView {
 // HBM-bound n1025 SpMV is unchanged.
 constexpr int CHUNK = 16;
 
-// acc[r] = acc[r] + (v[3r] p0 + v[3r+1] p1 + v[3r+2] p2) over the row's blocks in column order (the
+// One chunk of a row's blocks in registers: column, the nine values and the gathered vector entries (with the
+// fused direction update also z at the column). Every load of a chunk is issued before the first use.
+template <bool DIR>
+struct RowChunk {
+  uint32_t c[CHUNK];
+  float v[CHUNK][9];
+  float4 pc[CHUNK], zc[CHUNK];
+};
+
+template <bool DIR, bool STREAM>
+__device__ inline void chunk_load(const SellView& A, uint32_t base, int l, uint32_t n, uint32_t k0,
+                                  const float4* __restrict__ p, const float4* __restrict__ z, RowChunk<DIR>& rc) {
+#pragma unroll
+  for (int j = 0; j < CHUNK; ++j) {
+    if (k0 + j < n) {
+      const uint32_t q = base + k0 + j;
+      const float* vp = A.val + (size_t)q * 9 * 64 + l;
+      if (STREAM) {  // read once per product: keep the caches for the gathered vector
+        rc.c[j] = __builtin_nontemporal_load(A.col + q * 64 + l);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) rc.v[j][e] = __builtin_nontemporal_load(vp + 64 * e);
+      } else {
+        rc.c[j] = A.col[q * 64 + l];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) rc.v[j][e] = vp[64 * e];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CHUNK; ++j) {
+    if (k0 + j < n) {
+      rc.pc[j] = p[rc.c[j]];
+      if (DIR) rc.zc[j] = z[rc.c[j]];
+    }
+  }
+}
+
+// acc[r] = acc[r] + (v[3r] p0 + v[3r+1] p1 + v[3r+2] p2) over the chunk's blocks in column order (the
 // reference's per-row order, pcg.rs:359-369): contributions are formed independently, the sum stays
-// sequential. `dir` selects the fused direction update: the gathered value is z[c] + beta p[c].
+// sequential. With DIR the gathered value is z[c] + beta p[c].
+template <bool DIR>
+__device__ inline void chunk_add(const RowChunk<DIR>& rc, uint32_t n, uint32_t k0, float beta, float& a0, float& a1,
+                                 float& a2) {
+#pragma unroll
+  for (int j = 0; j < CHUNK; ++j) {
+    if (k0 + j < n) {
+      float4 w = rc.pc[j];
+      if (DIR) {
+        w.x = rc.zc[j].x + beta * w.x;
+        w.y = rc.zc[j].y + beta * w.y;
+        w.z = rc.zc[j].z + beta * w.z;
+      }
+      a0 = a0 + (rc.v[j][0] * w.x + rc.v[j][1] * w.y + rc.v[j][2] * w.z);
+      a1 = a1 + (rc.v[j][3] * w.x + rc.v[j][4] * w.y + rc.v[j][5] * w.z);
+      a2 = a2 + (rc.v[j][6] * w.x + rc.v[j][7] * w.y + rc.v[j][8] * w.z);
+    }
+  }
+}
+
+// The row product, chunk by chunk: one column-load and one gather latency per chunk instead of one per block.
 template <bool DIR, bool STREAM>
 __device__ inline float3 row_times_(const SellView& A, int phys, const float4* __restrict__ p,
                                     const float4* __restrict__ z, float beta) {
@@ -44,46 +101,9 @@ __device__ inline float3 row_times_(const SellView& A, int phys, const float4* _
   const uint32_t n = A.cnt[phys];
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
   for (uint32_t k0 = 0; k0 < n; k0 += CHUNK) {
-    uint32_t c[CHUNK];
-    float v[CHUNK][9];
-    float4 pc[CHUNK], zc[CHUNK];
-#pragma unroll
-    for (int j = 0; j < CHUNK; ++j) {
-      if (k0 + j < n) {
-        const uint32_t q = base + k0 + j;
-        const float* vp = A.val + (size_t)q * 9 * 64 + l;
-        if (STREAM) {  // read once per product: keep the caches for the gathered vector
-          c[j] = __builtin_nontemporal_load(A.col + q * 64 + l);
-#pragma unroll
-          for (int e = 0; e < 9; ++e) v[j][e] = __builtin_nontemporal_load(vp + 64 * e);
-        } else {
-          c[j] = A.col[q * 64 + l];
-#pragma unroll
-          for (int e = 0; e < 9; ++e) v[j][e] = vp[64 * e];
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < CHUNK; ++j) {
-      if (k0 + j < n) {
-        pc[j] = p[c[j]];
-        if (DIR) zc[j] = z[c[j]];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < CHUNK; ++j) {
-      if (k0 + j < n) {
-        float4 w = pc[j];
-        if (DIR) {
-          w.x = zc[j].x + beta * w.x;
-          w.y = zc[j].y + beta * w.y;
-          w.z = zc[j].z + beta * w.z;
-        }
-        a0 = a0 + (v[j][0] * w.x + v[j][1] * w.y + v[j][2] * w.z);
-        a1 = a1 + (v[j][3] * w.x + v[j][4] * w.y + v[j][5] * w.z);
-        a2 = a2 + (v[j][6] * w.x + v[j][7] * w.y + v[j][8] * w.z);
-      }
-    }
+    RowChunk<DIR> rc;
+    chunk_load<DIR, STREAM>(A, base, l, n, k0, p, z, rc);
+    chunk_add<DIR>(rc, n, k0, beta, a0, a1, a2);
   }
   return make_float3(a0, a1, a2);
 }

@@ -1061,9 +1061,30 @@ __global__ void __launch_bounds__(256) k_restrict_update(const uint32_t* __restr
                                                          const float4* __restrict__ q, int Nc, float* __restrict__ rc,
                                                          const float* __restrict__ partial_pq, int G, PcgState* st,
                                                          int it) {
-  if (st->status != 0) return;
+  const int entry = st->status;  // first: a launch queued past the solve's stop returns after this one load
+  float raw[4];
+  partials_start(partial_pq, G, raw);  // before the children's loads: loads complete in issue order
   const float rzv = st->rz[it & 1];
-  const float pq = wave_reduce_partials(partial_pq, G);
+  if (entry != 0) return;
+  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  // each lane's first child (a node has at most 49) is loaded before alpha is known, and the partials' tree runs
+  // while those loads are in flight
+  uint32_t t0 = 0, n = 0, s0 = 0;
+  float w0 = 0.0f;
+  float4 r0 = make_float4(0, 0, 0, 0), q0 = r0, l0 = r0;
+  if (I < Nc) {
+    t0 = child_ptr[I];
+    n = child_ptr[I + 1] - t0;
+  }
+  if ((uint32_t)lane < n) {
+    s0 = child[t0 + lane];
+    w0 = w[s0];
+    r0 = r[s0 >> 2];
+    q0 = q[s0 >> 2];
+    if (ROT) l0 = lever[s0];
+  }
+  const float pq = partials_finish(partial_pq, G, raw);  // every lane: the shuffles need the whole wavefront
   const float alpha = rzv / pq;
   if (!(pq > 0.0f && finite32(alpha))) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->status = 3;
@@ -1073,12 +1094,21 @@ __global__ void __launch_bounds__(256) k_restrict_update(const uint32_t* __restr
     st->pq = pq;
     st->alpha = alpha;
   }
-  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
   if (I >= Nc) return;
-  const uint32_t t0 = child_ptr[I], n = child_ptr[I + 1] - t0;
   float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  for (uint32_t j = lane; j < n; j += 64) {
+  if ((uint32_t)lane < n) {
+    const float rx = r0.x - alpha * q0.x, ry = r0.y - alpha * q0.y, rz = r0.z - alpha * q0.z;
+    a[0] = a[0] + w0 * rx;
+    a[1] = a[1] + w0 * ry;
+    a[2] = a[2] + w0 * rz;
+    if (ROT) {
+      const float3 c = lever_cross(l0, rx, ry, rz);
+      a[3] = a[3] + c.x;
+      a[4] = a[4] + c.y;
+      a[5] = a[5] + c.z;
+    }
+  }
+  for (uint32_t j = lane + 64; j < n; j += 64) {
     const uint32_t s = child[t0 + j];
     const float ww = w[s];
     const float4 v = r[s >> 2], qq = q[s >> 2];
@@ -1105,16 +1135,28 @@ __global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restric
                                                         const float* __restrict__ partial_pq, int G,
                                                         const PcgState* st, int it) {
   __shared__ float2 lds[256];
-  if (st->status != 0) return;
+  const int entry = st->status;  // first: a launch queued past the solve's stop returns after this one load
+  float raw[4];
+  partials_start(partial_pq, G, raw);  // before the row's loads: loads complete in issue order
+  const float rzv = st->rz[it & 1];
+  if (entry != 0) return;
   const int row = blockIdx.x * 256 + threadIdx.x;
+  // every operand that does not depend on alpha is loaded before the p.q partials are reduced, and the tree
+  // runs while the coarse solution's gathers are in flight
   float4 pr = make_float4(0, 0, 0, 0), qr = pr, xr = pr, rv = pr;
+  float d[9];
+  SlotOperands so;
   if (row < R) {
     pr = p[row];
     qr = q[row];
     xr = x[row];
     rv = r[row];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) d[e] = dinv[9 * row + e];
+    coarse_prolong_load(ppos, w, lever, yc, row, so);
   }
-  const float alpha = st->rz[it & 1] / wave_reduce_partials(partial_pq, G);
+  const float pq = partials_finish(partial_pq, G, raw);  // every lane: the shuffles need the whole wavefront
+  const float alpha = rzv / pq;
   float rzs = 0.0f, rrs = 0.0f;
   if (row < R) {
     xr.x = xr.x + alpha * pr.x;
@@ -1125,11 +1167,10 @@ __global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restric
     rv.z = rv.z - alpha * qr.z;
     x[row] = xr;
     r[row] = rv;
-    const float* d = dinv + 9 * row;
     float z0 = d[0] * rv.x + d[1] * rv.y + d[2] * rv.z;
     float z1 = d[3] * rv.x + d[4] * rv.y + d[5] * rv.z;
     float z2 = d[6] * rv.x + d[7] * rv.y + d[8] * rv.z;
-    coarse_prolong_add(ppos, w, lever, yc, row, z0, z1, z2);
+    coarse_prolong_apply(so, lever != nullptr, z0, z1, z2);
     z[row] = make_float4(z0, z1, z2, 0.0f);
     rzs = rv.x * z0 + rv.y * z1 + rv.z * z2;
     rrs = rv.x * rv.x + rv.y * rv.y + rv.z * rv.z;
