@@ -921,26 +921,31 @@ __device__ inline void split_range(const Split& p, int row, int N3p, int width, 
 }
 
 // y = T x for a triangular T stored row-major in full (zeros outside the structure): one wavefront per row
-// over the float4 groups of its range, lane sums then a fixed shuffle tree.
+// over the float4 groups of its range, lane sums then a fixed shuffle tree. A lane's loads go out GEMV_DEPTH at a
+// time; its elements are summed in index order whatever the depth, so the depth changes no result.
+constexpr int GEMV_DEPTH = 8;
 template <bool LOWER>
 __global__ void __launch_bounds__(256) k_tri_gemv(const float* __restrict__ T, const float* __restrict__ x, Split sp,
                                                   int N3p, float* __restrict__ y, const PcgState* st, int guard) {
   if (guard == 0 && st->status != 0) return;
   if (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= N3p || !split_real(sp, row)) return;
+  // longest rows first: a row of B (LOWER) grows with its index, so the last rows are dispatched first and the
+  // short ones fill in behind them (a row of B^T shrinks with its index: natural order)
+  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int row = LOWER ? N3p - 1 - slot : slot;
+  if (row < 0 || row >= N3p || !split_real(sp, row)) return;
   const float4* m = reinterpret_cast<const float4*>(T + (size_t)row * N3p);
   const float4* v = reinterpret_cast<const float4*>(x);
   int lo, hi, hs, he;
   split_range<LOWER>(sp, row, N3p, 4, &lo, &hi, &hs, &he);
   const int count = hi - lo - (he - hs);
   float acc = 0.0f;
-  // eight independent loads in flight per lane before the sequential accumulation
-  for (int v0 = lane; v0 < count; v0 += 8 * 64) {
-    float4 a[8], b[8];
+  // GEMV_DEPTH independent loads of B (and of x) in flight per lane before the sequential accumulation
+  for (int v0 = lane; v0 < count; v0 += GEMV_DEPTH * 64) {
+    float4 a[GEMV_DEPTH], b[GEMV_DEPTH];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < GEMV_DEPTH; ++u) {
       const int vv = v0 + 64 * u;
       if (vv < count) {
         const int j = lo + vv < hs ? lo + vv : lo + vv + (he - hs);
@@ -949,7 +954,7 @@ __global__ void __launch_bounds__(256) k_tri_gemv(const float* __restrict__ T, c
       }
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < GEMV_DEPTH; ++u)
       if (v0 + 64 * u < count) acc = acc + (a[u].x * b[u].x + a[u].y * b[u].y + a[u].z * b[u].z + a[u].w * b[u].w);
   }
 #pragma unroll
