@@ -124,7 +124,7 @@ POLICIES = {"truncate": {"fallback": 2, "block_jacobi_truncate": 1, "mu_carry": 
 def kernel_roofline(ctx, kernel: str, launches: int, note: str) -> dict:
     ms, nbytes = ctx.kernel_timing(kernel, launches)
     achieved = nbytes / (ms * 1e-3) / 1e9
-    unknowns = ctx.coarse_unknowns() if kernel.startswith("k_tri_gemv") else 0
+    unknowns = ctx.coarse_unknowns() if kernel.startswith(("k_tri_gemv", "k_gram")) else 0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get((kernel, ctx.columns, unknowns)),
             "kernel": kernel, "avg_kernel_us": round(1000.0 * ms, 3), "algorithmic_bytes": int(nbytes),
@@ -224,8 +224,12 @@ def main() -> None:
 
     if args.kernels_only:
         ctx, _ = leg(args.preconditioner, args.policy)
-        for k in ("k_pcg_spmv", "k_pcg_dir_spmv", "k_tri_gemv_lower", "k_tri_gemv_upper"):
-            ms, nbytes = ctx.kernel_timing(k, 400)
+        for k in ("k_pcg_spmv", "k_pcg_dir_spmv", "k_tri_gemv_lower", "k_tri_gemv_upper", "k_gram"):
+            try:
+                ms, nbytes = ctx.kernel_timing(k, 400)
+            except D.DrapeError as e:
+                print(json.dumps({"kernel": k, "skipped": str(e)}), flush=True)
+                continue
             print(json.dumps({"kernel": k, "avg_kernel_us": round(1000 * ms, 3), "algorithmic_bytes": nbytes}),
                   flush=True)
         print(json.dumps({"copy": copy_calibration()}), flush=True)
@@ -253,14 +257,23 @@ def main() -> None:
     if args.preconditioner != "block_jacobi":
         n = ctx.coarse_unknowns()
         gemv_mb = ctx.kernel_timing("k_tri_gemv_lower", 1)[1] / 1e6   # the stored triangle of B, one GEMV's bytes
-        cache = (f"n{args.columns} working set (matrix 7.7 MB, coarse inverse factor B and B^T over {n} coarse "
-                 f"unknowns, {gemv_mb:.1f} MB each in the split ordering) is resident in the 256 MB Infinity Cache")
+        try:
+            ctx.kernel_timing("k_gram", 1)
+            fused = True
+        except D.DrapeError:
+            fused = False
+        cache = (f"n{args.columns} working set (matrix 7.7 MB, coarse inverse factor B over {n} coarse unknowns, "
+                 f"{gemv_mb:.1f} MB in the split ordering{'' if fused else ', and B^T'}) is resident in the 256 MB "
+                 "Infinity Cache")
     else:
         cache = f"n{args.columns} working set (matrix 7.7 MB) is resident in the 256 MB Infinity Cache"
     kernels = [kernel_roofline(ctx, "k_pcg_dir_spmv", 400, "direction update fused into the SpMV, every "
                                "iteration after a chunk's first; " + cache),
                kernel_roofline(ctx, "k_pcg_spmv", 400, "plain SpMV, the first iteration of a chunk; " + cache)]
-    if args.preconditioner != "block_jacobi":
+    if args.preconditioner != "block_jacobi" and fused:
+        kernels += [kernel_roofline(ctx, "k_gram", 400, "fused coarse solve y_c = B^T (B r_c), one pass over B "
+                                    "(k_gram and k_gram_sum, timed together); " + cache)]
+    elif args.preconditioner != "block_jacobi":
         kernels += [kernel_roofline(ctx, "k_tri_gemv_lower", 400, "coarse solve t = B r_c; " + cache),
                     kernel_roofline(ctx, "k_tri_gemv_upper", 400, "coarse solve y_c = B^T t; " + cache)]
     ctx.close()

@@ -450,7 +450,7 @@ class Drape:
         _check(lib().drape_pcg_timing(self.h, C.byref(spmv), C.byref(solve), C.byref(n)))
         return spmv.value, solve.value
 
-    KERNELS = {"k_pcg_spmv": 0, "k_pcg_dir_spmv": 1, "k_tri_gemv_lower": 2, "k_tri_gemv_upper": 3}
+    KERNELS = {"k_pcg_spmv": 0, "k_pcg_dir_spmv": 1, "k_tri_gemv_lower": 2, "k_tri_gemv_upper": 3, "k_gram": 4}
 
     def kernel_timing(self, kernel, launches=200):
         """(average ms per launch, algorithmic bytes per launch) of one PCG kernel (drape_kernel_timing)."""

@@ -792,7 +792,7 @@ int drape_pcg_timing(drape_ctx* p, double* spmv_ms, double* solve_ms, int32_t* s
 
 int drape_kernel_timing(drape_ctx* p, int32_t kernel, int32_t launches, double* avg_ms, double* bytes) {
   Context* c = ctx_of(p);
-  if (!c || !avg_ms || !bytes || launches <= 0 || kernel < 0 || kernel > 3) {
+  if (!c || !avg_ms || !bytes || launches <= 0 || kernel < 0 || kernel > 4) {
     set_error("kernel timing: invalid argument");
     return DRAPE_E_INVALID;
   }
@@ -808,10 +808,10 @@ int drape_kernel_timing(drape_ctx* p, int32_t kernel, int32_t launches, double* 
       return DRAPE_OK;
     default:
       if (!c->two_level) {
-        set_error("kernel timing: the coarse GEMVs need the two-level preconditioner");
+        set_error("kernel timing: the coarse solve needs the two-level preconditioner");
         return DRAPE_E_INVALID;
       }
-      return two_level_gemv_timing(*c, kernel == 2, launches, avg_ms, bytes);
+      return two_level_gemv_timing(*c, kernel - 2, launches, avg_ms, bytes);
   }
 }
 

@@ -36,6 +36,8 @@
 
 namespace drape {
 
+struct GramBlock;
+
 struct TwoLevel {
   int nc = 0;        // coarse grid columns
   int Nc = 0;        // coarse rows (free coarse nodes)
@@ -93,7 +95,7 @@ struct TwoLevel {
   float4* d_Ac_lever = nullptr;     // [4R] the check entry point's levers (rotations only, allocated on demand)
   float* d_B = nullptr;             // [N3p * N3p] row-major B = L^-1 in the factor's ordering (zero above the
                                     // diagonal, between the domains and at padding)
-  float* d_BT = nullptr;            // [N3p * N3p] row-major B^T
+  float* d_BT = nullptr;            // [N3p * N3p] row-major B^T (only for the two-GEMV coarse solve)
   _Float16* d_Bh = nullptr;         // [N3p * N3p] FP16 B~ = S^-1 B, S = diag(B)
   _Float16* d_BTh = nullptr;        // [N3p * N3p] its transpose (the same values)
   bool half_valid = false;          // the FP16 copies belong to the current factorisation
@@ -131,6 +133,10 @@ struct TwoLevel {
   float* d_t = nullptr;             // [N3p] B r_c
   float* d_rc = nullptr;            // [N3p]
   float* d_yc = nullptr;            // [N3p]
+  GramBlock* d_gram = nullptr;      // [gram_blocks] row blocks of the fused coarse solve (null: two GEMVs)
+  int gram_blocks = 0;
+  int4* d_gram_cover = nullptr;     // [N3p / 4] the blocks that cover each column group (k_gram_sum)
+  float4* d_gram_part = nullptr;    // the blocks' partials of y_c
   int* d_info = nullptr;             // [2]: 64 tile + column + 1 of a failed pivot (0: none), replaced pivots
   int* h_info = nullptr;             // pinned [2]
   int64_t factorisations = 0, failures = 0;
@@ -162,7 +168,8 @@ void two_level_destroy(TwoLevel* t) {
                   (void*)t->d_pair_key, (void*)t->d_pair_terms, (void*)t->d_band, (void*)t->d_linv, (void*)t->d_lpanel,
                   (void*)t->d_border, (void*)t->d_lborder, (void*)t->d_sacc, (void*)t->d_pos, (void*)t->d_ppos, (void*)t->d_Ac, (void*)t->d_B, (void*)t->d_BT, (void*)t->d_Bh, (void*)t->d_BTh,
                   (void*)t->saved.linv, (void*)t->saved.B, (void*)t->saved.BT, (void*)t->saved.Bh, (void*)t->saved.BTh,
-                  (void*)t->d_t, (void*)t->d_rc, (void*)t->d_yc, (void*)t->d_info})
+                  (void*)t->d_t, (void*)t->d_rc, (void*)t->d_yc, (void*)t->d_info, (void*)t->d_gram,
+                  (void*)t->d_gram_part, (void*)t->d_gram_cover})
     if (p) (void)hipFree(p);
   if (t->h_info) (void)hipHostFree(t->h_info);
   delete t;
@@ -811,7 +818,7 @@ __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __re
     xo[(size_t)r * N3p + C] = out[v];
     if (C <= R && real_row(t, R) && real_row(t, C)) {
       B[(size_t)R * N3p + C] = (float)out[v];
-      BT[(size_t)C * N3p + R] = (float)out[v];
+      if (BT) BT[(size_t)C * N3p + R] = (float)out[v];
       if (Bh) {
         const double sr = linv[((size_t)I << 12) + r * 65];
         const _Float16 h = (_Float16)fmin(fmax(out[v] / sr, -65504.0), 65504.0);
@@ -1018,6 +1025,254 @@ __global__ void __launch_bounds__(256) k_tri_gemv_h(const _Float16* __restrict__
   }
 }
 
+// The coarse solve y_c = B^T (B r_c) in one pass over the stored triangle of B (FP32). Each workgroup owns a
+// block of consecutive real rows of one segment; their nonzero columns lie in one range [g0, g1) of float4
+// groups (a row R of the second domain spans groups d2 / 4 .. R / 4, any other row 0 .. R / 4). The block's row
+// slots (one wavefront each, or a pair when the range exceeds one wavefront's registers) take its rows P at a
+// time: per row, t = B_R . r_c (each lane summing its groups in increasing order, a pair adding its two
+// wavefronts' sums in order, then a fixed shuffle tree), and the slot adds t B_R to its partial of y in
+// registers from the row it still holds, so B is read once. The slots' partials are summed in slot order
+// through LDS and stored per block; k_gram_sum adds the blocks' partials per group in block order. Every sum
+// has a fixed order, so the result is reproducible.
+struct GramBlock {
+  int r0, r1;  // rows [r0, r1): real rows of one segment
+  int g0, g1;  // the float4 groups of their columns
+  int kr;      // groups per lane and row: 1, 2, 3, 4, 6 or 13
+  int wpr;     // wavefronts per row slot: 1 or 2
+  int off;     // the block's first float4 in the partials
+  int pad;
+};
+constexpr int GRAM_THREADS = 512;
+constexpr int GRAM_WAVES = GRAM_THREADS / 64;
+constexpr int GRAM_KR_MAX = 13;
+constexpr int GRAM_ROW_REGS = 13;                  // float4 registers of a lane for the rows of a chunk
+constexpr int GRAM_COMB = GRAM_WAVES * 64 * GRAM_KR_MAX;  // float4: every slot's partial of the widest block
+constexpr int GRAM_SUM_LANES = 32;                  // lanes per group in k_gram_sum
+constexpr int GRAM_SUM_PER = 16;                    // covering blocks per lane in k_gram_sum, at most
+constexpr int GRAM_MAX_BLOCKS = 1024;
+
+__device__ inline float dot4(const float4 a, const float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+// DPP steps of a wavefront sum (v + the value the DPP control `ctrl` moves in; lanes outside row_mask add 0)
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+// The sum over the 64 lanes in lane 63, in a fixed order: pairs and quads (quad_perm), rows of 16 (row_ror 4,
+// 8), then rows 0 + 1 and 2 + 3 (row_bcast 15) and (2 + 3) + (0 + 1) (row_bcast 31).
+__device__ inline float wave_sum_lane63(float v) {
+  v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+  v = dpp_add<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+  v = dpp_add<0x124, 0xF>(v);  // row_ror 4
+  v = dpp_add<0x128, 0xF>(v);  // row_ror 8
+  v = dpp_add<0x142, 0xA>(v);  // row_bcast 15 into rows 1 and 3
+  v = dpp_add<0x143, 0xC>(v);  // row_bcast 31 into rows 2 and 3
+  return v;
+}
+
+// The sums over lanes 0-31 and 32-63 in lanes 31 and 63, in a fixed order (the steps of wave_sum_lane63 up to
+// row_bcast 15)
+__device__ inline float half_wave_sum_lane31(float v) {
+  v = dpp_add<0xB1, 0xF>(v);
+  v = dpp_add<0x4E, 0xF>(v);
+  v = dpp_add<0x124, 0xF>(v);
+  v = dpp_add<0x128, 0xF>(v);
+  v = dpp_add<0x142, 0xA>(v);
+  return v;
+}
+
+// v where c holds, else zero: componentwise, so that the select stays in registers (a select of whole vectors
+// became a select of their addresses in private memory)
+__device__ inline float4 keep_if(bool c, const float4 v) {
+  return make_float4(c ? v.x : 0.0f, c ? v.y : 0.0f, c ? v.z : 0.0f, c ? v.w : 0.0f);
+}
+
+template <int KR, int WPR>
+__device__ inline void gram_block(const GramBlock& gb, const float4* __restrict__ B4, const float4* __restrict__ x4,
+                                  int n4, float4* __restrict__ part, float4* comb, float* red) {
+  constexpr int P = GRAM_ROW_REGS / KR;     // rows of a slot per chunk
+  constexpr int S = GRAM_WAVES / WPR;      // row slots
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: the row branches stay scalar
+  const int slot = wave / WPR, half = wave % WPR;
+  const int ncols = gb.g1 - gb.g0;
+  // A lane reads its groups of every row over the block's whole range: B is stored in full, so the groups past
+  // a row's end hold zeros (above the diagonal). The groups past the range (the last lanes) are read from the
+  // range's last group, and the lane's r_c there is zero, so they add nothing to t; their y is never stored.
+  // Every load is unconditional: a load under a lane condition made the compiler wait for each load before
+  // issuing the next.
+  int col[KR];
+  float4 x[KR], y[KR];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    col[k] = 64 * (WPR * k + half) + lane;
+    x[k] = x4[gb.g0 + min(col[k], ncols - 1)];
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    x[k] = keep_if(col[k] < ncols, x[k]);
+    y[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  const int chunks = (gb.r1 - gb.r0 + S * P - 1) / (S * P);
+  // the rows of chunk ch, loaded unconditionally (a row past the block's end reads its last row again, and
+  // live marks the rows that exist; wave-uniform)
+  auto load = [&](int ch, float4 (&a)[P][KR], bool (&live)[P]) {
+    const int rbase = gb.r0 + (ch * S + slot) * P;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      live[p] = rbase + p < gb.r1;
+      const float4* row = B4 + (size_t)min(rbase + p, gb.r1 - 1) * n4 + gb.g0;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) a[p][k] = row[min(col[k], ncols - 1)];
+    }
+  };
+  // t = B_R . r_c per row (a lane's groups in order, fused multiply-adds, then the DPP tree), y += t B_R
+  auto process = [&](int ch, const float4 (&a)[P][KR], const bool (&live)[P]) {
+    float t[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      float acc = 0.0f;
+      if (live[p]) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+          acc = __builtin_fmaf(a[p][k].x, x[k].x, acc);
+          acc = __builtin_fmaf(a[p][k].y, x[k].y, acc);
+          acc = __builtin_fmaf(a[p][k].z, x[k].z, acc);
+          acc = __builtin_fmaf(a[p][k].w, x[k].w, acc);
+        }
+      }
+      t[p] = wave_sum_lane63(acc);
+    }
+    if (WPR == 1) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) t[p] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t[p]), 63));
+    } else {
+      // the pair's sums through LDS, double-buffered by chunk so that one barrier per chunk orders them
+      float* rb = red + (ch & 1) * (2 * S * P);
+      if (lane == 63)
+#pragma unroll
+        for (int p = 0; p < P; ++p) rb[(slot * P + p) * 2 + half] = t[p];
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < P; ++p) t[p] = rb[(slot * P + p) * 2] + rb[(slot * P + p) * 2 + 1];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      if (live[p]) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+          y[k].x = __builtin_fmaf(t[p], a[p][k].x, y[k].x);
+          y[k].y = __builtin_fmaf(t[p], a[p][k].y, y[k].y);
+          y[k].z = __builtin_fmaf(t[p], a[p][k].z, y[k].z);
+          y[k].w = __builtin_fmaf(t[p], a[p][k].w, y[k].w);
+        }
+      }
+  };
+  // two row buffers in turn: the next chunk's rows go out before this one is reduced
+  float4 a0[P][KR], a1[P][KR];
+  bool l0[P], l1[P];
+  load(0, a0, l0);
+  for (int ch = 0; ch < chunks; ch += 2) {
+    load(ch + 1, a1, l1);
+    process(ch, a0, l0);
+    load(ch + 2, a0, l0);
+    if (ch + 1 < chunks) process(ch + 1, a1, l1);
+  }
+  // the slots' partials summed in slot order
+  const int stride = 64 * WPR * KR;
+#pragma unroll
+  for (int k = 0; k < KR; ++k)
+    if (col[k] < ncols) comb[slot * stride + col[k]] = y[k];
+  __syncthreads();
+  for (int c = threadIdx.x; c < ncols; c += GRAM_THREADS) {
+    float4 s = comb[c];
+#pragma unroll
+    for (int q = 1; q < S; ++q) {
+      const float4 v = comb[q * stride + c];
+      s.x = s.x + v.x;
+      s.y = s.y + v.y;
+      s.z = s.z + v.z;
+      s.w = s.w + v.w;
+    }
+    part[gb.off + c] = s;
+  }
+}
+
+__global__ void __launch_bounds__(GRAM_THREADS) k_gram(const GramBlock* __restrict__ blocks, const float* __restrict__ B,
+                                                      const float* __restrict__ x, int N3p, float4* __restrict__ part,
+                                                      const PcgState* st, int guard) {
+  if (guard == 0 && st->status != 0) return;
+  if (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
+  __shared__ float4 comb[GRAM_COMB];
+  __shared__ float red[2 * GRAM_WAVES * GRAM_ROW_REGS];
+  const GramBlock gb = blocks[blockIdx.x];
+  const float4* B4 = reinterpret_cast<const float4*>(B);
+  const float4* x4 = reinterpret_cast<const float4*>(x);
+  const int n4 = N3p / 4;
+  if (gb.wpr == 2) {
+    gram_block<13, 2>(gb, B4, x4, n4, part, comb, red);
+    return;
+  }
+  switch (gb.kr) {
+    case 1: gram_block<1, 1>(gb, B4, x4, n4, part, comb, red); break;
+    case 2: gram_block<2, 1>(gb, B4, x4, n4, part, comb, red); break;
+    case 3: gram_block<3, 1>(gb, B4, x4, n4, part, comb, red); break;
+    case 4: gram_block<4, 1>(gb, B4, x4, n4, part, comb, red); break;
+    case 6: gram_block<6, 1>(gb, B4, x4, n4, part, comb, red); break;
+    default: gram_block<13, 1>(gb, B4, x4, n4, part, comb, red); break;
+  }
+}
+
+// y_c = the sum of the blocks' partials, per float4 group in block order. The blocks that cover group g are the
+// blocks [a, e) of its segment whose rows reach it and then the blocks [s2, nb) (the separator's, which cover every
+// column of the domains): cover[g] = (a, e, s2), C of them. GRAM_SUM_LANES lanes per group, lane j adding the
+// covering blocks j, j + 32, ... in order, then a fixed DPP tree over the lanes. The loads are issued together
+// (a wave-uniform count, clamped indices, values of the lanes past C replaced by zero).
+__global__ void __launch_bounds__(256) k_gram_sum(const GramBlock* __restrict__ blocks, const int4* __restrict__ cover,
+                                                  int nb, const float4* __restrict__ part, int n4,
+                                                  float* __restrict__ y, const PcgState* st, int guard) {
+  if (guard == 0 && st->status != 0) return;
+  if (guard == 1 && (st->status != 0 || !(st->rr > st->above2 * st->rr0))) return;
+  __shared__ int3 tab[GRAM_MAX_BLOCKS];
+  for (int b = threadIdx.x; b < nb; b += 256) tab[b] = make_int3(blocks[b].g0, blocks[b].g1, blocks[b].off);
+  __syncthreads();
+  const int gl = blockIdx.x * (256 / GRAM_SUM_LANES) + (int)threadIdx.x / GRAM_SUM_LANES;
+  const int g = min(gl, n4 - 1);
+  const int j = threadIdx.x % GRAM_SUM_LANES;
+  const int4 cv = cover[g];
+  const int first = cv.y - cv.x, count = first + nb - cv.z;
+  // loads per lane: the larger count of the wavefront's two groups, wave-uniform
+  const int c0 = __builtin_amdgcn_readlane(count, 0), c1 = __builtin_amdgcn_readlane(count, 32);
+  const int per = (max(c0, c1) + GRAM_SUM_LANES - 1) / GRAM_SUM_LANES;
+  float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (per > 0) {
+    float4 v[GRAM_SUM_PER];
+#pragma unroll
+    for (int i = 0; i < GRAM_SUM_PER; ++i) {
+      if (i < per) {
+        const int m = min(j + GRAM_SUM_LANES * i, max(count - 1, 0));
+        const int3 e = tab[min(m < first ? cv.x + m : cv.z + m - first, nb - 1)];  // (count 0: any block)
+        v[i] = part[e.z + min(max(g - e.x, 0), e.y - e.x - 1)];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < GRAM_SUM_PER; ++i) {
+      if (i < per) {
+        const float4 u = keep_if(j + GRAM_SUM_LANES * i < count, v[i]);
+        s.x = s.x + u.x;
+        s.y = s.y + u.y;
+        s.z = s.z + u.z;
+        s.w = s.w + u.w;
+      }
+    }
+  }
+  static_assert(GRAM_SUM_LANES == 32, "the lane tree sums halves of a wavefront");
+  s.x = half_wave_sum_lane31(s.x);
+  s.y = half_wave_sum_lane31(s.y);
+  s.z = half_wave_sum_lane31(s.z);
+  s.w = half_wave_sum_lane31(s.w);
+  if (j == GRAM_SUM_LANES - 1 && gl < n4) reinterpret_cast<float4*>(y)[g] = s;
+}
+
 // z = D^-1 r + sum_k w_ik y_c[parent(i, k)], partial (r.z, r.r); mode 2 (setup) also sets p = z. ppos holds the
 // position of each parent slot's node in the factor's ordering (-1: empty slot).
 __global__ void __launch_bounds__(256) k_prolong(const int32_t* __restrict__ ppos, const float* __restrict__ w,
@@ -1192,6 +1447,76 @@ __global__ void __launch_bounds__(256) k_prolong_update(const int32_t* __restric
 // bilinear weights, children and the Galerkin term lists.
 // The coarsening factor of rules.coarse_factor; 0 picks the smallest power of two from 4 up that keeps the coarse
 // grid at most 33 vertices a side (n129: 4, n257: 8), so that the dense inverse factor stays near 43 MB.
+// The rows of the fused coarse solve in blocks of about equal cost (the groups a row reads, plus
+// GRAM_ROW_COST for its fixed work), inside each segment's real rows; false when a block's columns exceed what
+// k_gram holds in registers (then the coarse solve runs as two GEMVs).
+constexpr int GRAM_TARGET_BLOCKS = 256;
+constexpr int GRAM_ROW_COST = 32;
+static bool gram_partition(const TwoLevel* t, std::vector<GramBlock>& out, std::vector<int4>& cover, int* part4) {
+  struct Seg {
+    int first, count, lo;
+  };
+  const Seg segs[3] = {{0, t->real[0], 0},
+                       {TILE * t->seg[1], t->real[1], TILE * t->seg[1] / 4},
+                       {TILE * t->seg[2], t->real[2], 0}};
+  double total = 0.0;
+  for (const Seg& g : segs)
+    for (int R = g.first; R < g.first + g.count; ++R) total += R / 4 + 1 - g.lo + GRAM_ROW_COST;
+  const double target = total / GRAM_TARGET_BLOCKS;
+  out.clear();
+  int off = 0;
+  for (const Seg& g : segs) {
+    int start = g.first;
+    double acc = 0.0;
+    for (int R = g.first; R < g.first + g.count; ++R) {
+      acc += R / 4 + 1 - g.lo + GRAM_ROW_COST;
+      if (acc < target && R + 1 < g.first + g.count) continue;
+      GramBlock b{};
+      b.r0 = start;
+      b.r1 = R + 1;
+      b.g0 = g.lo;
+      b.g1 = R / 4 + 1;
+      const int ncols = b.g1 - b.g0;
+      b.wpr = ncols <= 64 * GRAM_KR_MAX ? 1 : 2;
+      if (ncols > 64 * b.wpr * GRAM_KR_MAX) return false;
+      const int need = (ncols + 64 * b.wpr - 1) / (64 * b.wpr);
+      b.kr = b.wpr == 2 ? GRAM_KR_MAX : (need <= 4 ? need : (need <= 6 ? 6 : GRAM_KR_MAX));
+      b.off = off;
+      off += ncols;
+      out.push_back(b);
+      start = R + 1;
+      acc = 0.0;
+    }
+  }
+  *part4 = off;
+  const int nb = (int)out.size();
+  if (nb == 0 || nb > GRAM_MAX_BLOCKS) return false;
+  // the covering blocks of each column group: a suffix [a, e) of its segment's blocks, then the separator's
+  // blocks [s2, nb) for the columns of the domains (they span every column below the separator)
+  int segb[4] = {0, 0, 0, nb};
+  for (int q = 1; q < 3; ++q) {
+    segb[q] = segb[q - 1];
+    while (segb[q] < nb && out[segb[q]].r0 < segs[q].first) ++segb[q];
+  }
+  const int n4 = t->N3p / 4;
+  cover.assign(n4, make_int4(0, 0, nb, 0));
+  for (int g = 0; g < n4; ++g) {
+    const int q = 4 * g < segs[1].first ? 0 : (4 * g < segs[2].first ? 1 : 2);
+    int a = segb[q];
+    while (a < segb[q + 1] && out[a].g1 <= g) ++a;
+    cover[g] = make_int4(a, segb[q + 1], q < 2 ? segb[2] : nb, 0);
+    int count = cover[g].y - cover[g].x + nb - cover[g].z;
+    for (int b = cover[g].x; b < cover[g].y; ++b)
+      if (g < out[b].g0 || g >= out[b].g1) return false;
+    for (int b = cover[g].z; b < nb; ++b)
+      if (g < out[b].g0 || g >= out[b].g1) return false;
+    for (int b = 0; b < nb; ++b)
+      count -= (g >= out[b].g0 && g < out[b].g1) ? 1 : 0;
+    if (count != 0 || cover[g].y - cover[g].x + nb - cover[g].z > GRAM_SUM_LANES * GRAM_SUM_PER) return false;
+  }
+  return true;
+}
+
 int coarse_factor_of(const drape_rules& r, int columns) {
   if (r.coarse_factor > 0) return r.coarse_factor;
   int f = 4;
@@ -1413,8 +1738,17 @@ int two_level_create(Context& c, int columns) {
     return fail(e);
   if ((e = hipEventCreateWithFlags(&t->ev_inverse, hipEventDisableTiming | hipEventReleaseToDevice)) != hipSuccess)
     return fail(e);
+  std::vector<GramBlock> gram;
+  std::vector<int4> gram_cover;
+  int gram_part = 0;
+  if (gram_partition(t, gram, gram_cover, &gram_part)) {
+    TL_ALLOC(t->d_gram, gram.size());
+    TL_ALLOC(t->d_gram_part, (size_t)gram_part);
+    TL_ALLOC(t->d_gram_cover, gram_cover.size());
+    t->gram_blocks = (int)gram.size();
+  }
   TL_ALLOC(t->d_B, (size_t)t->N3p * t->N3p);
-  TL_ALLOC(t->d_BT, (size_t)t->N3p * t->N3p);
+  if (t->gram_blocks == 0) TL_ALLOC(t->d_BT, (size_t)t->N3p * t->N3p);
   TL_ALLOC(t->d_Bh, (size_t)t->N3p * t->N3p);
   TL_ALLOC(t->d_BTh, (size_t)t->N3p * t->N3p);
   TL_ALLOC(t->d_t, (size_t)t->N3p);
@@ -1439,11 +1773,14 @@ int two_level_create(Context& c, int columns) {
       (e = up(t->d_pair_ptr, pair_ptr.data(), pair_ptr.size() * 4)) != hipSuccess ||
       (e = up(t->d_pair_key, pair_key.data(), pair_key.size() * 4)) != hipSuccess ||
       (e = up(t->d_pair_terms, terms.data(), terms.size() * sizeof(uint4))) != hipSuccess ||
+      (e = up(t->d_gram, gram.data(), t->gram_blocks * sizeof(GramBlock))) != hipSuccess ||
+      (e = up(t->d_gram_cover, gram_cover.data(), t->gram_blocks ? gram_cover.size() * sizeof(int4) : 0)) !=
+          hipSuccess ||
       (e = hipMemsetAsync(t->d_rc, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_yc, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_t, 0, (size_t)t->N3p * 4, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_B, 0, (size_t)t->N3p * t->N3p * 4, s)) != hipSuccess ||
-      (e = hipMemsetAsync(t->d_BT, 0, (size_t)t->N3p * t->N3p * 4, s)) != hipSuccess ||
+      (e = t->d_BT ? hipMemsetAsync(t->d_BT, 0, (size_t)t->N3p * t->N3p * 4, s) : hipSuccess) != hipSuccess ||
       (e = hipMemsetAsync(t->d_Bh, 0, (size_t)t->N3p * t->N3p * 2, s)) != hipSuccess ||
       (e = hipMemsetAsync(t->d_BTh, 0, (size_t)t->N3p * t->N3p * 2, s)) != hipSuccess ||
       (e = hipStreamSynchronize(s)) != hipSuccess)
@@ -1587,12 +1924,12 @@ static int swap_saved(Context& c, bool keep) {
     const size_t nb = (size_t)t->N3p * t->N3p;
     HIP_TRY(hipMalloc((void**)&v.linv, (size_t)t->nt * TILE * TILE * sizeof(double)));
     HIP_TRY(hipMalloc((void**)&v.B, nb * sizeof(float)));
-    HIP_TRY(hipMalloc((void**)&v.BT, nb * sizeof(float)));
+    if (t->d_BT) HIP_TRY(hipMalloc((void**)&v.BT, nb * sizeof(float)));
     HIP_TRY(hipMalloc((void**)&v.Bh, nb * sizeof(_Float16)));
     HIP_TRY(hipMalloc((void**)&v.BTh, nb * sizeof(_Float16)));
     // the copies are written below/above the diagonal only: the rest stays zero
     HIP_TRY(hipMemsetAsync(v.B, 0, nb * sizeof(float), c.stream));
-    HIP_TRY(hipMemsetAsync(v.BT, 0, nb * sizeof(float), c.stream));
+    if (v.BT) HIP_TRY(hipMemsetAsync(v.BT, 0, nb * sizeof(float), c.stream));
     HIP_TRY(hipMemsetAsync(v.Bh, 0, nb * sizeof(_Float16), c.stream));
     HIP_TRY(hipMemsetAsync(v.BTh, 0, nb * sizeof(_Float16), c.stream));
     // the prolongation travels with its factorisation; a fresh copy starts as the current one
@@ -1718,13 +2055,13 @@ static int factorise_async(Context& c) {
     if (t->dof == 6) HIP_TRY(hipMalloc((void**)&q.lever, 4 * (size_t)c.R * sizeof(float4)));
     HIP_TRY(hipMalloc((void**)&q.linv, (size_t)t->nt * TILE * TILE * sizeof(double)));
     HIP_TRY(hipMalloc((void**)&q.B, nb * sizeof(float)));
-    HIP_TRY(hipMalloc((void**)&q.BT, nb * sizeof(float)));
+    if (t->d_BT) HIP_TRY(hipMalloc((void**)&q.BT, nb * sizeof(float)));
     HIP_TRY(hipMalloc((void**)&q.Bh, nb * sizeof(_Float16)));
     HIP_TRY(hipMalloc((void**)&q.BTh, nb * sizeof(_Float16)));
     HIP_TRY(hipHostMalloc((void**)&q.h_info, 2 * sizeof(int)));
     // the copies are written below/above the diagonal only: the rest stays zero
     HIP_TRY(hipMemsetAsync(q.B, 0, nb * sizeof(float), c.stream));
-    HIP_TRY(hipMemsetAsync(q.BT, 0, nb * sizeof(float), c.stream));
+    if (q.BT) HIP_TRY(hipMemsetAsync(q.BT, 0, nb * sizeof(float), c.stream));
     HIP_TRY(hipMemsetAsync(q.Bh, 0, nb * sizeof(_Float16), c.stream));
     HIP_TRY(hipMemsetAsync(q.BTh, 0, nb * sizeof(_Float16), c.stream));
   }
@@ -2022,6 +2359,13 @@ void two_level_observe(Context& c, int iterations, double relative_residual) {
 
 // y_c = B^T B r_c: from the FP16 scaled copies when they are selected and were formed with the current
 // factorisation (and not `fp32`), else from FP32 B.
+static void launch_gram(const TwoLevel* t, hipStream_t s, const PcgState* st, int guard) {
+  k_gram<<<t->gram_blocks, GRAM_THREADS, 0, s>>>(t->d_gram, t->d_B, t->d_rc, t->N3p, t->d_gram_part, st, guard);
+  const int n4 = t->N3p / 4, per = 256 / GRAM_SUM_LANES;
+  k_gram_sum<<<(n4 + per - 1) / per, 256, 0, s>>>(t->d_gram, t->d_gram_cover, t->gram_blocks, t->d_gram_part, n4,
+                                                  t->d_yc, st, guard);
+}
+
 static void coarse_solve(Context& c, const TwoLevel* t, int guard, bool fp32) {
   hipStream_t s = c.stream;
   const int blocks = t->N3p / 4;
@@ -2029,25 +2373,39 @@ static void coarse_solve(Context& c, const TwoLevel* t, int guard, bool fp32) {
   if (t->half_valid && use_half(c, t) && !fp32) {
     k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, sp, t->N3p, t->d_t, c.state, guard);
     k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, sp, t->N3p, t->d_yc, c.state, guard);
+  } else if (t->gram_blocks > 0) {
+    launch_gram(t, s, c.state, guard);
   } else {
     k_tri_gemv<true><<<blocks, 256, 0, s>>>(t->d_B, t->d_rc, sp, t->N3p, t->d_t, c.state, guard);
     k_tri_gemv<false><<<blocks, 256, 0, s>>>(t->d_BT, t->d_t, sp, t->N3p, t->d_yc, c.state, guard);
   }
 }
 
-// One of the coarse solve's triangular GEMVs (lower: t = B r_c, else y_c = B^T t) of the current factorisation,
-// `launches` times back to back between two events on the context's stream: average ms per launch and the
-// algorithmic bytes of one launch (the triangle in its storage precision, x read and y written once).
-int two_level_gemv_timing(Context& c, bool lower, int launches, double* avg_ms, double* bytes) {
+// One of the coarse solve's triangular GEMVs (which 0: t = B r_c, 1: y_c = B^T t) of the current factorisation,
+// or (which 2) the fused y_c = B^T B r_c (k_gram and k_gram_sum), `launches` times back to back between two
+// events on the context's stream: average ms per launch and the algorithmic bytes of one launch (the triangle
+// in its storage precision, x read and y written once; the fused solve reads the triangle once as well).
+int two_level_gemv_timing(Context& c, int which, int launches, double* avg_ms, double* bytes) {
   TRY_RC(adopt_pending(c));
   TwoLevel* t = c.two_level;
   hipStream_t s = c.stream;
   const int blocks = t->N3p / 4;
   const Split sp = split_of(t);
   const bool half = t->half_valid && use_half(c, t);
+  const bool lower = which == 0;
+  if (which == 2 && (half || t->gram_blocks == 0)) {
+    set_error("kernel timing: the coarse solve of this layout runs as two GEMVs");
+    return DRAPE_E_INVALID;
+  }
+  if (which == 1 && !half && !t->d_BT) {
+    set_error("kernel timing: B^T is not stored (the coarse solve of this layout is fused)");
+    return DRAPE_E_INVALID;
+  }
   *bytes = t->tri_entries * (half ? 2.0 : 4.0) + 8.0 * t->N3;
   auto launch = [&]() {
-    if (half) {
+    if (which == 2) {
+      launch_gram(t, s, c.state, 2);
+    } else if (half) {
       if (lower) k_tri_gemv_h<true><<<blocks, 256, 0, s>>>(t->d_Bh, t->d_rc, t->d_linv, sp, t->N3p, t->d_t, c.state, 2);
       else k_tri_gemv_h<false><<<blocks, 256, 0, s>>>(t->d_BTh, t->d_t, t->d_linv, sp, t->N3p, t->d_yc, c.state, 2);
     } else {
