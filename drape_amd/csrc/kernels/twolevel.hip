@@ -529,6 +529,43 @@ __device__ inline void invert_tile(const double* L, double* X, double* Y, const 
   }
 }
 
+// Lane l's copy of lane `src`'s value of v (src wave-uniform): two v_readlane_b32.
+__device__ inline double lane_value(double v, int src) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The 16 columns c0 .. c0 + 15 of the panel on wavefront 0 (lane = row c0 + lane, p = its 16 entries) without the
+// pivot floor and the failure check, one column at a time: the pivot comes from the lane that owns it, 1/sqrt is
+// taken once for the wavefront, the next pivot is formed from two values read ahead of it, and the column's other
+// factor entries reach the rows through LDS (colw). The operations, and their order per entry, are those of the
+// checked chain when it replaces no pivot, so the results are the same bits. Each column's pivot goes to piv[] and
+// its reciprocal square root to rd[]; the caller checks the pivots afterwards and factors the panel again with the
+// checked chain when one of them would have been floored or is not positive and finite.
+__device__ inline void panel_columns_unchecked(double (&p)[16], double* colw, double* piv, double* rd) {
+  const int lane = threadIdx.x & 63;
+  double w = lane_value(p[0], 0);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double pj = w;
+    const double r = rsqrt_f64(pj);
+    piv[j] = pj;
+    rd[j] = r;
+    const double lj = p[j] * r;
+    if (j + 1 < 16) {
+      const double l1 = lane_value(p[j], j + 1) * r;  // the factor entry of row j + 1
+      w = fma(-l1, l1, lane_value(p[j + 1], j + 1));  // the next pivot, as row j + 1 forms it below
+      colw[j * TILE + lane] = lj;
+      p[j + 1] = fma(-lj, l1, p[j + 1]);
+#pragma unroll
+      for (int k = j + 2; k < 16; ++k) p[k] = fma(-lj, colw[j * TILE + k], p[k]);
+    }
+    p[j] = lj;
+  }
+}
+
 __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __restrict__ linv, Step st, int* info) {
   __shared__ double sa[TILE * LSTR];
   __shared__ double sb[TILE * LSTR];
@@ -536,6 +573,8 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
   __shared__ double sy[3 * 16 * 17];
   __shared__ double blk[16];
   __shared__ double colbuf[4 * TILE];
+  __shared__ double colw[15 * TILE];  // the unchecked chain's factor columns
+  __shared__ double pivs[16];         // the unchecked chain's pivots
   __shared__ double rdiag[TILE];
   __shared__ double adg[TILE];  // the panel's diagonal of A_c for the pivot floor (read in the pivot chain)
   __shared__ int fail;
@@ -582,6 +621,21 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
 #pragma unroll
       for (int j = 0; j < 16; ++j) p[j] = act ? Ls[row * LSTR + c0 + j] : 0.0;
       int bad = 0;
+      panel_columns_unchecked(p, colw, pivs, rdiag + c0);
+      // lane l < 16 checks column l's pivot as the checked chain would
+      bool trip = false;
+      if (lane < 16) {
+        const double pv = pivs[lane];
+        if (t.floor > 0.0) {
+          const double a = adg[c0 + lane];
+          trip = !(pv > t.floor * a) && pv > -INFINITY && a > 0.0 && a < INFINITY;
+        }
+        trip = trip || !(pv > 0.0) || !(pv < INFINITY);
+      }
+      // the checked chain, from the panel as it was, when a pivot is floored or fails
+      if (__any(trip)) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) p[j] = act ? Ls[row * LSTR + c0 + j] : 0.0;
 #pragma unroll
       for (int q = 0; q < 16; q += 4) {
         if (bad == 0) {
@@ -660,6 +714,7 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
           }
           wave_order();
         }
+      }
       }
       if (act) {
 #pragma unroll
