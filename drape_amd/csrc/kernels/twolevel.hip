@@ -89,6 +89,7 @@ struct TwoLevel {
   double* d_lborder = nullptr;      // the same tiles of L
   double* d_sacc = nullptr;         // [ts][ts][64 x 64]: the second domain's Schur update of the separator block
   double* d_xring = nullptr;        // [3][kb + 1][64][N3p]: the last tile rows of L^-1 per segment in FP64
+  double* d_ysep = nullptr;         // [ts][64][N3p]: the separator rows' border sums over the domain columns
   hipStream_t inv_stream = nullptr; // the inverse sweep, behind each panel of the factorisation chain
   hipEvent_t ev_panel = nullptr, ev_inverse = nullptr;
   double* d_Ac = nullptr;           // [N3 * N3] dense A_c for the check entry point (allocated on demand)
@@ -154,6 +155,7 @@ void two_level_destroy(TwoLevel* t) {
     if (e) (void)hipEventDestroy(e);
   if (t->inv_stream) (void)hipStreamDestroy(t->inv_stream);
   if (t->d_xring) (void)hipFree(t->d_xring);
+  if (t->d_ysep) (void)hipFree(t->d_ysep);
   for (void* p : {(void*)q.w, (void*)q.lever, (void*)q.linv, (void*)q.B, (void*)q.BT, (void*)q.Bh, (void*)q.BTh})
     if (p) (void)hipFree(p);
   if (q.h_info) (void)hipHostFree(q.h_info);
@@ -195,6 +197,8 @@ struct Tiles {
   double* border;   // [2][kb + 1][ts]: tile (separator tile a, border tile bstart[e] + j) of A_c
   double* lborder;  // the same tiles of the factor
   double* sacc;     // [ts][ts]: the second domain's Schur update of the separator block (below the diagonal)
+  double* ysep;     // [ts][64][N3p]: -sum over the border tiles K of L(I, K) X(K, J) for separator rows I and the
+                    // domain columns J, formed once the domains are factored (null: the rows sum them themselves)
   int kb, nt, ts;
   int seg[4];       // first tile of each segment (first domain, second domain, separator), seg[3] = nt
   int bstart[2];    // first border tile of each domain (= its segment's end without a separator)
@@ -802,16 +806,46 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
   }
 }
 
+constexpr int INV_COLS = 16;
+__device__ inline int inverse_first_column(const Tiles& t, int I) { return seg_of(t, I) == 1 ? TILE * t.seg[1] : 0; }
+// tile row K of X in its segment's ring
+template <typename T>
+__device__ inline T* inverse_ring(const Tiles& t, T* X, int N3p, int K) {
+  const int sk = seg_of(t, K);
+  return X + (size_t)(sk * (t.kb + 1) + (K - t.seg[sk]) % (t.kb + 1)) * TILE * N3p;
+}
+// acc[c] -= L(I, K) X(K, jb + 16 c ..), c < NB, for this wavefront's 16 rows: FP64 MFMA with the operands read
+// from L2 (A[i][k] = L(I, K)[16 bi + i][k0 + k] from lane 16 k + i, B[k][j] = X(K)[k0 + k][jb + 16 c + j] from
+// lane 16 k + j), the factor rows read once for the NB column blocks
+template <int NB>
+__device__ inline void inverse_terms(const Tiles& t, const double* X, int N3p, int I, int K, int jb, v4d (&acc)[NB]) {
+  const int lane = threadIdx.x & 63, bi = threadIdx.x >> 6;
+  const double* a = l_tile(t, I, K) + (16 * bi + (lane & 15)) * TILE + (lane >> 4);
+  const double* x = inverse_ring(t, X, N3p, K) + (size_t)(lane >> 4) * N3p + jb + (lane & 15);
+  double av[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) av[q] = -a[4 * q];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    double xv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) xv[q] = x[(size_t)4 * q * N3p + INV_COLS * c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], xv[q], acc[c], 0, 0, 0);
+  }
+}
+
 // Tile rows of B = L^-1 as a sweep over tile rows, so that row I can run as soon as the panel of tile column I
 // has produced L_II^-1 (the factor tiles L(I, K) came from earlier TRSMs): X(I, J) = L_II^-1 (E(I, J) -
 // sum_K L(I, K) X(K, J)) over the structurally nonzero X(K, J): K >= J in I's segment band, and for a separator
-// row the border tiles of J's domain. One launch serves tile row rows.x (the first w1 workgroups) and rows.y;
-// 16 columns per workgroup, over the row's nonzero columns (the second domain's rows start at its first
-// column). X keeps the last kb + 1 tile rows of each segment in FP64 ([3][kb + 1][64][N3p]); B (row-major),
-// B^T and, when Bh is not null, the FP16 copies B~ = S^-1 B (S = diag(B), from L_II^-1) and its transpose get
-// the entries between real unknowns. Wavefront w forms rows 16 w .. 16 w + 15 with FP64 MFMA (a fixed order).
-constexpr int INV_COLS = 16;
-__device__ inline int inverse_first_column(const Tiles& t, int I) { return seg_of(t, I) == 1 ? TILE * t.seg[1] : 0; }
+// row the border tiles of J's domain (summed beforehand by k_separator_border when t.ysep is set). One launch
+// serves tile row rows.x (the first w1 workgroups) and rows.y; 16 columns per workgroup, over the row's nonzero
+// columns (the second domain's rows start at its first column): short-lived workgroups, which leave the chain's
+// steps their CUs (64-column workgroups, a quarter of the factor-tile reads, held CUs long enough to delay the
+// chain by more than they saved). X keeps the last kb + 1 tile rows of each segment in FP64
+// ([3][kb + 1][64][N3p]); B (row-major), B^T and, when Bh is not null, the FP16 copies B~ = S^-1 B (S = diag(B),
+// from L_II^-1) and its transpose get the entries between real unknowns. Wavefront w forms rows 16 w .. 16 w + 15
+// with FP64 MFMA (a fixed order).
 __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __restrict__ linv, int2 rows, int w1,
                                                      int N3p, double* __restrict__ X, float* __restrict__ B,
                                                      float* __restrict__ BT, _Float16* __restrict__ Bh,
@@ -823,37 +857,25 @@ __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __re
   const int s = seg_of(t, I);
   const int lane = threadIdx.x & 63, bi = threadIdx.x >> 6;  // wavefront bi: rows 16 bi .. 16 bi + 15
   const int j0 = inverse_first_column(t, I) + INV_COLS * (second ? blockIdx.x - w1 : blockIdx.x), J = j0 / TILE;
-  auto ring = [&](int K) -> double* {
-    const int sk = seg_of(t, K);
-    return X + (size_t)(sk * (t.kb + 1) + (K - t.seg[sk]) % (t.kb + 1)) * TILE * N3p;
-  };
-  // acc = -sum_K L(I, K) X(K, j0 ..) for this wavefront's 16 rows: FP64 MFMA with the operands read from L2
-  // (A[i][k] = L(I, K)[16 bi + i][k0 + k] from lane 16 k + i, B[k][j] = X(K)[k0 + k][j0 + j] from lane 16 k + j)
-  v4d acc = {0.0, 0.0, 0.0, 0.0};
-  auto term = [&](int K) {
-    const double* a = l_tile(t, I, K) + (16 * bi + (lane & 15)) * TILE + (lane >> 4);
-    const double* x = ring(K) + (size_t)(lane >> 4) * N3p + j0 + (lane & 15);
-    double av[16], xv[16];
+  // acc = -sum_K L(I, K) X(K, j0 ..) for this wavefront's 16 rows
+  v4d acc[1] = {v4d{0.0, 0.0, 0.0, 0.0}};
+  if (s == 2 && seg_of(t, J) < 2) {
+    if (t.ysep) {
+      // the border sum k_separator_border formed, in the order the loop below would have formed it
+      const double* y = t.ysep + (size_t)(I - t.seg[2]) * TILE * N3p;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      av[q] = a[4 * q];
-      xv[q] = x[(size_t)4 * q * N3p];
+      for (int v = 0; v < 4; ++v) acc[0][v] = y[(size_t)(16 * bi + (lane >> 4) + 4 * v) * N3p + j0 + (lane & 15)];
+    } else {
+      const int sJ = seg_of(t, J);
+      for (int K = max(t.bstart[sJ], J); K < t.seg[sJ + 1]; ++K) inverse_terms(t, X, N3p, I, K, j0, acc);
     }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[q], xv[q], acc, 0, 0, 0);
-  };
-  if (s == 2) {
-    const int sJ = seg_of(t, J);
-    if (sJ < 2)
-      for (int K = t.bstart[sJ]; K < t.seg[sJ + 1]; ++K)
-        if (K >= J) term(K);
   }
-  for (int d = 1; d <= t.kb && I - d >= t.seg[s] && I - d >= J; ++d) term(I - d);
+  for (int d = 1; d <= t.kb && I - d >= t.seg[s] && I - d >= J; ++d) inverse_terms(t, X, N3p, I, I - d, j0, acc);
   // T = E(I, j0 ..) + acc in LDS, then X(I, j0 ..) = L_II^-1 T
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = 16 * bi + (lane >> 4) + 4 * v, col = j0 + (lane & 15);
-    Ts[r * 17 + (lane & 15)] = (col == TILE * I + r ? 1.0 : 0.0) + acc[v];
+    Ts[r * 17 + (lane & 15)] = (col == TILE * I + r ? 1.0 : 0.0) + acc[0][v];
   }
   __syncthreads();
   const double* li = linv + ((size_t)I << 12) + (16 * bi + (lane & 15)) * TILE + (lane >> 4);
@@ -866,7 +888,7 @@ __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __re
   v4d out = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < 16; ++q) out = __builtin_amdgcn_mfma_f64_16x16x4f64(lv[q], tv[q], out, 0, 0, 0);
-  double* xo = ring(I);
+  double* xo = inverse_ring(t, X, N3p, I);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = 16 * bi + (lane >> 4) + 4 * v, C = j0 + (lane & 15), R = TILE * I + r;
@@ -882,6 +904,30 @@ __global__ void __launch_bounds__(256) k_inverse_row(Tiles t, const double* __re
       }
     }
   }
+}
+
+// The separator rows' border sums over the domain columns, Y(I, J) = -sum_{K >= J} L(I, K) X(K, J) over the
+// border tiles K of J's domain, for every separator tile row I at once (ts * seg[2] workgroups of one 64-column
+// tile; the factor tiles are read once per tile of Y, half the time of 16-column workgroups): the terms
+// k_inverse_row would sum first for these columns, in its order, so that the rows, which must follow one another,
+// only add the separator's own terms. It runs on the inverse stream once every domain tile row
+// of L^-1 (and so every border row of X in the rings) is formed, beside the separator's steps of the chain.
+__global__ void __launch_bounds__(256) k_separator_border(Tiles t, int N3p, const double* __restrict__ X,
+                                                          const int* info) {
+  if (*info != 0) return;
+  const int I = t.seg[2] + (int)blockIdx.x / t.seg[2], J = (int)blockIdx.x % t.seg[2], jb = TILE * J;
+  const int lane = threadIdx.x & 63, bi = threadIdx.x >> 6;
+  const int sJ = seg_of(t, J);
+  constexpr int NB = TILE / INV_COLS;
+  v4d acc[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int K = max(t.bstart[sJ], J); K < t.seg[sJ + 1]; ++K) inverse_terms(t, X, N3p, I, K, jb, acc);
+  double* y = t.ysep + (size_t)(I - t.seg[2]) * TILE * N3p + jb + (lane & 15);
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[(size_t)(16 * bi + (lane >> 4) + 4 * v) * N3p + INV_COLS * c] = acc[c][v];
 }
 
 // dense A_c (both triangles, row-major, DOF unknowns per coarse node) for the check entry point
@@ -1786,6 +1832,7 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_lborder, (size_t)2 * (t->kb + 1) * std::max(t->ts, 1) * TILE * TILE);
   TL_ALLOC(t->d_sacc, (size_t)std::max(t->ts * t->ts, 1) * TILE * TILE);
   TL_ALLOC(t->d_xring, (size_t)3 * (t->kb + 1) * TILE * t->N3p);
+  if (t->ts > 0) TL_ALLOC(t->d_ysep, (size_t)t->ts * TILE * t->N3p);
   if ((e = hipStreamCreateWithFlags(&t->inv_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
   // the chain's events order the two device streams only: a device-scope release is enough, and spares each
   // panel the system-scope cache writeback that stalled the next launch of the chain
@@ -1940,6 +1987,7 @@ static Tiles tiles_of(const TwoLevel* t) {
   v.border = t->d_border;
   v.lborder = t->d_lborder;
   v.sacc = t->d_sacc;
+  v.ysep = t->d_ysep;
   v.kb = t->kb;
   v.nt = t->nt;
   v.ts = t->ts;
@@ -2318,6 +2366,12 @@ static int factorise_chain(Context& c, hipStream_t s, const float* w, const floa
   // the two domains side by side, then the separator
   TRY_RC(steps(0, 1));
   if (t->ts > 0) {
+    // every domain row of L^-1 is queued, and the event after the domains' last step (whose workers form the
+    // last border factor tiles) orders the separator rows' border sums behind them on the inverse stream
+    TRY_RC(flush_rows());
+    HIP_TRY(hipEventRecord(t->ev_panel, s));
+    HIP_TRY(hipStreamWaitEvent(t->inv_stream, t->ev_panel, 0));
+    k_separator_border<<<t->ts * t->seg[2], 256, 0, t->inv_stream>>>(tl, t->N3p, t->d_xring, t->d_info);
     k_add_separator<<<16 * t->ts * t->ts, 256, 0, s>>>(tl);
     TRY_RC(steps(2, 2));
   }
