@@ -14,10 +14,12 @@ window is PTC steps 6-25, which end near the FP32 representation floor of the n1
 Prints one JSON line.
 
 `roofline` is the fused direction update + SpMV (`k_pcg_dir_spmv`), the most-launched SpMV of the measured
-run; `rooflines` adds the plain SpMV and the two triangular GEMVs of the coarse solve. Each is timed live with
-HIP events on the context's stream over back-to-back launches on the last assembled system, against its
-algorithmic bytes per launch (SpMV 40 B per block + 28 B per row + 4; the fused kernel 24 B per row more; a
-GEMV the triangle in FP32 plus 8 B per coarse unknown). `roofline_large_grid` is the plain SpMV at n1025,
+run; `rooflines` adds the plain SpMV, the coarse solve (the fused one-pass `k_gram` + `k_gram_sum`, or the two
+triangular GEMVs where the layout keeps them) and the whole coarse factorisation. Each is timed live with HIP
+events on the context's stream over back-to-back launches on the last assembled system, against its algorithmic
+bytes per launch (SpMV 40 B per block + 28 B per row + 4; the fused kernel 24 B per row more; the coarse solve
+the stored triangle of B in FP32 plus 8 B per coarse unknown) or, for the factorisation, the FP64 flops of its
+tile Cholesky factorisation and inverse sweep against the 78.6 TFLOP/s FP64 matrix peak. `roofline_large_grid` is the plain SpMV at n1025,
 whose matrix exceeds the Infinity Cache. The CPU baseline is the oracle (f64 Newton with band Cholesky, as
 the reference's Rust oracle does) timed for one Newton iteration of the same first PTC step on one core.
 """
@@ -38,6 +40,7 @@ sys.path.insert(0, ROOT)
 import drape_amd as D  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 # HBM bytes per launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by the same counters of a
 # known-size copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0 on this part), keyed by (kernel, grid columns, coarse
 # unknowns; 0 for the SpMVs); the summaries are under profiles/. (At n129 the bytes are L2 misses served mostly
@@ -276,6 +279,19 @@ def main() -> None:
     elif args.preconditioner != "block_jacobi":
         kernels += [kernel_roofline(ctx, "k_tri_gemv_lower", 400, "coarse solve t = B r_c; " + cache),
                     kernel_roofline(ctx, "k_tri_gemv_upper", 400, "coarse solve y_c = B^T t; " + cache)]
+    if args.preconditioner != "block_jacobi":
+        # the coarse factorisation, the other part of the measured time: FP64 MFMA work, latency-bound
+        ms, flops = ctx.kernel_timing("coarse_factorisation", 10)
+        achieved = flops / (ms * 1e-3) / 1e12
+        kernels += [{"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "coarse factorisation (k_galerkin_band, the k_chol_step chain, k_inverse_row and "
+                               "k_separator_border on the inverse stream)",
+                     "avg_kernel_us": round(1000.0 * ms, 1), "algorithmic_flops": flops, "launches": 10,
+                     "note": f"one factorisation of the last assembled system over {n} coarse unknowns, HIP events "
+                             "around the whole of it; FP64 flops of the tile Cholesky factorisation and the inverse "
+                             "sweep (the Galerkin product's terms not counted); its 59 dependent chain steps make it "
+                             "latency-bound"}]
     ctx.close()
     line = {
         "metric": "ms/Newton step + PCG iters at n129: learned 2-level vs block-Jacobi PCG",

@@ -450,10 +450,12 @@ class Drape:
         _check(lib().drape_pcg_timing(self.h, C.byref(spmv), C.byref(solve), C.byref(n)))
         return spmv.value, solve.value
 
-    KERNELS = {"k_pcg_spmv": 0, "k_pcg_dir_spmv": 1, "k_tri_gemv_lower": 2, "k_tri_gemv_upper": 3, "k_gram": 4}
+    KERNELS = {"k_pcg_spmv": 0, "k_pcg_dir_spmv": 1, "k_tri_gemv_lower": 2, "k_tri_gemv_upper": 3, "k_gram": 4,
+               "coarse_factorisation": 5}
 
     def kernel_timing(self, kernel, launches=200):
-        """(average ms per launch, algorithmic bytes per launch) of one PCG kernel (drape_kernel_timing)."""
+        """(average ms per launch, algorithmic bytes per launch) of one PCG kernel (drape_kernel_timing);
+        for "coarse_factorisation", (ms per factorisation, FP64 flops of its Cholesky factorisation and inverse)."""
         ms, nbytes = C.c_double(), C.c_double()
         _check(lib().drape_kernel_timing(self.h, self.KERNELS[kernel], launches, C.byref(ms), C.byref(nbytes)))
         return ms.value, nbytes.value

@@ -792,7 +792,7 @@ int drape_pcg_timing(drape_ctx* p, double* spmv_ms, double* solve_ms, int32_t* s
 
 int drape_kernel_timing(drape_ctx* p, int32_t kernel, int32_t launches, double* avg_ms, double* bytes) {
   Context* c = ctx_of(p);
-  if (!c || !avg_ms || !bytes || launches <= 0 || kernel < 0 || kernel > 4) {
+  if (!c || !avg_ms || !bytes || launches <= 0 || kernel < 0 || kernel > 5) {
     set_error("kernel timing: invalid argument");
     return DRAPE_E_INVALID;
   }
@@ -811,6 +811,7 @@ int drape_kernel_timing(drape_ctx* p, int32_t kernel, int32_t launches, double* 
         set_error("kernel timing: the coarse solve needs the two-level preconditioner");
         return DRAPE_E_INVALID;
       }
+      if (kernel == 5) return two_level_factor_timing(*c, launches, avg_ms, bytes);
       return two_level_gemv_timing(*c, kernel - 2, launches, avg_ms, bytes);
   }
 }

@@ -272,6 +272,7 @@ int two_level_galerkin(Context& c, double* out);       // A_c of the assembled s
 int two_level_inverse_factor(Context& c, float* out);  // B = L^-1 of the current factorisation, to the host
 int two_level_ordering(Context& c, int32_t* order);    // the natural unknown at each position of the factor
 int two_level_gemv_timing(Context& c, int which, int launches, double* avg_ms, double* bytes);
+int two_level_factor_timing(Context& c, int launches, double* avg_ms, double* flops);
 int two_level_set_model(Context& c, const float* theta, int count);  // learned prolongation (NULL: none)
 // Factorises the assembled system with the current weights (the network is not run), unconditionally.
 int two_level_factor_weights(Context& c, bool* ok);

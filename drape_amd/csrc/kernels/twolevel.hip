@@ -2534,6 +2534,65 @@ int two_level_gemv_timing(Context& c, int which, int launches, double* avg_ms, d
   return DRAPE_OK;
 }
 
+// FP64 flops of the tile algorithm of one factorisation (the Galerkin product's terms not counted), with T = 64^3:
+// per tile column with m tiles below the diagonal, the diagonal factor (2/3) T, its inverse (1/3) T, m TRSMs of T
+// each and the trailing updates (m tiles of T on the diagonal, m (m - 1) / 2 of 2 T below it); per tile (I, J) of
+// L^-1 below the diagonal, 2 T for each nonzero L(I, K), J <= K < I, and 2 T for the product with L_II^-1.
+static double factor_flops(const TwoLevel* t) {
+  const double T = 64.0 * 64.0 * 64.0;
+  double f = 0.0;
+  for (int s = 0; s < 3; ++s)
+    for (int k = t->seg[s]; k < t->seg[s + 1]; ++k) {
+      int m = std::min(t->kb, t->seg[s + 1] - 1 - k);
+      if (s < 2 && t->ts > 0 && k >= t->bstart[s]) m += t->ts;
+      f += T * (2.0 / 3.0 + 1.0 / 3.0 + 2.0 * m + (double)m * (m - 1));
+    }
+  for (int s = 0; s < 3; ++s)
+    for (int I = t->seg[s]; I < t->seg[s + 1]; ++I)
+      for (int J = s == 1 ? t->seg[1] : 0; J < I; ++J) {
+        const int sJ = J < t->seg[1] ? 0 : (J < t->seg[2] ? 1 : 2);
+        int terms = std::max(0, std::min(t->kb, I - std::max(t->seg[s], J)));
+        if (s == 2 && sJ < 2) terms += std::max(0, t->seg[sJ + 1] - std::max(t->bstart[sJ], J));
+        f += 2.0 * T * (terms + 1);
+      }
+  return f;
+}
+
+// Device time of one factorisation of the current system with the current weights and levers (the Galerkin
+// product, the chain and the inverse sweep; HIP events on the context's stream, which waits for the inverse stream
+// at the end of each), averaged over `launches` back-to-back ones, and its FP64 flops (factor_flops). The factor is
+// formed again from the same inputs, so the preconditioner afterwards is that of the current system; the
+// factorisation count is not changed.
+int two_level_factor_timing(Context& c, int launches, double* avg_ms, double* flops) {
+  TRY_RC(adopt_pending(c));
+  TwoLevel* t = c.two_level;
+  if (t->factored == ~0ull) {
+    set_error("kernel timing: no coarse factorisation yet");
+    return DRAPE_E_INVALID;
+  }
+  hipStream_t s = c.stream;
+  auto once = [&]() -> int {
+    return factorise_chain(c, s, t->d_w, t->d_lever, t->d_linv, t->d_B, t->d_BT, t->d_Bh, t->d_BTh, t->half_valid,
+                           nullptr);
+  };
+  TRY_RC(once());
+  HIP_TRY(hipEventRecord(c.ev_spmv0, s));
+  for (int i = 0; i < launches; ++i) TRY_RC(once());
+  HIP_TRY(hipEventRecord(c.ev_spmv1, s));
+  HIP_TRY(hipEventSynchronize(c.ev_spmv1));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, c.ev_spmv0, c.ev_spmv1));
+  *avg_ms = ms / launches;
+  *flops = factor_flops(t);
+  TRY_RC(read_back(c, t->h_info, t->d_info, 2 * sizeof(int)));
+  t->factored = c.system_version;
+  t->factored_shift = c.shift_version;
+  t->factored_h = c.shift_h;
+  t->ok = *t->h_info == 0;
+  t->solves = 0;
+  return DRAPE_OK;
+}
+
 // z = D^-1 r + P B^T B P^T r for the current r (guard: 0 running, 1 replacement gate, 2 setup with p = z).
 static void launch_restrict(Context& c, const TwoLevel* t, const float4* r, int guard) {
   hipStream_t s = c.stream;

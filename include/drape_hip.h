@@ -440,8 +440,10 @@ int drape_pcg_timing(drape_ctx* ctx, double* spmv_ms, double* solve_ms, int32_t*
  * stream, `launches` back-to-back launches): kernel 0 the plain SpMV (k_pcg_spmv), 1 the direction update fused
  * into the SpMV (k_pcg_dir_spmv), 2 / 3 the lower / upper triangular GEMV of the two-level coarse solve
  * (k_tri_gemv, current factorisation), 4 the fused coarse solve y_c = B^T B r_c (k_gram and k_gram_sum, one pass
- * over B; the layouts whose coarse solve runs as the two GEMVs reject it). Average ms per launch and the
- * kernel's algorithmic bytes per launch.
+ * over B; the layouts whose coarse solve runs as the two GEMVs reject it), 5 one whole coarse factorisation of
+ * the current system (Galerkin product, Cholesky chain and inverse sweep; it is formed again from the same
+ * inputs). Average ms per launch and the kernel's algorithmic bytes per launch (kernel 5: the FP64 flops of the
+ * Cholesky factorisation and the inverse sweep of the tile algorithm).
  * No reference counterpart. */
 int drape_kernel_timing(drape_ctx* ctx, int32_t kernel, int32_t launches, double* avg_ms, double* bytes);
 

@@ -172,3 +172,28 @@ def test_rotations_beat_translations_on_the_n129_drape():
     it_t, st_t = out[(s17, "translations", 1e-2)]
     assert st_r == "breakdown" and it_r <= 5
     assert st_t != "breakdown" or it_r < it_t
+
+
+def test_factorisation_timing_forms_the_same_factor_again():
+    """drape_kernel_timing's kernel 5 times whole coarse factorisations of the current system (HIP events): it forms
+    the factor again from the same inputs, so B keeps its bits, the factorisation count is unchanged, and the next
+    solve takes the same iterations. Its flop count is that of the tile algorithm: at least the dense Cholesky
+    flops of one 64-wide tile column per tile, at most those of the dense N x N factorisation and inverse."""
+    arrays, x, h = c2_state()
+    ctx = system_at(arrays, x, h, ROT)
+    st0, _ = ctx.solve_once(1e-3, 500)
+    B0 = ctx.two_level_inverse_factor()
+    count = ctx.two_level_layout()[3][0]
+    ms, flops = ctx.kernel_timing("coarse_factorisation", 3)
+    B1 = ctx.two_level_inverse_factor()
+    assert ms > 0.0
+    assert np.array_equal(B0.view(np.uint32), B1.view(np.uint32))
+    assert ctx.two_level_layout()[3][0] == count
+    st1, _ = ctx.solve_once(1e-3, 500)
+    assert st1.iterations == st0.iterations
+    n = ctx.coarse_unknowns()
+    tiles = (n + 63) // 64
+    assert tiles * (2.0 / 3.0) * 64 ** 3 <= flops <= 2.0 * (64 * tiles) ** 3
+    print(f"[gpu] n65 factorisation with rotations ({n} unknowns): {1000 * ms:.1f} us, {flops / 1e9:.3f} GFLOP",
+          flush=True)
+    ctx.close()
