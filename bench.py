@@ -48,7 +48,8 @@ FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 TRAFFIC = {("k_pcg_spmv", 1025, 0): 523_831_470, ("k_pcg_spmv", 129, 0): 8_638_625,
            ("k_pcg_dir_spmv", 129, 0): 9_762_528,
            ("k_tri_gemv_lower", 129, 3267): 12_372_435, ("k_tri_gemv_upper", 129, 3267): 13_493_029,
-           ("k_tri_gemv_lower", 129, 6534): 48_591_794, ("k_tri_gemv_upper", 129, 6534): 50_698_324}
+           ("k_tri_gemv_lower", 129, 6534): 48_591_794, ("k_tri_gemv_upper", 129, 6534): 50_698_324,
+           ("k_gram", 129, 6534): 58_299_872}
 
 
 def spmv_bytes(rows: int, blocks: int) -> int:
