@@ -90,6 +90,7 @@ struct TwoLevel {
   double* d_sacc = nullptr;         // [ts][ts][64 x 64]: the second domain's Schur update of the separator block
   double* d_xring = nullptr;        // [3][kb + 1][64][N3p]: the last tile rows of L^-1 per segment in FP64
   double* d_ysep = nullptr;         // [ts][64][N3p]: the separator rows' border sums over the domain columns
+  float* d_blockval = nullptr;      // [9B]: the assembled blocks' values, contiguous, for the Galerkin product
   hipStream_t inv_stream = nullptr; // the inverse sweep, behind each panel of the factorisation chain
   hipEvent_t ev_panel = nullptr, ev_inverse = nullptr;
   double* d_Ac = nullptr;           // [N3 * N3] dense A_c for the check entry point (allocated on demand)
@@ -156,6 +157,7 @@ void two_level_destroy(TwoLevel* t) {
   if (t->inv_stream) (void)hipStreamDestroy(t->inv_stream);
   if (t->d_xring) (void)hipFree(t->d_xring);
   if (t->d_ysep) (void)hipFree(t->d_ysep);
+  if (t->d_blockval) (void)hipFree(t->d_blockval);
   for (void* p : {(void*)q.w, (void*)q.lever, (void*)q.linv, (void*)q.B, (void*)q.BT, (void*)q.Bh, (void*)q.BTh})
     if (p) (void)hipFree(p);
   if (q.h_info) (void)hipHostFree(q.h_info);
@@ -259,17 +261,19 @@ __device__ inline void tile_to_lds(const double* __restrict__ src, double* dst) 
 // One Galerkin term P_ik^T A_b P_jl of the coarse block pair: the fine block A_b (SELL planes, stride 64) between
 // the slots ik and jl, added to the DOF x DOF accumulator in FP64. With rotations the slot blocks are
 // [w I | S], S = -[v]x (coarse.hpp), so the pair block is [[wi wj A, wi A Sj], [wj Si^T A, Si^T A Sj]].
-template <int DOF>
+// a: the fine 3 x 3 block's first value, its values STRIDE floats apart (64 in the SELL-64 slots, 1 in the
+// contiguous copy k_block_values makes)
+template <int DOF, int STRIDE = 64>
 __device__ inline void galerkin_term(double* acc, const float* __restrict__ a, const float* __restrict__ w,
                                      const float4* __restrict__ lever, uint32_t si, uint32_t sj) {
   if (DOF == 3) {
     const double ww = (double)w[si] * (double)w[sj];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] += ww * (double)a[64 * q];
+    for (int q = 0; q < 9; ++q) acc[q] += ww * (double)a[STRIDE * q];
   } else {
     double A[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) A[q] = (double)a[64 * q];
+    for (int q = 0; q < 9; ++q) A[q] = (double)a[STRIDE * q];
     const double wi = w[si], wj = w[sj];
     const float4 fi = lever[si], fj = lever[sj];
     const double vi[3] = {fi.x, fi.y, fi.z}, vj[3] = {fj.x, fj.y, fj.z};
@@ -297,18 +301,33 @@ __device__ inline void galerkin_term(double* acc, const float* __restrict__ a, c
   }
 }
 
+// The assembled blocks' values in block order, 9 contiguous floats each, copied from their SELL-64 slots (where a
+// block's values lie 64 floats apart) once per factorisation: the Galerkin product reads each fine block once per
+// pair of its rows' parent slots (16 times), and reads it whole from here.
+__global__ void __launch_bounds__(256) k_block_values(const float* __restrict__ sell_val,
+                                                      const uint32_t* __restrict__ block_dst, int nblocks,
+                                                      float* __restrict__ out) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nblocks) return;
+  const float* a = sell_val + block_dst[b];
+  float v[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) v[q] = a[64 * q];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) out[9 * (size_t)b + q] = v[q];
+}
+
 // The coarse block pair (I, J) into the tiles, entries on or below the diagonal of the factor's ordering
 // (pos: the position of each coarse node's first unknown): one wavefront per pair, lane l summing the pair's
 // terms l, l + 64, ... for all DOF x DOF entries, then a fixed-order wavefront tree, so the sum is
 // reproducible. (One thread per entry walking a pair's few hundred terms in sequence was latency-bound: 248 us
-// per product at n129.)
+// per product at n129.) The fine blocks come from blockval, 9 contiguous floats each (k_block_values).
 template <int DOF>
 __global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restrict__ pair_ptr,
                                                        const uint32_t* __restrict__ pair_key,
                                                        const uint4* __restrict__ terms, const float* __restrict__ w,
                                                        const float4* __restrict__ lever,
-                                                       const float* __restrict__ sell_val,
-                                                       const uint32_t* __restrict__ block_dst, int npairs, int Nc,
+                                                       const float* __restrict__ blockval, int npairs, int Nc,
                                                        const int32_t* __restrict__ pos, Tiles tl) {
   constexpr int NE = DOF * DOF;
   const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -322,7 +341,7 @@ __global__ void __launch_bounds__(256) k_galerkin_band(const uint32_t* __restric
   const uint32_t t1 = pair_ptr[p + 1];
   for (uint32_t t = pair_ptr[p] + lane; t < t1; t += 64) {
     const uint4 e = terms[t];
-    galerkin_term<DOF>(acc, sell_val + block_dst[e.x], w, lever, e.y, e.z);
+    galerkin_term<DOF, 1>(acc, blockval + 9 * (size_t)e.x, w, lever, e.y, e.z);
   }
 #pragma unroll
   for (int q = 0; q < NE; ++q)
@@ -1833,6 +1852,7 @@ int two_level_create(Context& c, int columns) {
   TL_ALLOC(t->d_sacc, (size_t)std::max(t->ts * t->ts, 1) * TILE * TILE);
   TL_ALLOC(t->d_xring, (size_t)3 * (t->kb + 1) * TILE * t->N3p);
   if (t->ts > 0) TL_ALLOC(t->d_ysep, (size_t)t->ts * TILE * t->N3p);
+  TL_ALLOC(t->d_blockval, (size_t)9 * c.B);
   if ((e = hipStreamCreateWithFlags(&t->inv_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
   // the chain's events order the two device streams only: a device-scope release is enough, and spares each
   // panel the system-scope cache writeback that stalled the next launch of the chain
@@ -2315,14 +2335,13 @@ static int factorise_chain(Context& c, hipStream_t s, const float* w, const floa
     HIP_TRY(hipMemsetAsync(t->d_sacc, 0, (size_t)t->ts * t->ts * tile_bytes, s));
   }
   HIP_TRY(hipMemsetAsync(t->d_info, 0, 2 * sizeof(int), s));
+  k_block_values<<<blocks_for(c.B), 256, 0, s>>>(c.sell_val, c.block_dst, c.B, t->d_blockval);
   if (t->dof == 6)
     k_galerkin_band<6><<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, w,
-                                                                 lever, c.sell_val, c.block_dst, t->npairs, t->Nc,
-                                                                 t->d_pos, tl);
+                                                                 lever, t->d_blockval, t->npairs, t->Nc, t->d_pos, tl);
   else
     k_galerkin_band<3><<<blocks_for(t->npairs, 4), 256, 0, s>>>(t->d_pair_ptr, t->d_pair_key, t->d_pair_terms, w,
-                                                                 nullptr, c.sell_val, c.block_dst, t->npairs, t->Nc,
-                                                                 t->d_pos, tl);
+                                                                 nullptr, t->d_blockval, t->npairs, t->Nc, t->d_pos, tl);
   if (after_product) HIP_TRY(hipEventRecord(after_product, s));
   k_band_diagonal<<<blocks_for(t->nt * TILE), 256, 0, s>>>(tl, 1.0 + c.rules.coarse_shift);
   const int slots = t->kb + t->ts;
