@@ -763,14 +763,21 @@ __global__ void __launch_bounds__(STEP_THREADS) k_chol_step(Tiles t, double* __r
       __syncthreads();
     }
     if (wave == 0 && lane == 0 && nrep > 0) atomicAdd(info + 1, nrep);
-    invert_tile(Ls, sa, sy, rdiag);
-    // the factor back (zero above the diagonal), its inverse, and L(k, prev)
-    for (int e = tid; e < TILE * TILE; e += STEP_THREADS) {
-      const int i = e >> 6, j = e & 63;
-      tile[e] = j <= i ? Ls[i * LSTR + j] : 0.0;
+    // waves 4 .. 7, idle in the blockwise inverse (waves 0 .. 3), store the factor tile (zero above the diagonal)
+    // and L(k, prev) meanwhile; then every wave stores the inverse
+    if (wave >= 4) {
+      constexpr int HALF = STEP_THREADS / 2;
+      for (int e = tid - HALF; e < TILE * TILE; e += HALF) {
+        const int i = e >> 6, j = e & 63;
+        tile[e] = j <= i ? Ls[i * LSTR + j] : 0.0;
+      }
+      if (prev >= 0) {
+        double* lt = l_tile(t, k, prev);
+        for (int e = tid - HALF; e < TILE * TILE; e += HALF) lt[e] = sb[(e >> 6) * LSTR + (e & 63)];
+      }
     }
+    invert_tile(Ls, sa, sy, rdiag);
     lds_to_tile_step(sa, linv + ((size_t)k << 12));
-    if (prev >= 0) lds_to_tile_step(sb, l_tile(t, k, prev));
     return;
   }
   const int w = bid - st.panels;
